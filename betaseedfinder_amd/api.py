@@ -1,0 +1,164 @@
+"""ctypes binding of libbsf.so (include/bsf.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG, "lib")
+
+BSF_PROFILE_GENERAL = 0
+BSF_PROFILE_MEGA = 1
+BSF_PROFILE_RECORD60 = 2
+BSF_PROFILE_RECORD80 = 3
+PROFILE_NAMES = {"general": 0, "mega": 1, "record60": 2, "record80": 3}
+
+_LIB = None
+_INIT_DEVICE = None
+
+
+class BsfError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("libbsf error %d: %s" % (code, message))
+        self.code = code
+
+
+class IslandParams(ctypes.Structure):
+    _fields_ = [("chunk_x0", ctypes.c_int32), ("chunk_z0", ctypes.c_int32),
+                ("chunks_x", ctypes.c_int32), ("chunks_z", ctypes.c_int32),
+                ("min_blocks", ctypes.c_int32), ("min_columns", ctypes.c_int32),
+                ("min_y_for_match", ctypes.c_int32), ("min_width", ctypes.c_int32),
+                ("min_depth", ctypes.c_int32), ("min_avg_thickness", ctypes.c_double)]
+
+
+HIT_FIELDS = ["chunk_x", "chunk_z", "min_x", "max_x", "min_z", "max_z", "min_y", "max_y",
+              "blocks", "columns", "center_x", "center_z", "flags", "window_index"]
+
+
+class IslandHit(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64)] + [(n, ctypes.c_int32) for n in HIT_FIELDS]
+
+    def as_tuple(self):
+        return (self.seed,) + tuple(getattr(self, n) for n in HIT_FIELDS)
+
+
+class Stage0Counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "worlds", "regions", "p20_passed", "stage1_passed", "stage05_passed",
+        "stage075_evaluated", "stage075_passed", "stage075_rejected",
+        "mega_topology_rejected", "coarse_evaluated", "coarse_passed")]
+
+
+def lib_path():
+    return os.path.join(LIB_DIR, "libbsf.so")
+
+
+def lib():
+    """Load libbsf.so; raises if it has not been built (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libbsf.so is not built (run __graft_entry__.build() or "
+                               "python betaseedfinder_amd/build.py)")
+        L = ctypes.CDLL(path)
+        L.bsf_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _check(code):
+    if code != 0:
+        raise BsfError(code, lib().bsf_last_error().decode(errors="replace"))
+
+
+def init(device=0):
+    global _INIT_DEVICE
+    if _INIT_DEVICE != device:
+        _check(lib().bsf_init(device))
+        _INIT_DEVICE = device
+
+
+def device_banner():
+    init()
+    buf = ctypes.create_string_buffer(1024)
+    _check(lib().bsf_device_banner(buf, ctypes.c_size_t(len(buf))))
+    return buf.value.decode()
+
+
+def _seeds(seeds):
+    return np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+
+
+def density_grid(seeds, chunk_x=0, chunk_z=0):
+    """(n, 425) float64 densities, index (x*5 + z)*17 + y (generateTerrainNoise)."""
+    init()
+    s = _seeds(seeds)
+    out = np.empty((len(s), 425), dtype=np.float64)
+    _check(lib().bsf_density_grid(s.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(s)), chunk_x, chunk_z,
+                                  out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def scout_range(first_seed, count, threshold, cap=1 << 22):
+    init()
+    out = np.empty(cap, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_scout_range(ctypes.c_int64(first_seed), ctypes.c_int64(count), ctypes.c_double(threshold),
+                                 out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(cap), ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
+def island_params(chunk_x0=-2, chunk_z0=-2, chunks_x=4, chunks_z=4, min_blocks=1000, min_columns=100,
+                  min_y_for_match=60, min_width=10, min_depth=10, min_avg_thickness=2.0):
+    return IslandParams(chunk_x0, chunk_z0, chunks_x, chunks_z, min_blocks, min_columns, min_y_for_match,
+                        min_width, min_depth, min_avg_thickness)
+
+
+def island_range(first_seed, count, params, cap=1 << 16):
+    init()
+    out = (IslandHit * cap)()
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_island_range(ctypes.c_int64(first_seed), ctypes.c_int64(count), ctypes.byref(params), out,
+                                  ctypes.c_int64(cap), ctypes.byref(n)))
+    return [out[i].as_tuple() for i in range(n.value)]
+
+
+def island_seeds(seeds, params, cap=1 << 16):
+    init()
+    s = _seeds(seeds)
+    out = (IslandHit * cap)()
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_island_seeds(s.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(s)), ctypes.byref(params),
+                                  out, ctypes.c_int64(cap), ctypes.byref(n)))
+    return [out[i].as_tuple() for i in range(n.value)]
+
+
+def island_stage1_range(first_seed, count, params, cap=1 << 24):
+    init()
+    out = np.empty(cap, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_island_stage1_range(ctypes.c_int64(first_seed), ctypes.c_int64(count), ctypes.byref(params),
+                                         out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(cap), ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
+def p20_counts(seeds):
+    init()
+    s = _seeds(seeds)
+    out = np.empty(len(s), dtype=np.int32)
+    _check(lib().bsf_p20_counts(s.ctypes.data_as(ctypes.c_void_p), ctypes.c_int32(len(s)),
+                                out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def stage0_records(seeds, profile="general", research=True, centers=8):
+    init()
+    s = _seeds(seeds)
+    rb = 34 if research else 16
+    out = np.zeros(len(s) * centers * rb, dtype=np.uint8)
+    c = Stage0Counters()
+    _check(lib().bsf_stage0_records(s.ctypes.data_as(ctypes.c_void_p), ctypes.c_int32(len(s)),
+                                    PROFILE_NAMES[profile], int(research), centers,
+                                    out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(c)))
+    return out.reshape(len(s), centers, rb), c

@@ -1,0 +1,398 @@
+// bsf_device.hpp -- gfx950 device primitives for the BetaSeedFinder worker.
+//
+// One seed per lane.  Every value that reaches a hit record or a density is
+// IEEE binary64 computed in the Java evaluation order of the reference's Beta
+// 1.7.3 terrain code; the translation unit is built with -ffp-contract=off so no
+// multiply-add is ever fused.
+//
+// Permutation storage.  Each lane owns one 256-byte Perlin permutation in LDS.
+// Byte k of lane L lives at ((k >> 2) * 64 + L) * 4 + (k & 3) inside its wave's
+// 16 KiB region: the dword of any k sits in bank L mod 64, so the random gathers
+// of a 64-lane wave never conflict.  The Java table is 512 entries with
+// d[i + 256] == d[i] (NoiseGeneratorPerlin173.java:74); indexing with k & 255 is
+// the same table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bsf {
+
+constexpr uint64_t kLcgMul = 0x5DEECE66DULL;
+constexpr uint64_t kLcgAdd = 0xBULL;
+constexpr uint64_t kLcgMask = (1ULL << 48) - 1;
+
+// Division table for Random.nextInt(bound), bound in [1, 256] (filled by the host).
+// For a non-power-of-two bound d with 2^k < d < 2^(k+1): m = ceil(2^(32+k) / d) and
+// floor(n / d) == umulhi(n, m) >> k for every n < 2^31 (error term n*(m - 2^(32+k)/d)
+// / 2^(32+k) < 2^-(k+1) < 1/d).  For d = 2^k, shift holds 31 - k and m is 0.
+struct DivMagic {
+    uint32_t m;
+    uint32_t shift;
+};
+
+// ---------------------------------------------------------------------------
+// java.util.Random, 48-bit LCG (call sites NoiseGeneratorPerlin173.java:63-78,
+// NoiseGenerator2173.java:27-48; algorithm per the JDK 17 Javadoc).
+// ---------------------------------------------------------------------------
+struct Lcg {
+    uint64_t s;
+    __device__ __forceinline__ void set_seed(int64_t seed) { s = (static_cast<uint64_t>(seed) ^ kLcgMul) & kLcgMask; }
+    __device__ __forceinline__ void step() { s = (s * kLcgMul + kLcgAdd) & kLcgMask; }
+    __device__ __forceinline__ uint32_t next(int bits) {
+        step();
+        return static_cast<uint32_t>(s >> (48 - bits));
+    }
+    // nextDouble(): ((next(26) << 27) + next(27)) * 2^-53, then the caller's * 256.0.
+    // hi * 2^27 + lo < 2^53 is exact in binary64, so this equals Java's long -> double path.
+    __device__ __forceinline__ double next_double() {
+        const uint32_t hi = next(26);
+        const uint32_t lo = next(27);
+        const double combined = static_cast<double>(hi) * 134217728.0 + static_cast<double>(lo);
+        return combined * 0x1.0p-53;
+    }
+};
+
+// One nextInt(bound) draw for a wave-uniform bound, Random.nextInt(int) (JDK 17).
+__device__ __forceinline__ uint32_t next_int_uniform_bound(Lcg& r, uint32_t bound, DivMagic mg) {
+    uint32_t bits = r.next(31);
+    if ((bound & (bound - 1)) == 0) return bits >> mg.shift;  // (bound * bits) >> 31
+    uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
+    uint32_t val = bits - q * bound;
+    // Rejection when bits - val + (bound - 1) overflows int32 (probability < bound / 2^31).
+    while (static_cast<int32_t>(bits - val + (bound - 1)) < 0) {
+        bits = r.next(31);
+        q = __umulhi(bits, mg.m) >> mg.shift;
+        val = bits - q * bound;
+    }
+    return val;
+}
+
+// ---------------------------------------------------------------------------
+// Lane-private permutation in LDS.
+// ---------------------------------------------------------------------------
+struct LanePerm {
+    uint8_t* base;  // wave region base + lane * 4
+    __device__ __forceinline__ static uint32_t off(uint32_t k) { return ((k & 0xFCu) << 6) | (k & 3u); }
+    __device__ __forceinline__ int get(int k) const { return base[off(static_cast<uint32_t>(k) & 255u)]; }
+    __device__ __forceinline__ void set(int k, uint8_t v) const { base[off(static_cast<uint32_t>(k) & 255u)] = v; }
+};
+
+// A Perlin (or simplex) generator build: NoiseGeneratorPerlin173.reseed (:63-78) and
+// NoiseGenerator2173.reseed (:27-48) draw the same sequence: three nextDouble()*256
+// offsets, then a Fisher-Yates shuffle j = nextInt(256 - i) + i.
+struct GenOffsets {
+    double a, b, c;
+};
+
+__device__ __forceinline__ GenOffsets build_generator(Lcg& r, LanePerm p, const DivMagic* __restrict__ magic) {
+    GenOffsets o;
+    o.a = r.next_double() * 256.0;
+    o.b = r.next_double() * 256.0;
+    o.c = r.next_double() * 256.0;
+    // identity permutation, four bytes per dword
+    uint32_t* w = reinterpret_cast<uint32_t*>(p.base);
+#pragma unroll 8
+    for (int k = 0; k < 64; ++k) w[k * 64] = 0x03020100u + 0x04040404u * static_cast<uint32_t>(k);
+#pragma unroll 4
+    for (int i = 0; i < 256; ++i) {
+        const uint32_t bound = 256u - static_cast<uint32_t>(i);
+        const uint32_t j = next_int_uniform_bound(r, bound, magic[bound]) + static_cast<uint32_t>(i);
+        uint8_t* pi = p.base + LanePerm::off(static_cast<uint32_t>(i));
+        uint8_t* pj = p.base + LanePerm::off(j);
+        const uint8_t vi = *pi;
+        const uint8_t vj = *pj;
+        *pi = vj;
+        *pj = vi;
+    }
+    return o;
+}
+
+// Advance the stream exactly like one Perlin constructor without keeping the table
+// (NoiseGeneratorPerlin173.consumeConstructorRandom, :81-88).
+__device__ __forceinline__ void consume_generator(Lcg& r, const DivMagic* __restrict__ magic) {
+    r.step(); r.step(); r.step(); r.step(); r.step(); r.step();  // three nextDouble()
+#pragma unroll 4
+    for (int i = 0; i < 256; ++i) {
+        const uint32_t bound = 256u - static_cast<uint32_t>(i);
+        (void)next_int_uniform_bound(r, bound, magic[bound]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact scalar helpers (NoiseGeneratorPerlin173.java)
+// ---------------------------------------------------------------------------
+// Java: int i = (int) v; if (v < (double) i) --i;
+__device__ __forceinline__ int java_floor(double v) {
+    int i = static_cast<int>(v);
+    return v < static_cast<double>(i) ? i - 1 : i;
+}
+__device__ __forceinline__ double fade(double v) { return v * v * v * (v * (v * 6.0 - 15.0) + 10.0); }
+// b(t, a, b) = a + t * (b - a), :119-121
+__device__ __forceinline__ double lerp(double t, double a, double b) { return a + t * (b - a); }
+
+// a(int, x, y, z), :145-165.  Branch-free and exact: each case is (+/-u) + (+/-v) with
+// u = x for h < 8 else y, v = y for h < 4, x for h in {12,14}, else z.  Negation is exact.
+__device__ __forceinline__ double grad3(int h, double x, double y, double z) {
+    h &= 15;
+    const double u = h < 8 ? x : y;
+    const double v = h < 4 ? y : ((h == 12 || h == 14) ? x : z);
+    return ((h & 1) ? -u : u) + ((h & 2) ? -v : v);
+}
+
+// a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0.
+__device__ __forceinline__ double grad2_legacy(int h, double x, double z) {
+    const double zeroX = 0.0 * x;
+    switch (h & 15) {
+        case 0: return x + 0.0;
+        case 1: return -x + 0.0;
+        case 2: return x + -0.0;
+        case 3: return -x + -0.0;
+        case 4: return x + z;
+        case 5: return -x + z;
+        case 6: return x + -z;
+        case 7: return -x + -z;
+        case 8: return zeroX + z;
+        case 9: return -zeroX + z;
+        case 10: return zeroX + -z;
+        case 11: return -zeroX + -z;
+        case 12: return zeroX + x;
+        case 13: return -zeroX + z;
+        case 14: return zeroX + -x;
+        default: return -zeroX + -z;
+    }
+}
+
+// Axis sample: Java floor index (& 255), fraction, fraction - 1, fade.
+struct Axis {
+    int i;
+    double f, fm1, fd;
+};
+__device__ __forceinline__ Axis axis(double v) {
+    Axis a;
+    const int fl = java_floor(v);
+    a.i = fl & 255;
+    a.f = v - static_cast<double>(fl);
+    a.fm1 = a.f - 1.0;
+    a.fd = fade(a.f);
+    return a;
+}
+
+// The 2D legacy column value (yLen == 1 branch of a(...), :188-223) at an already
+// offset x / z: the y index is 0 and y never enters.
+__device__ __forceinline__ double perlin2(LanePerm p, const Axis& ax, const Axis& az) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+    const int p00 = p.get(xp0) + az.i;
+    const int p10 = p.get(xp1) + az.i;
+    const double q0 = lerp(ax.fd, grad2_legacy(p.get(p00), ax.f, az.f), grad3(p.get(p10), ax.fm1, 0.0, az.f));
+    const double q1 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, 0.0, az.fm1), grad3(p.get(p10 + 1), ax.fm1, 0.0, az.fm1));
+    return lerp(az.fd, q0, q1);
+}
+
+// ---------------------------------------------------------------------------
+// Y axis of one 3D octave over the lattice levels [Y0, Y1): the legacy bulk method
+// (:254-276) recomputes the four x-lerped corner pairs only when the masked y index
+// changes (or at y == 0) and then keeps using the *first* y of the cell's fraction.
+// Recomputing the corners at every y with the cell-start fraction is therefore the
+// same value; cell_f / cell_fm1 hold that cell-start fraction.
+// ---------------------------------------------------------------------------
+template <int Y0, int Y1>
+struct YAxis {
+    int yi[Y1 - Y0];
+    double fd[Y1 - Y0];
+    double cell_f[Y1 - Y0];
+    double cell_fm1[Y1 - Y0];
+
+    __device__ __forceinline__ void build(double sy, double b) {
+        // walk back from Y0 to the start of its cell (only levels below Y0 that share Y0's index)
+        double v0 = static_cast<double>(Y0) * sy + b;
+        int fl0 = java_floor(v0);
+        double start_f = v0 - static_cast<double>(fl0);
+        if (Y0 > 0) {
+            int prev_idx = fl0 & 255;
+            for (int y = Y0 - 1; y >= 0; --y) {
+                const double v = static_cast<double>(y) * sy + b;
+                const int fl = java_floor(v);
+                if ((fl & 255) != prev_idx) break;
+                start_f = v - static_cast<double>(fl);
+            }
+        }
+        int prev_i = fl0 & 255;
+#pragma unroll
+        for (int k = 0; k < Y1 - Y0; ++k) {
+            const int y = Y0 + k;
+            const double v = static_cast<double>(y) * sy + b;
+            const int fl = java_floor(v);
+            const int idx = fl & 255;
+            const double f = v - static_cast<double>(fl);
+            if (k > 0 && idx != prev_i) start_f = f;
+            prev_i = idx;
+            yi[k] = idx;
+            fd[k] = fade(f);
+            cell_f[k] = start_f;
+            cell_fm1[k] = start_f - 1.0;
+        }
+    }
+};
+
+// One 3D octave at one lattice column, all levels [Y0, Y1): out[k] = value * weight
+// (the per-point addend of :281-282).
+template <int Y0, int Y1>
+__device__ __forceinline__ void perlin3_column(LanePerm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
+                                               double weight, double* out) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+#pragma unroll
+    for (int k = 0; k < Y1 - Y0; ++k) {
+        const int yi = ya.yi[k];
+        const double yf = ya.cell_f[k];
+        const double yfm1 = ya.cell_fm1[k];
+        const int p0 = xp0 + yi;
+        const int p00 = p.get(p0) + az.i;
+        const int p01 = p.get(p0 + 1) + az.i;
+        const int p1 = xp1 + yi;
+        const int p10 = p.get(p1) + az.i;
+        const int p11 = p.get(p1 + 1) + az.i;
+        const double q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
+        const double q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
+        const double q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
+        const double q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        const double r0 = lerp(ya.fd[k], q00, q01);
+        const double r1 = lerp(ya.fd[k], q10, q11);
+        out[k] = lerp(az.fd, r0, r1) * weight;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D simplex (NoiseGenerator2173.aStrided, :117-172) at already-offset coordinates.
+// ---------------------------------------------------------------------------
+// Java: 0.5*(Math.sqrt(3)-1) and (3-Math.sqrt(3))/6, correctly rounded (checked on the host).
+constexpr double kSimplexF = 0.36602540378443860;
+constexpr double kSimplexG = 0.21132486540518713;
+
+__device__ __forceinline__ int simplex_floor(double v) {
+    const int t = static_cast<int>(v);
+    return v > 0.0 ? t : t - 1;
+}
+__device__ __forceinline__ double simplex_grad(int g, double x, double z) {
+    switch (g) {
+        case 0: return 1.0 * x + 1.0 * z;
+        case 1: return -1.0 * x + 1.0 * z;
+        case 2: return 1.0 * x + -1.0 * z;
+        case 3: return -1.0 * x + -1.0 * z;
+        case 4:
+        case 6: return 1.0 * x + 0.0 * z;
+        case 5:
+        case 7: return -1.0 * x + 0.0 * z;
+        case 8:
+        case 10: return 0.0 * x + 1.0 * z;
+        default: return 0.0 * x + -1.0 * z;
+    }
+}
+__device__ __forceinline__ double simplex2(LanePerm p, double d5, double d6) {
+    const double f = kSimplexF, g = kSimplexG;
+    const double d7 = (d5 + d6) * f;
+    const int j1 = simplex_floor(d5 + d7);
+    const int k1 = simplex_floor(d6 + d7);
+    const double d8 = static_cast<double>(j1 + k1) * g;
+    const double d9 = static_cast<double>(j1) - d8;
+    const double d10 = static_cast<double>(k1) - d8;
+    const double d11 = d5 - d9;
+    const double d12 = d6 - d10;
+    const int b0 = d11 > d12 ? 1 : 0;
+    const int b1 = 1 - b0;
+    const double d13 = d11 - static_cast<double>(b0) + g;
+    const double d14 = d12 - static_cast<double>(b1) + g;
+    const double d15 = d11 - 1.0 + 2.0 * g;
+    const double d16 = d12 - 1.0 + 2.0 * g;
+    const int l1 = j1 & 255;
+    const int i2 = k1 & 255;
+    const int j2 = p.get(l1 + p.get(i2)) % 12;
+    const int k2 = p.get(l1 + b0 + p.get(i2 + b1)) % 12;
+    const int l2 = p.get(l1 + 1 + p.get(i2 + 1)) % 12;
+    double d17 = 0.5 - d11 * d11 - d12 * d12;
+    double d18 = 0.0;
+    if (!(d17 < 0.0)) { d17 *= d17; d18 = d17 * d17 * simplex_grad(j2, d11, d12); }
+    double d19 = 0.5 - d13 * d13 - d14 * d14;
+    double d20 = 0.0;
+    if (!(d19 < 0.0)) { d19 *= d19; d20 = d19 * d19 * simplex_grad(k2, d13, d14); }
+    double d21 = 0.5 - d15 * d15 - d16 * d16;
+    double d22 = 0.0;
+    if (!(d21 < 0.0)) { d21 *= d21; d22 = d21 * d21 * simplex_grad(l2, d15, d16); }
+    return 70.0 * (d18 + d20 + d22);
+}
+
+// WorldChunkManager173 climate post-processing of one sample (:100-127).
+__device__ __forceinline__ void climate_finish(double tRaw, double rRaw, double cRaw, double& temp, double& rain) {
+    const double d0 = cRaw * 1.1 + 0.5;
+    double d1 = 0.01;
+    double d2 = 1.0 - d1;
+    double d3 = (tRaw * 0.15 + 0.7) * d2 + d0 * d1;
+    d1 = 0.0020;
+    d2 = 1.0 - d1;
+    double d4 = (rRaw * 0.15 + 0.5) * d2 + d0 * d1;
+    d3 = 1.0 - (1.0 - d3) * (1.0 - d3);
+    if (d3 < 0.0) d3 = 0.0;
+    if (d4 < 0.0) d4 = 0.0;
+    if (d3 > 1.0) d3 = 1.0;
+    if (d4 > 1.0) d4 = 1.0;
+    temp = d3;
+    rain = d4;
+}
+
+// generateTerrainNoise column shape (BetaTerrain173.java:741-790) for yLen = 17.
+struct ColumnShape {
+    double d5, d7;
+};
+__device__ __forceinline__ ColumnShape column_shape(double temp, double rain, double n4, double n5) {
+    const double d3 = rain * temp;
+    double d4 = 1.0 - d3;
+    d4 *= d4;
+    d4 *= d4;
+    d4 = 1.0 - d4;
+    double d5 = (n4 + 256.0) / 512.0;
+    d5 *= d4;
+    if (d5 > 1.0) d5 = 1.0;
+    double d6 = n5 / 8000.0;
+    if (d6 < 0.0) d6 = -d6 * 0.3;
+    d6 = d6 * 3.0 - 2.0;
+    if (d6 < 0.0) {
+        d6 /= 2.0;
+        if (d6 < -1.0) d6 = -1.0;
+        d6 /= 1.4;
+        d6 /= 2.0;
+        d5 = 0.0;
+    } else {
+        if (d6 > 1.0) d6 = 1.0;
+        d6 /= 8.0;
+    }
+    if (d5 < 0.0) d5 = 0.0;
+    d5 += 0.5;
+    d6 = d6 * 17.0 / 16.0;
+    ColumnShape s;
+    s.d5 = d5;
+    s.d7 = 17.0 / 2.0 + d6 * 4.0;
+    return s;
+}
+
+// One lattice density (BetaTerrain173.java:792-829) at level y of a 17-level column.
+__device__ __forceinline__ double lattice_density(const ColumnShape& cs, int y, double n1, double n2, double n3) {
+    double d9 = (static_cast<double>(y) - cs.d7) * 12.0 / cs.d5;
+    if (d9 < 0.0) d9 *= 4.0;
+    const double d10 = n2 / 512.0;
+    const double d11 = n3 / 512.0;
+    const double d12 = (n1 / 10.0 + 1.0) / 2.0;
+    double d8;
+    if (d12 < 0.0) d8 = d10;
+    else if (d12 > 1.0) d8 = d11;
+    else d8 = d10 + (d11 - d10) * d12;
+    d8 -= d9;
+    if (y > 13) {
+        const double d13 = static_cast<double>(static_cast<float>(y - 13) / 3.0F);
+        d8 = d8 * (1.0 - d13) + -10.0 * d13;
+    }
+    return d8;
+}
+
+}  // namespace bsf

@@ -1,0 +1,67 @@
+// bsf_host.hpp -- host-side context and helpers of libbsf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bsf_kernels.hpp"
+
+namespace bsf {
+
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what);
+void set_error(const std::string& m);
+
+// Grow-only device allocation.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t size = 0;
+    void* ensure(size_t bytes);
+    void release();
+};
+
+struct HostColumns {
+    std::vector<int2> noise;
+    std::vector<int4> dens;
+    ColumnSet dev{};
+    void upload(DeviceBuffer& nbuf, DeviceBuffer& dbuf, hipStream_t s);
+};
+
+struct Context {
+    bool initialised = false;
+    int device = 0;
+    int cu_count = 0;
+    std::string name, arch;
+    size_t global_mem = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer seeds, out, scratch, scratch2, survivors, counters, lattice, hits, small;
+    DeviceBuffer cols_noise, cols_dens, cols_noise2, cols_dens2;
+    DeviceBuffer s0_a, s0_b, s0_c, s0_d, s0_e, s0_f, s0_g, s0_h;
+    void release_all() {
+        for (DeviceBuffer* b : {&seeds, &out, &scratch, &scratch2, &survivors, &counters, &lattice, &hits, &small,
+                                &cols_noise, &cols_dens, &cols_noise2, &cols_dens2, &s0_a, &s0_b, &s0_c, &s0_d, &s0_e,
+                                &s0_f, &s0_g, &s0_h})
+            b->release();
+    }
+};
+
+extern Context g_ctx;
+
+void require_init();
+
+template <class K>
+int resident_blocks(K kernel, int block, size_t dyn_lds);
+
+}  // namespace bsf

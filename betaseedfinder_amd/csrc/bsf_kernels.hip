@@ -1,0 +1,552 @@
+// bsf_kernels.hip -- terrain lattice, Stage-0 scout and island-flood kernels (gfx950).
+//
+// Lattice kernels run one seed per lane with 64-thread blocks: each lane builds every
+// Perlin generator of its seed in order (Random stream of BetaTerrain173.java:84-92) into
+// its own LDS permutation and accumulates the octave sums of the requested lattice
+// columns in a lane-interleaved global scratch slot (one slot per resident block, the
+// grid is persistent).  Island floods run one 256-thread workgroup per Stage-1 survivor
+// on LDS bitsets.
+#include "bsf_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace bsf {
+
+__constant__ DivMagic c_magic[257];
+
+hipError_t upload_tables() {
+    DivMagic t[257] = {};
+    for (uint32_t d = 1; d <= 256; ++d) {
+        uint32_t k = 0;
+        while ((2u << k) <= d) ++k;  // 2^k <= d < 2^(k+1)
+        if ((d & (d - 1)) == 0) {
+            t[d].m = 0;
+            t[d].shift = 31 - k;
+        } else {
+            const unsigned long long num = 1ULL << (32 + k);
+            t[d].m = static_cast<uint32_t>((num + d - 1) / d);
+            t[d].shift = k;
+        }
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_magic), t, sizeof(t));
+}
+
+// ---------------------------------------------------------------------------
+// generic lattice pass
+// ---------------------------------------------------------------------------
+struct LaneCtx {
+    int lane;
+    LanePerm perm;
+    double* slot;  // scratch base of this block; element e of this lane at slot[e*64 + lane]
+    __device__ __forceinline__ double& at(int e) const { return slot[(static_cast<size_t>(e) << 6) + lane]; }
+};
+
+// 3D octave group (NoiseGeneratorOctaves173.generateNoise, :46-83 with the legacy bulk
+// method): kind-major accumulator block acc0 .. acc0 + nc*NY.
+template <int Y0, int Y1>
+__device__ __forceinline__ void octaves3d(const LaneCtx& L, Lcg& r, int octaves, double s_xz, double s_y,
+                                          const int2* __restrict__ cols, int nc, int acc0) {
+    constexpr int NY = Y1 - Y0;
+    double amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        const GenOffsets g = build_generator(r, L.perm, c_magic);
+        const double sx = s_xz * amp;
+        const double sy = s_y * amp;
+        const double w = 1.0 / amp;
+        YAxis<Y0, Y1> ya;
+        ya.build(sy, g.b);
+        for (int c = 0; c < nc; ++c) {
+            const int2 xz = cols[c];
+            const Axis ax = axis(static_cast<double>(xz.x) * sx + g.a);
+            const Axis az = axis(static_cast<double>(xz.y) * sx + g.c);
+            double v[NY];
+            perlin3_column<Y0, Y1>(L.perm, ax, az, ya, w, v);
+            const int e0 = acc0 + c * NY;
+            if (o == 0) {
+#pragma unroll
+                for (int k = 0; k < NY; ++k) L.at(e0 + k) = 0.0 + v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < NY; ++k) L.at(e0 + k) += v[k];
+            }
+        }
+        amp /= 2.0;
+    }
+}
+
+// 2D octave group (the yLen == 1 generateNoise overload, NoiseGeneratorOctaves173.java:85-90).
+__device__ __forceinline__ void octaves2d(const LaneCtx& L, Lcg& r, int octaves, double s_xz,
+                                          const int2* __restrict__ cols, int nc, int acc0) {
+    double amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        const GenOffsets g = build_generator(r, L.perm, c_magic);
+        const double sx = s_xz * amp;
+        const double w = 1.0 / amp;
+        for (int c = 0; c < nc; ++c) {
+            const int2 xz = cols[c];
+            const Axis ax = axis(static_cast<double>(xz.x) * sx + g.a);
+            const Axis az = axis(static_cast<double>(xz.y) * sx + g.c);
+            const double v = perlin2(L.perm, ax, az) * w;
+            if (o == 0) L.at(acc0 + c) = 0.0 + v;
+            else L.at(acc0 + c) += v;
+        }
+        amp /= 2.0;
+    }
+}
+
+// Climate simplex group (NoiseGeneratorOctaves2173.a, :29-51) at block coordinates.
+__device__ __forceinline__ void climate_group(const LaneCtx& L, int64_t rng_seed, int octaves, double start_scale,
+                                              double octave_scale, const int4* __restrict__ dens, int nd, int acc0) {
+    Lcg r;
+    r.set_seed(rng_seed);
+    const double d2 = start_scale / 1.5;
+    double d6 = 1.0, d7 = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        const GenOffsets g = build_generator(r, L.perm, c_magic);
+        const double sc = d2 * d7;
+        const double w = 0.55 / d6;
+        for (int d = 0; d < nd; ++d) {
+            const int4 dc = dens[d];
+            const double x = static_cast<double>(dc.y) * sc + g.a;
+            const double z = static_cast<double>(dc.z) * sc + g.b;
+            const double v = simplex2(L.perm, x, z) * w;
+            if (o == 0) L.at(acc0 + d) = 0.0 + v;
+            else L.at(acc0 + d) += v;
+        }
+        d7 *= octave_scale;
+        d6 *= 0.5;
+    }
+}
+
+// Scratch element map of one lane for a column set.
+template <int Y0, int Y1>
+struct ScratchMap {
+    static constexpr int NY = Y1 - Y0;
+    int nc, nd;
+    __device__ __forceinline__ int n1(int c, int k) const { return c * NY + k; }
+    __device__ __forceinline__ int n2(int c, int k) const { return (nc + c) * NY + k; }
+    __device__ __forceinline__ int n3(int c, int k) const { return (2 * nc + c) * NY + k; }
+    __device__ __forceinline__ int n4(int c) const { return 3 * nc * NY + c; }
+    __device__ __forceinline__ int n5(int c) const { return 3 * nc * NY + nc + c; }
+    __device__ __forceinline__ int clim(int t, int d) const { return 3 * nc * NY + 2 * nc + t * nd + d; }
+    __host__ __device__ static int total(int nc_, int nd_) { return 3 * nc_ * NY + 2 * nc_ + 3 * nd_; }
+};
+
+// Full generator sequence of one seed (BetaTerrain173 constructor order, :84-92, and the
+// climate Random seeds of WorldChunkManager173.java:25-30) into the lane's scratch.
+template <int Y0, int Y1>
+__device__ void evaluate_columns(const LaneCtx& L, int64_t seed, const ColumnSet& cs) {
+    ScratchMap<Y0, Y1> m{cs.nc, cs.nd};
+    Lcg r;
+    r.set_seed(seed);
+    octaves3d<Y0, Y1>(L, r, 16, 684.412, 684.412, cs.noise_xz, cs.nc, m.n2(0, 0));
+    octaves3d<Y0, Y1>(L, r, 16, 684.412, 684.412, cs.noise_xz, cs.nc, m.n3(0, 0));
+    octaves3d<Y0, Y1>(L, r, 8, 684.412 / 80.0, 684.412 / 160.0, cs.noise_xz, cs.nc, m.n1(0, 0));
+    for (int i = 0; i < 8; ++i) consume_generator(r, c_magic);
+    octaves2d(L, r, 10, 1.121, cs.noise_xz, cs.nc, m.n4(0));
+    octaves2d(L, r, 16, 200.0, cs.noise_xz, cs.nc, m.n5(0));
+    const uint64_t us = static_cast<uint64_t>(seed);
+    climate_group(L, static_cast<int64_t>(us * 9871ULL), 4, 0.02500000037252903, 0.25, cs.dens, cs.nd, m.clim(0, 0));
+    climate_group(L, static_cast<int64_t>(us * 39811ULL), 4, 0.05000000074505806, 0.3333333333333333, cs.dens,
+                  cs.nd, m.clim(1, 0));
+    climate_group(L, static_cast<int64_t>(us * 543321ULL), 2, 0.25, 0.5882352941176471, cs.dens, cs.nd,
+                  m.clim(2, 0));
+}
+
+template <int Y0, int Y1>
+__device__ __forceinline__ ColumnShape density_shape(const LaneCtx& L, const ScratchMap<Y0, Y1>& m, int d, int nc_of_d) {
+    double temp, rain;
+    climate_finish(L.at(m.clim(0, d)), L.at(m.clim(1, d)), L.at(m.clim(2, d)), temp, rain);
+    return column_shape(temp, rain, L.at(m.n4(nc_of_d)), L.at(m.n5(nc_of_d)));
+}
+
+__device__ __forceinline__ int64_t seed_of(const LatticeArgs& a, int64_t i) {
+    return a.seeds != nullptr ? a.seeds[i] : a.first_seed + i;
+}
+
+__device__ __forceinline__ void append_seed(bool pass, int64_t seed, int64_t* out, unsigned long long* count,
+                                            int64_t cap) {
+    const unsigned long long mask = __ballot(pass);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (pass) {
+        const unsigned long long rank = __popcll(mask & ((1ULL << lane) - 1ULL));
+        if (static_cast<int64_t>(base + rank) < cap) out[base + rank] = seed;
+    }
+}
+
+// MODE_DENSITY: densities at all 17 levels, out[i*nd*17 + d*17 + y].
+// MODE_CANDIDATE: island Stage 1, survivors appended to out_seeds.
+// MODE_LATTICE: levels 8..15 per density column, out[i*nd*8 + d*8 + k].
+template <int Y0, int Y1, int MODE>
+__global__ __launch_bounds__(64) void lattice_kernel(LatticeArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    const int lane = threadIdx.x;
+    LaneCtx L;
+    L.lane = lane;
+    L.perm.base = perm_lds + lane * 4;
+    const int vals = ScratchMap<Y0, Y1>::total(a.cols.nc, a.cols.nd);
+    L.slot = a.scratch + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(vals) * 64;
+    int64_t count = a.count;
+    if (a.d_count != nullptr) {
+        const int64_t dc = static_cast<int64_t>(*a.d_count);
+        count = dc < a.count ? dc : a.count;
+    }
+    const int64_t groups = (count + 63) >> 6;
+    ScratchMap<Y0, Y1> m{a.cols.nc, a.cols.nd};
+    for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const int64_t i = (grp << 6) + lane;
+        const bool active = i < count;
+        const int64_t seed = seed_of(a, active ? i : count - 1);
+        evaluate_columns<Y0, Y1>(L, seed, a.cols);
+        if (MODE == MODE_DENSITY) {
+            for (int d = 0; d < a.cols.nd; ++d) {
+                const int4 dc = a.cols.dens[d];
+                const ColumnShape cs = density_shape(L, m, d, dc.x);
+#pragma unroll
+                for (int k = 0; k < Y1 - Y0; ++k) {
+                    const double v = lattice_density(cs, Y0 + k, L.at(m.n1(dc.x, k)), L.at(m.n2(dc.x, k)),
+                                                     L.at(m.n3(dc.x, k)));
+                    if (active) a.out[(i * a.cols.nd + d) * (Y1 - Y0) + k] = v;
+                }
+            }
+        } else if (MODE == MODE_CANDIDATE) {
+            // Candidate start at a sample column (IslandSearchEngine.java:2909-2925): stone at
+            // block y >= 68 whose block y - 1 is not stone.  At a lattice column the block
+            // density is the y-chain d1 += d5 of generateBareTerrain (BetaTerrain173.java:599-664).
+            bool found = false;
+            for (int d = 0; d < a.cols.nd; ++d) {
+                const int4 dc = a.cols.dens[d];
+                const ColumnShape cs = density_shape(L, m, d, dc.x);
+                double lat[9];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    lat[k] = lattice_density(cs, 8 + k, L.at(m.n1(dc.x, k)), L.at(m.n2(dc.x, k)), L.at(m.n3(dc.x, k)));
+                lat[8] = -10.0;
+                bool below = false;
+#pragma unroll
+                for (int k1 = 0; k1 < 8; ++k1) {
+                    double d1 = lat[k1];
+                    const double d5 = (lat[k1 + 1] - d1) * 0.125;
+#pragma unroll
+                    for (int l1 = 0; l1 < 8; ++l1) {
+                        const int y = 64 + k1 * 8 + l1;
+                        const bool stone = d1 > 0.0;
+                        if (y >= 68 && stone && !below) found = true;
+                        below = stone;
+                        d1 += d5;
+                    }
+                }
+            }
+            append_seed(active && found, seed, a.out_seeds, a.out_count, a.out_cap);
+        } else {  // MODE_LATTICE
+            for (int d = 0; d < a.cols.nd; ++d) {
+                const int4 dc = a.cols.dens[d];
+                const ColumnShape cs = density_shape(L, m, d, dc.x);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const double v = lattice_density(cs, 8 + k, L.at(m.n1(dc.x, k)), L.at(m.n2(dc.x, k)),
+                                                     L.at(m.n3(dc.x, k)));
+                    if (active) a.out[(i * a.cols.nd + d) * 8 + k] = v;
+                }
+            }
+        }
+    }
+}
+
+template __global__ void lattice_kernel<0, 17, MODE_DENSITY>(LatticeArgs);
+template __global__ void lattice_kernel<8, 16, MODE_CANDIDATE>(LatticeArgs);
+template __global__ void lattice_kernel<8, 16, MODE_LATTICE>(LatticeArgs);
+
+// ---------------------------------------------------------------------------
+// Stage-0 scout: first terrain generator, octave 0, chunk (0,0) 5x17x5, S = sum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void scout_kernel(ScoutArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    const int lane = threadIdx.x;
+    LanePerm P;
+    P.base = perm_lds + lane * 4;
+    const int64_t groups = (a.count + 63) >> 6;
+    for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const int64_t i = (grp << 6) + lane;
+        const bool active = i < a.count;
+        const int64_t seed = a.first_seed + (active ? i : 0);
+        Lcg r;
+        r.set_seed(seed);
+        const GenOffsets g = build_generator(r, P, c_magic);
+        YAxis<0, 17> ya;
+        ya.build(684.412, g.b);
+        double s = 0.0;
+        for (int x = 0; x < 5; ++x) {
+            const Axis ax = axis(static_cast<double>(x) * 684.412 + g.a);
+            for (int z = 0; z < 5; ++z) {
+                const Axis az = axis(static_cast<double>(z) * 684.412 + g.c);
+                double v[17];
+                perlin3_column<0, 17>(P, ax, az, ya, 1.0, v);
+#pragma unroll
+                for (int k = 0; k < 17; ++k) s += 0.0 + v[k];
+            }
+        }
+        append_seed(active && s > a.threshold, seed, a.out_seeds, a.out_count, a.out_cap);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Island flood (Stage 2b): one 256-thread workgroup per Stage-1 survivor.
+// Bitsets: one uint64 z-mask per (x, y) row, y in [68, 127]; window z extent <= 64.
+// ---------------------------------------------------------------------------
+constexpr int kFloorY = 68;
+constexpr int kLevels = 128 - kFloorY;  // 60 rows per x
+
+struct FloodShared {
+    unsigned long long* rem;   // stone not yet visited, [x * kLevels + (y - 68)]
+    unsigned long long* comp;  // current component
+    unsigned long long* st67;  // stone at y = 67 per x
+};
+
+// Block density chain of generateBareTerrain (BetaTerrain173.java:599-664) for one (x, y)
+// row across the window, from the window lattice lat[(chunk*25 + i*5 + l)*8 + (k - 8)],
+// level 16 = -10.0.  Returns the stone z-mask.
+__device__ unsigned long long stone_row(const double* __restrict__ lat, int x, int y, int cz_n) {
+    const int cxi = x >> 4, lx = x & 15;
+    const int i1 = lx >> 2, i2 = lx & 3;
+    const int k1 = y >> 3, l1 = y & 7;
+    unsigned long long bits = 0;
+    for (int czi = 0; czi < cz_n; ++czi) {
+        const double* cl = lat + static_cast<size_t>(cxi * cz_n + czi) * 25 * 8;
+        for (int j1 = 0; j1 < 4; ++j1) {
+            auto L = [&](int ii, int jj, int kk) -> double {
+                return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
+            };
+            double d1 = L(i1, j1, k1), d2 = L(i1, j1 + 1, k1), d3 = L(i1 + 1, j1, k1), d4 = L(i1 + 1, j1 + 1, k1);
+            const double d5 = (L(i1, j1, k1 + 1) - d1) * 0.125;
+            const double d6 = (L(i1, j1 + 1, k1 + 1) - d2) * 0.125;
+            const double d7 = (L(i1 + 1, j1, k1 + 1) - d3) * 0.125;
+            const double d8 = (L(i1 + 1, j1 + 1, k1 + 1) - d4) * 0.125;
+            for (int s = 0; s < l1; ++s) {
+                d1 += d5;
+                d2 += d6;
+                d3 += d7;
+                d4 += d8;
+            }
+            double d10 = d1, d11 = d2;
+            const double d12 = (d3 - d1) * 0.25;
+            const double d13 = (d4 - d2) * 0.25;
+            for (int s = 0; s < i2; ++s) {
+                d10 += d12;
+                d11 += d13;
+            }
+            double d15 = d10;
+            const double d16 = (d11 - d10) * 0.25;
+            for (int k2 = 0; k2 < 4; ++k2) {
+                if (d15 > 0.0) bits |= 1ULL << (czi * 16 + j1 * 4 + k2);
+                d15 += d16;
+            }
+        }
+    }
+    return bits;
+}
+
+struct CompStats {
+    int blocks, columns, minx, maxx, minz, maxz, miny, maxy;
+    long long sumx, sumz;
+    int bottom, side;
+};
+
+__device__ __forceinline__ int sum_of_bit_positions(unsigned long long m) {
+    int s = 0;
+    s += 1 * __popcll(m & 0xAAAAAAAAAAAAAAAAULL);
+    s += 2 * __popcll(m & 0xCCCCCCCCCCCCCCCCULL);
+    s += 4 * __popcll(m & 0xF0F0F0F0F0F0F0F0ULL);
+    s += 8 * __popcll(m & 0xFF00FF00FF00FF00ULL);
+    s += 16 * __popcll(m & 0xFFFF0000FFFF0000ULL);
+    s += 32 * __popcll(m & 0xFFFFFFFF00000000ULL);
+    return s;
+}
+
+// Java Math.round for a positive or negative double of small magnitude: floor(a + 1/2) exact.
+__device__ __forceinline__ int java_round(double a) {
+    const double f = floor(a);
+    int r = static_cast<int>(f);
+    if (a - f >= 0.5) ++r;
+    return r;
+}
+
+__device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats& b) {
+    // compareComponents(a, b) > 0, IslandSearchEngine.java:4079-4094
+    if (a.blocks != b.blocks) return a.blocks > b.blocks;
+    if (a.columns != b.columns) return a.columns > b.columns;
+    const int areaA = (a.maxx - a.minx + 1) * (a.maxz - a.minz + 1);
+    const int areaB = (b.maxx - b.minx + 1) * (b.maxz - b.minz + 1);
+    return areaA > areaB;
+}
+
+__global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
+    __shared__ int s_changed;
+    __shared__ CompStats s_cs;
+    __shared__ unsigned long long s_zor;
+    const int tid = threadIdx.x;
+    const int nx = a.p.chunks_x * 16;
+    const int nz = a.p.chunks_z * 16;
+    const int rows = nx * kLevels;
+    FloodShared S;
+    S.rem = flood_lds;
+    S.comp = flood_lds + rows;
+    S.st67 = flood_lds + 2 * rows;
+    int64_t nsurv = a.cap;
+    if (a.d_count != nullptr && static_cast<int64_t>(*a.d_count) < nsurv) nsurv = static_cast<int64_t>(*a.d_count);
+    const int nd = a.p.chunks_x * a.p.chunks_z * 25;
+    const int minWX = a.p.chunk_x0 * 16, minWZ = a.p.chunk_z0 * 16;
+
+    for (int64_t sv = blockIdx.x; sv < nsurv; sv += gridDim.x) {
+        const double* lat = a.lattice + static_cast<size_t>(sv) * nd * 8;
+        // 1. upsample rows y in [67, 127]
+        for (int r = tid; r < nx * (kLevels + 1); r += 256) {
+            const int x = r / (kLevels + 1);
+            const int y = 67 + r % (kLevels + 1);
+            const unsigned long long bits = stone_row(lat, x, y, a.p.chunks_z);
+            if (y == 67) S.st67[x] = bits;
+            else S.rem[x * kLevels + (y - kFloorY)] = bits;
+        }
+        __syncthreads();
+
+        bool have_best = false;
+        CompStats best{};
+        // 2. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
+        for (int ch = 0; ch < a.p.chunks_x * a.p.chunks_z; ++ch) {
+            const int cxi = ch / a.p.chunks_z, czi = ch % a.p.chunks_z;
+            for (int sc = 0; sc < 4; ++sc) {
+                const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
+                const int z = czi * 16 + ((sc & 1) ? 12 : 4);
+                for (int y = kFloorY; y < 128; ++y) {
+                    // stone here and not below (Java reads the chunk's own blocks array)
+                    const unsigned long long bitz = 1ULL << z;
+                    const unsigned long long here = S.rem[x * kLevels + (y - kFloorY)];
+                    // a visited stone block is no longer in rem: Java skips it (visited != 0)
+                    if ((here & bitz) == 0) continue;
+                    // below: the original stone mask; rem only loses bits of flooded components,
+                    // and a flooded component containing the block below would contain this one.
+                    const unsigned long long below = y == kFloorY ? S.st67[x] : S.rem[x * kLevels + (y - 1 - kFloorY)];
+                    if ((below & bitz) != 0) continue;
+                    // flood this component
+                    for (int r = tid; r < rows; r += 256) S.comp[r] = 0;
+                    __syncthreads();
+                    if (tid == 0) S.comp[x * kLevels + (y - kFloorY)] = bitz;
+                    __syncthreads();
+                    for (;;) {
+                        if (tid == 0) s_changed = 0;
+                        __syncthreads();
+                        int changed = 0;
+                        for (int r = tid; r < rows; r += 256) {
+                            const int rx = r / kLevels, ry = r % kLevels;
+                            const unsigned long long c = S.comp[r];
+                            unsigned long long nb = c | (c << 1) | (c >> 1);
+                            if (rx > 0) nb |= S.comp[r - kLevels];
+                            if (rx < nx - 1) nb |= S.comp[r + kLevels];
+                            if (ry > 0) nb |= S.comp[r - 1];
+                            if (ry < kLevels - 1) nb |= S.comp[r + 1];
+                            const unsigned long long nw = nb & S.rem[r];
+                            if (nw != c) {
+                                S.comp[r] = nw;
+                                changed = 1;
+                            }
+                        }
+                        if (changed) s_changed = 1;
+                        __syncthreads();
+                        const int any = s_changed;
+                        __syncthreads();
+                        if (!any) break;
+                    }
+                    // statistics of the component
+                    if (tid == 0) {
+                        s_cs.blocks = 0; s_cs.columns = 0;
+                        s_cs.minx = 0x7fffffff; s_cs.maxx = -0x7fffffff; s_cs.miny = 0x7fffffff; s_cs.maxy = -0x7fffffff;
+                        s_cs.minz = 0; s_cs.maxz = 0; s_cs.sumx = 0; s_cs.sumz = 0; s_cs.bottom = 0; s_cs.side = 0;
+                        s_zor = 0;
+                    }
+                    __syncthreads();
+                    for (int rx = tid; rx < nx; rx += 256) {
+                        unsigned long long col = 0;
+                        int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
+                        for (int ry = 0; ry < kLevels; ++ry) {
+                            const unsigned long long c = S.comp[rx * kLevels + ry];
+                            if (c) {
+                                col |= c;
+                                blk += __popcll(c);
+                                ymin = min(ymin, ry + kFloorY);
+                                ymax = max(ymax, ry + kFloorY);
+                            }
+                        }
+                        if (col) {
+                            const int ncol = __popcll(col);
+                            atomicAdd(&s_cs.blocks, blk);
+                            atomicAdd(&s_cs.columns, ncol);
+                            atomicMin(&s_cs.minx, rx);
+                            atomicMax(&s_cs.maxx, rx);
+                            atomicMin(&s_cs.miny, ymin);
+                            atomicMax(&s_cs.maxy, ymax);
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumx),
+                                      static_cast<unsigned long long>(static_cast<long long>(ncol) * (minWX + rx)));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumz),
+                                      static_cast<unsigned long long>(static_cast<long long>(ncol) * minWZ +
+                                                                      sum_of_bit_positions(col)));
+                            atomicOr(&s_zor, col);
+                            if ((S.comp[rx * kLevels] & S.st67[rx]) != 0) atomicOr(&s_cs.bottom, 1);
+                            if (rx == 0 || rx == nx - 1) atomicOr(&s_cs.side, 1);
+                        }
+                    }
+                    // remove the component from the unvisited stone
+                    for (int r = tid; r < rows; r += 256) S.rem[r] &= ~S.comp[r];
+                    __syncthreads();
+                    CompStats cs = s_cs;
+                    const unsigned long long zor = s_zor;
+                    cs.minz = __ffsll(static_cast<long long>(zor)) - 1;
+                    cs.maxz = 63 - __clzll(static_cast<long long>(zor));
+                    if ((zor & 1ULL) || ((zor >> (nz - 1)) & 1ULL)) cs.side = 1;
+                    // isMatch, IslandSearchEngine.java:3972-4003 (world bounds = window bounds + offsets)
+                    const int width = cs.maxx - cs.minx + 1;
+                    const int depth = cs.maxz - cs.minz + 1;
+                    const bool match = cs.blocks >= a.p.min_blocks && cs.columns >= a.p.min_columns &&
+                                       cs.maxy >= a.p.min_y_for_match && !cs.bottom && width >= a.p.min_width &&
+                                       depth >= a.p.min_depth &&
+                                       cs.blocks / static_cast<double>(cs.columns > 1 ? cs.columns : 1) >=
+                                               a.p.min_avg_thickness;
+                    if (match && (!have_best || comp_better(cs, best))) {
+                        best = cs;
+                        have_best = true;
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (tid == 0 && have_best) {
+            const unsigned long long slot = atomicAdd(a.hit_count, 1ULL);
+            if (static_cast<int64_t>(slot) < a.hit_cap) {
+                bsf_island_hit h;
+                h.seed = a.seeds[sv];
+                h.min_x = minWX + best.minx;
+                h.max_x = minWX + best.maxx;
+                h.min_z = minWZ + best.minz;
+                h.max_z = minWZ + best.maxz;
+                h.min_y = best.miny;
+                h.max_y = best.maxy;
+                h.blocks = best.blocks;
+                h.columns = best.columns;
+                h.center_x = java_round(static_cast<double>(best.sumx) / best.columns);
+                h.center_z = java_round(static_cast<double>(best.sumz) / best.columns);
+                h.chunk_x = h.center_x >> 4;
+                h.chunk_z = h.center_z >> 4;
+                h.flags = best.side ? 1 : 0;
+                h.window_index = a.window_index;
+                a.hits[slot] = h;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace bsf

@@ -1,0 +1,121 @@
+/*
+ * bsf.h -- C ABI of the MI355X BetaSeedFinder search worker library (libbsf.so).
+ *
+ * Plain C types only: host or device pointers and sizes, int status codes (0 = ok),
+ * and bsf_last_error() for the message of the last failure on the calling thread.
+ * Every entry point runs hand-written gfx950 HIP kernels; there is no CPU fallback.
+ *
+ * The reference worker (native/src/gpu_p20_benchmark.cpp) exposes its GPU pipeline
+ * only through process argv + stdin/stdout records; the entry points below are the
+ * in-process calls its stream loops would make.  Each cites the code it replaces.
+ */
+#ifndef BSF_H
+#define BSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSF_OK 0
+#define BSF_ERR_ARG 1     /* invalid argument (sizes, null pointers, unknown profile) */
+#define BSF_ERR_HIP 2     /* a HIP runtime call failed */
+#define BSF_ERR_STATE 3   /* library not initialised / wrong device */
+#define BSF_ERR_CAPACITY 4 /* output capacity exceeded (results truncated) */
+
+/* ---- lifecycle ---------------------------------------------------------------- */
+/* Select the HIP device and upload constant tables.  Replaces the implicit
+ * hipGetDevice default of the worker (gpu_p20_benchmark.cpp:979-983). */
+int         bsf_init(int device);
+void        bsf_shutdown(void);
+const char* bsf_last_error(void);
+/* Four-line device banner: "GPU:", "Architecture:", "Compute units:", "Global memory:"
+ * (printDevice, gpu_p20_benchmark.cpp:979-992). */
+int         bsf_device_banner(char* buf, size_t cap);
+
+/* ---- terrain density grid (BetaTerrain173.generateTerrainNoise, BetaTerrain173.java:667-834)
+ * For each seed, the 5x17x5 FP64 density lattice of chunk (chunk_x, chunk_z), index
+ * (x*5 + z)*17 + y, bit-exact to Java.  out holds count*425 doubles. */
+int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t chunk_x, int32_t chunk_z, double* out);
+/* Same with device-resident seeds/output on a caller stream (hipStream_t as void*). */
+int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t chunk_x, int32_t chunk_z,
+                            double* d_out, void* stream);
+
+/* ---- Stage-0 scout over a contiguous seed range ------------------------------------
+ * Per seed: setSeed + the first terrain Perlin build (NoiseGeneratorPerlin173.java:63-78)
+ * and its octave-0 values on chunk (0,0)'s 5x17x5 lattice; S = their sum in index order.
+ * Seeds with S > threshold are returned sorted ascending. */
+int bsf_scout_range(int64_t first_seed, int64_t count, double threshold,
+                    int64_t* out_seeds, int64_t cap, int64_t* n_found);
+/* Device form: survivors appended unsorted to d_out; *d_count is incremented (not reset). */
+int bsf_scout_range_device(int64_t first_seed, int64_t count, double threshold,
+                           int64_t* d_out, int64_t cap, unsigned long long* d_count, void* stream);
+
+/* ---- full floating-island pipeline (IslandSearchEngine.findFloatingIslandInSeed exact
+ * block scan, IslandSearchEngine.java:2861-2977, with isMatch :3972-4003, floodFillFlat
+ * :4096-4202, compareComponents :4079-4094) over a chunk window. --------------------- */
+typedef struct bsf_island_params {
+    int32_t chunk_x0, chunk_z0;      /* first chunk of the window */
+    int32_t chunks_x, chunks_z;      /* window size in chunks, 1..8 each */
+    int32_t min_blocks;              /* SearchSettings.minBlocks (minimum island volume) */
+    int32_t min_columns;             /* SearchSettings.minColumns */
+    int32_t min_y_for_match;         /* SearchSettings.minYForMatch */
+    int32_t min_width, min_depth;    /* SearchSettings.minWidth / minDepth */
+    double  min_avg_thickness;       /* SearchSettings.minAvgThickness */
+} bsf_island_params;
+
+/* One hit: the best matching component of a seed (Java's bestMatch). 64 bytes. */
+typedef struct bsf_island_hit {
+    int64_t seed;
+    int32_t chunk_x, chunk_z;        /* chunk containing the island centre */
+    int32_t min_x, max_x, min_z, max_z; /* world block bounds */
+    int32_t min_y, max_y;
+    int32_t blocks, columns;
+    int32_t center_x, center_z;      /* Math.round(sum / columns) */
+    int32_t flags;                   /* bit0: touches the window's side border */
+    int32_t window_index;            /* job id in batched runs, else 0 */
+} bsf_island_hit;
+
+/* Hits for seeds first_seed .. first_seed+count-1, sorted by seed. */
+int bsf_island_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
+                     bsf_island_hit* out, int64_t cap, int64_t* n_found);
+/* Hits for an explicit seed list (host), sorted by seed. */
+int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_params* p,
+                     bsf_island_hit* out, int64_t cap, int64_t* n_found);
+/* Stage-1 survivors only (seeds with a candidate start at y >= 68 over non-stone in a
+ * sample column), sorted.  Exposed for gate audits. */
+int bsf_island_stage1_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
+                            int64_t* out_seeds, int64_t cap, int64_t* n_found);
+
+/* ---- worker stream protocol bodies --------------------------------------------------- */
+/* `stream` mode batch: positive P20 column count per seed (0..64) for the 8x8 P20 sample of
+ * the 16x16 Stage0 lattice around the origin.  Replaces the p20KernelFromSeeds launch and
+ * count read-back of streamFilter (gpu_p20_benchmark.cpp:7655-7668). */
+int bsf_p20_counts(const int64_t* seeds, int32_t count, int32_t* out_counts);
+
+/* Hunt profiles of `stage0stream` (gpu_p20_benchmark.cpp:7270-7290). */
+#define BSF_PROFILE_GENERAL 0
+#define BSF_PROFILE_MEGA 1
+#define BSF_PROFILE_RECORD60 2
+#define BSF_PROFILE_RECORD80 3
+
+typedef struct bsf_stage0_counters {
+    uint64_t worlds, regions, p20_passed, stage1_passed, stage05_passed;
+    uint64_t stage075_evaluated, stage075_passed, stage075_rejected;
+    uint64_t mega_topology_rejected, coarse_evaluated, coarse_passed;
+} bsf_stage0_counters;
+
+/* `stage0stream` batch: count worlds x centers (1 or 8) records of 16 (lean) or 34 (research)
+ * bytes in the reference's little-endian layout, record index world*centers + center.
+ * Replaces the per-batch body of streamStage0Filter (gpu_p20_benchmark.cpp:7411-7579).
+ * counters (optional) accumulate the closing-summary statistics. */
+int bsf_stage0_records(const int64_t* seeds, int32_t count, int32_t profile, int32_t research,
+                       int32_t centers, uint8_t* out_records, bsf_stage0_counters* counters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
