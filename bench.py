@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: full floating-island pipeline seeds/s on one MI355X (bit-exact hit list).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-log2 B] [--no-cpu]
+
+Workload (the full-pipeline configuration): a 2^36-seed contiguous range starting at
+splitmix64(0xB5F00004) & (2^48 - 1); a 4x4 chunk window {-2..1}^2 around the origin;
+minimum island volume V = 200 + floor(1801 u), u = splitmix64(0xB5F00005) as a 53-bit
+double; the other island gates at the reference's SearchSettings defaults.  One step is one
+batch of 2^B consecutive seeds of that range (inputs are a seed range: nothing is copied to
+the device).  The timed region covers K steps; seeds/s = K * 2^B / elapsed.
+
+The JSON line also carries:
+  roofline      FP64 VALU roofline of the dominant kernel (island Stage 1), timed with HIP
+                events on its own stream over the timed region;
+  cpu_baseline  the CPU oracle on a bounded sample of the same range, all host threads
+                (rank 0, N = 1 only), with the sample's hit list compared to the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MASK48 = (1 << 48) - 1
+METRIC = "seeds/sec full island pipeline (bit-exact) + p50/p99 job latency, served vs plain"
+FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64, FMA counted as two flops
+FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no FMA: -ffp-contract=off)
+
+
+def splitmix64(x):
+    z = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+def unit53(x):
+    return (splitmix64(x) >> 11) * (1.0 / (1 << 53))
+
+
+def island_workload():
+    start = splitmix64(0xB5F00004) & MASK48
+    min_blocks = 200 + int(1801 * unit53(0xB5F00005))
+    return start, min_blocks
+
+
+# Algorithmic FP64 operations (add, sub, mul, div, compare, int<->double conversion) per
+# construct of the Stage-1 kernel, counted from the source (bsf_device.hpp / bsf_kernels.hip).
+OPS = {
+    "build": 18,            # 3 x nextDouble (2 cvt, mul, add, mul) x 256.0
+    "axis": 16,             # coordinate (cvt, mul, add) + Java floor (2 cvt, cmp) + frac (cvt, sub) + fm1 + fade (7)
+    "yaxis_level": 16,      # level coordinate, floor, frac, fade, cell fm1
+    "yaxis_setup": 14,      # Y0 coordinate + one cell walk-back step
+    "point3": 31,           # 8 gradients, 7 lerps, weight, accumulate
+    "point2": 18,           # 1 legacy + 3 gradients, 3 lerps, weight, accumulate (+0.0*x)
+    "simplex": 70,          # one simplex2 sample with all three corners live, weight, accumulate
+    "finish": 16,           # climate post-processing
+    "shape": 20,            # column shape d5 / d7
+    "density_level": 15,    # one lattice density
+    "chain_level": 18,      # candidate y-chain per lattice level (sub, mul, 8 x (cmp, add))
+}
+
+
+def stage1_fp64_ops_per_seed(n_cols=64, levels=8):
+    o3d = 40 * (OPS["yaxis_setup"] + levels * OPS["yaxis_level"] + n_cols * (2 * OPS["axis"] + levels * OPS["point3"]))
+    o2d = 26 * n_cols * (2 * OPS["axis"] + OPS["point2"])
+    clim = 10 * n_cols * OPS["simplex"]
+    builds = 84 * OPS["build"]
+    combine = n_cols * (OPS["finish"] + OPS["shape"] + levels * (OPS["density_level"] + OPS["chain_level"]))
+    return o3d + o2d + clim + builds + combine
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch-log2", type=int, default=23)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: every box in this setup has a single GPU, so --gpus %d is "
+              "not supported (seeds are independent; see the sharding note in DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import betaseedfinder_amd as bsf
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None  # noqa: E731
+
+    bsf.init(0)
+    L = bsf.lib()
+    start, min_blocks = island_workload()
+    params = bsf.island_params(-2, -2, 4, 4, min_blocks=min_blocks)
+    batch = 1 << args.batch_log2
+
+    def step(k):
+        return bsf.island_range(start + k * batch, batch, params, cap=1 << 20)
+
+    for k in range(args.warmup):
+        step(k)
+    L.bsf_timing_reset()
+    L.bsf_timing_enable(1)
+    sync()
+    t0 = time.perf_counter()
+    hits = 0
+    for k in range(args.warmup, args.warmup + args.steps):
+        hits += len(step(k))
+    sync()
+    elapsed = time.perf_counter() - t0
+    L.bsf_timing_enable(0)
+
+    def kstats(name):
+        ms, launches, units = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        L.bsf_kernel_stats(name.encode(), ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(units))
+        return ms.value, launches.value, units.value
+
+    seeds = args.steps * batch
+    value = seeds / elapsed
+    s1_ms, s1_n, s1_units = kstats("island_stage1")
+    s2_ms, s2_n, s2_units = kstats("island_stage2")
+    fl_ms, fl_n, _ = kstats("island_flood")
+    ops_seed = stage1_fp64_ops_per_seed()
+    avg_launch_s = s1_ms / 1e3 / max(1, s1_n)
+    seeds_per_launch = s1_units / max(1, s1_n)
+    achieved_tflops = ops_seed * seeds_per_launch / avg_launch_s / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "seeds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (contiguous seed range)",
+        "config": {
+            "workload": "full floating-island pipeline: seeds [start, start+2^36) with start = splitmix64(0xB5F00004) "
+                        "& (2^48-1); 4x4 chunk window {-2..1}^2; min island volume %d" % min_blocks,
+            "seeds_per_step": batch,
+            "first_seed": start,
+            "min_island_blocks": min_blocks,
+            "hits_in_timed_region": hits,
+            "stage1_survivor_fraction": round(s2_units / max(1, s1_units), 5),
+            "kernel_ms": {"island_stage1": round(s1_ms, 3), "island_stage2": round(s2_ms, 3),
+                          "island_flood": round(fl_ms, 3)},
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "lattice_kernel<8,16,MODE_CANDIDATE> (island Stage 1)",
+            "achieved": round(achieved_tflops, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
+            "frac_of_nonfused_issue_peak": round(achieved_tflops / FP64_NONFUSED_PEAK, 4),
+            "fp64_ops_per_seed": ops_seed,
+            "seeds_per_launch": int(seeds_per_launch),
+            "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+            "traffic": None,
+        },
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        # bounded sample: grow until ~cpu_seconds of oracle work
+        n = 256
+        t = time.perf_counter()
+        cpu_hits = oracle.run_island(start, n, oracle.island_params(-2, -2, 4, 4, min_blocks=min_blocks),
+                                     threads=threads)
+        dt = time.perf_counter() - t
+        if dt < args.cpu_seconds / 4:
+            n = int(n * args.cpu_seconds / max(dt, 1e-3))
+            n = max(256, min(n, 1 << 20))
+            t = time.perf_counter()
+            cpu_hits = oracle.run_island(start, n, oracle.island_params(-2, -2, 4, 4, min_blocks=min_blocks),
+                                         threads=threads)
+            dt = time.perf_counter() - t
+        gpu_sample = bsf.island_range(start, n, params)
+        result["cpu_baseline"] = {
+            "value": round(n / dt, 2),
+            "unit": "seeds/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": "first %d seeds of the same range through the CPU oracle (%d threads, %.1f s); "
+                      "hit lists equal: %s" % (n, threads, dt, gpu_sample == cpu_hits),
+        }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -44,6 +45,64 @@ void DeviceBuffer::release() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     size = 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch timing
+// ---------------------------------------------------------------------------
+struct TimerStats {
+    double total_ms = 0.0;
+    uint64_t launches = 0, units = 0;
+};
+struct PendingLaunch {
+    std::string name;
+    hipEvent_t a, b;
+    uint64_t units;
+};
+bool g_timing = false;
+std::map<std::string, TimerStats> g_timer_stats;
+std::vector<PendingLaunch> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t take_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+LaunchTimer::LaunchTimer(const char* n, hipStream_t s) : name(n), stream(s) {
+    if (!g_timing) return;
+    start = take_event();
+    check(hipEventRecord(start, s), "hipEventRecord");
+    active = true;
+}
+
+void LaunchTimer::stop(uint64_t units) {
+    if (!active) return;
+    hipEvent_t e = take_event();
+    check(hipEventRecord(e, stream), "hipEventRecord");
+    g_pending.push_back(PendingLaunch{name, start, e, units});
+    active = false;
+}
+
+void resolve_timers() {
+    for (auto& p : g_pending) {
+        check(hipEventSynchronize(p.b), "hipEventSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        TimerStats& t = g_timer_stats[p.name];
+        t.total_ms += ms;
+        t.launches += 1;
+        t.units += p.units;
+        g_event_pool.push_back(p.a);
+        g_event_pool.push_back(p.b);
+    }
+    g_pending.clear();
 }
 
 void require_init() {
@@ -151,8 +210,10 @@ void run_density(const int64_t* d_seeds, int64_t count, int cx, int cz, double* 
     a.cols = cols.dev;
     a.scratch = static_cast<double*>(g_ctx.scratch.ensure(static_cast<size_t>(grid) * vals * 64 * sizeof(double)));
     a.out = d_out;
+    LaunchTimer t("density", s);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, s, a);
     check(hipGetLastError(), "lattice_kernel launch");
+    t.stop(static_cast<uint64_t>(count));
 }
 
 void run_scout(int64_t first, int64_t count, double thr, int64_t* d_out, int64_t cap, unsigned long long* d_count,
@@ -161,8 +222,10 @@ void run_scout(int64_t first, int64_t count, double thr, int64_t* d_out, int64_t
     const int64_t groups = (count + 63) / 64;
     const int grid = static_cast<int>(std::min<int64_t>(groups, resident_blocks(scout_kernel, 64, 0)));
     ScoutArgs a{first, count, thr, d_out, d_count, cap};
+    LaunchTimer t("scout", s);
     hipLaunchKernelGGL(scout_kernel, dim3(grid), dim3(64), 0, s, a);
     check(hipGetLastError(), "scout_kernel launch");
+    t.stop(static_cast<uint64_t>(count));
 }
 
 // Stage 1 over seeds [first, first+count) or an explicit device list.
@@ -183,8 +246,10 @@ void run_island_stage1(const int64_t* d_seeds, int64_t first, int64_t count, con
     a.out_seeds = d_surv;
     a.out_count = d_nsurv;
     a.out_cap = cap;
+    LaunchTimer t("island_stage1", s);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, s, a);
     check(hipGetLastError(), "stage1 launch");
+    t.stop(static_cast<uint64_t>(count));
 }
 
 // Stage 2 (window lattice) + flood for n survivors at d_surv.
@@ -210,8 +275,10 @@ void run_island_stage2(const int64_t* d_surv, int64_t n, const bsf_island_params
         a.cols = cols.dev;
         a.scratch = static_cast<double*>(g_ctx.scratch2.ensure(static_cast<size_t>(grid) * vals * 64 * sizeof(double)));
         a.out = d_lat;
+        LaunchTimer t2("island_stage2", s);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, s, a);
         check(hipGetLastError(), "stage2 launch");
+        t2.stop(static_cast<uint64_t>(m));
         FloodArgs f{};
         f.p = p;
         f.seeds = d_surv + off;
@@ -223,8 +290,10 @@ void run_island_stage2(const int64_t* d_surv, int64_t n, const bsf_island_params
         f.hit_cap = hit_cap;
         f.window_index = window_index;
         const int fgrid = static_cast<int>(std::min<int64_t>(m, resident_blocks(island_flood_kernel, 256, flood_lds)));
+        LaunchTimer t3("island_flood", s);
         hipLaunchKernelGGL(island_flood_kernel, dim3(fgrid), dim3(256), flood_lds, s, f);
         check(hipGetLastError(), "flood launch");
+        t3.stop(static_cast<uint64_t>(m));
         check(hipStreamSynchronize(s), "stage2 sync");
     }
 }
@@ -329,6 +398,35 @@ void bsf_shutdown(void) {
 }
 
 const char* bsf_last_error(void) { return g_last_error.c_str(); }
+
+int bsf_timing_enable(int on) {
+    return guarded([&] {
+        require_init();
+        resolve_timers();
+        g_timing = on != 0;
+    });
+}
+
+int bsf_timing_reset(void) {
+    return guarded([&] {
+        require_init();
+        resolve_timers();
+        g_timer_stats.clear();
+    });
+}
+
+int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uint64_t* units) {
+    return guarded([&] {
+        require_init();
+        if (!name || !total_ms || !launches || !units) throw ArgError("null stats argument");
+        resolve_timers();
+        auto it = g_timer_stats.find(name);
+        const TimerStats t = it == g_timer_stats.end() ? TimerStats{} : it->second;
+        *total_ms = t.total_ms;
+        *launches = t.launches;
+        *units = t.units;
+    });
+}
 
 int bsf_device_banner(char* buf, size_t cap) {
     return guarded([&] {

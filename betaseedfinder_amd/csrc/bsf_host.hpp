@@ -59,6 +59,18 @@ struct Context {
 
 extern Context g_ctx;
 
+// HIP-event timing of named kernel launches (enabled by bsf_timing_enable).  Each launch is
+// bracketed by two events on the stream it is launched on; durations resolve lazily.
+struct LaunchTimer {
+    const char* name;
+    hipEvent_t start = nullptr;
+    hipStream_t stream = nullptr;
+    bool active = false;
+    LaunchTimer(const char* n, hipStream_t s);
+    void stop(uint64_t units);
+};
+void resolve_timers();
+
 void require_init();
 
 template <class K>

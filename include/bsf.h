@@ -35,6 +35,14 @@ const char* bsf_last_error(void);
  * (printDevice, gpu_p20_benchmark.cpp:979-992). */
 int         bsf_device_banner(char* buf, size_t cap);
 
+/* ---- launch timing (HIP events on the launching stream; for benchmarks) ---------------
+ * Kernel names: "density", "scout", "island_stage1", "island_stage2", "island_flood",
+ * "stage0_p20", "stage0_upper", "stage0_lower", "stage0_coarse".  units = seeds (or
+ * survivor seeds / worlds) processed by the timed launches. */
+int bsf_timing_enable(int on);
+int bsf_timing_reset(void);
+int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uint64_t* units);
+
 /* ---- terrain density grid (BetaTerrain173.generateTerrainNoise, BetaTerrain173.java:667-834)
  * For each seed, the 5x17x5 FP64 density lattice of chunk (chunk_x, chunk_z), index
  * (x*5 + z)*17 + y, bit-exact to Java.  out holds count*425 doubles. */
