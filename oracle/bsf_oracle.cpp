@@ -1059,6 +1059,20 @@ void bsfo_random_next_bits(int64_t seed, int n, int32_t bits, int32_t* out) {
     for (int i = 0; i < n; ++i) out[i] = r.next(bits);
 }
 
+void bsfo_random_mixed(int64_t seed, int n, const int32_t* ops, int64_t* out) {
+    JRandom r(seed);
+    for (int i = 0; i < n; ++i) {
+        if (ops[i] == -1) {
+            double d = r.nextDouble();
+            std::memcpy(&out[i], &d, 8);
+        } else if (ops[i] == -2) {
+            out[i] = r.next(31);
+        } else {
+            out[i] = r.nextInt(ops[i]);
+        }
+    }
+}
+
 void bsfo_perlin_state(int64_t seed, int generator_index, uint8_t perm256[256], double abc[3]) {
     JRandom r(seed);
     Perlin p;
@@ -1289,16 +1303,24 @@ int bsfo_mega_topology_reject(double score, int extreme, const bsfo_p19_features
 
 int bsfo_coarse_score_signs(const uint8_t* signs) { return coarseScoreSigns(signs); }
 
-int bsfo_coarse_score(int64_t seed, int32_t ox, int32_t oz, uint8_t* signs_out) {
+void bsfo_coarse_arrays(int64_t seed, int32_t ox, int32_t oz, double* n1, double* n2, double* n3, double* n4,
+                        double* n5, double* temp, double* rain, uint8_t* signs) {
     Terrain t(seed);
     const int SIZE = kCoarseSize, YL = kCoarseY;
     const int fromX = -28 + ox, fromZ = -28 + oz;
-    std::vector<double> temp(SIZE * SIZE), rain(SIZE * SIZE);
-    t.climate.sample(fromX * 4 + 2, fromZ * 4 + 2, SIZE, SIZE, 4, 4, temp.data(), rain.data());
+    // coarse climate at block coarse*4+2 (BetaTerrain173.java:2562-2584, strided sampler)
+    t.climate.sample(fromX * 4 + 2, fromZ * 4 + 2, SIZE, SIZE, 4, 4, temp, rain);
     std::vector<double> dens(SIZE * YL * SIZE);
-    t.terrainNoise(dens.data(), fromX, 0, fromZ, SIZE, YL, SIZE, temp.data(), rain.data(), true);
-    std::vector<uint8_t> signs(SIZE * SIZE * YL);
-    for (size_t i = 0; i < signs.size(); ++i) signs[i] = dens[i] > 0.0 ? 1 : 0;
+    t.terrainNoise(dens.data(), fromX, 0, fromZ, SIZE, YL, SIZE, temp, rain, true, n1, n2, n3, n4, n5);
+    for (size_t i = 0; i < dens.size(); ++i) signs[i] = dens[i] > 0.0 ? 1 : 0;
+}
+
+int bsfo_coarse_score(int64_t seed, int32_t ox, int32_t oz, uint8_t* signs_out) {
+    const int SIZE = kCoarseSize, YL = kCoarseY, CELLS = SIZE * SIZE * YL, COLS = SIZE * SIZE;
+    std::vector<double> n1(CELLS), n2(CELLS), n3(CELLS), n4(COLS), n5(COLS), temp(COLS), rain(COLS);
+    std::vector<uint8_t> signs(CELLS);
+    bsfo_coarse_arrays(seed, ox, oz, n1.data(), n2.data(), n3.data(), n4.data(), n5.data(), temp.data(),
+                       rain.data(), signs.data());
     if (signs_out) std::memcpy(signs_out, signs.data(), signs.size());
     return coarseScoreSigns(signs.data());
 }

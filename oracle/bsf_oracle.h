@@ -56,6 +56,9 @@ void     bsfo_random_doubles(int64_t seed, int n, double* out);
 void     bsfo_random_ints(int64_t seed, int n, int32_t bound, int32_t* out);
 void     bsfo_random_next_bits(int64_t seed, int n, int32_t bits, int32_t* out);
 
+/* scripted sequence: op -1 = nextDouble (bits returned), -2 = next(31), b > 0 = nextInt(b) */
+void     bsfo_random_mixed(int64_t seed, int n, const int32_t* ops, int64_t* out);
+
 /* ---- generator builds: generator index g of a Random(seed) stream built like the
  *      terrain constructor (g counts Perlin constructors, consumed ones included) ---- */
 void     bsfo_perlin_state(int64_t seed, int generator_index, uint8_t perm256[256], double abc[3]);
@@ -133,6 +136,10 @@ int      bsfo_mega_topology_reject(double score, int extreme, const bsfo_p19_fea
 /* coarse 61x61x17 sign grid at a coarse offset and its component score */
 int      bsfo_coarse_score(int64_t seed, int32_t coarse_offset_x, int32_t coarse_offset_z, uint8_t* signs_out);
 int      bsfo_coarse_score_signs(const uint8_t* signs);
+/* the coarse grid's intermediate arrays: n1/n2/n3/signs [61*61*17], n4/n5/temp/rain [61*61] */
+void     bsfo_coarse_arrays(int64_t seed, int32_t coarse_offset_x, int32_t coarse_offset_z,
+                            double* n1, double* n2, double* n3, double* n4, double* n5,
+                            double* temp, double* rain, uint8_t* signs);
 
 #ifdef __cplusplus
 }

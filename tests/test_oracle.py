@@ -1,0 +1,453 @@
+"""Pin the CPU oracle: reference fixtures (tests/golden/ref_*.json, written by the reference's
+own C++ host code), Java known answers, closed forms and brute force on tiny inputs."""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)["cases"]
+
+
+def h2d(h):
+    return np.array([int(h, 16)], dtype=np.uint64).view(np.float64)[0]
+
+
+def d2h(v):
+    return "%016x" % np.array([v], dtype=np.float64).view(np.uint64)[0]
+
+
+def plus0(v):
+    """Java's `0.0 + v` accumulator start turns -0.0 into +0.0."""
+    return 0.0 + v
+
+
+# ---------------------------------------------------------------- java.util.Random
+
+def test_random_known_java_answers(oracle):
+    L = oracle.lib()
+    out = (ctypes.c_int32 * 4)()
+    L.bsfo_random_next_bits(ctypes.c_int64(0), 4, 32, out)
+    assert list(out) == [-1155484576, -723955400, 1033096058, -1690734402]  # new Random(0).nextInt()
+    L.bsfo_random_next_bits(ctypes.c_int64(42), 4, 32, out)
+    assert list(out) == [-1170105035, 234785527, -1360544799, 205897768]    # new Random(42).nextInt()
+    d = (ctypes.c_double * 1)()
+    L.bsfo_random_doubles(ctypes.c_int64(0), 1, d)
+    assert d[0] == 0.730967787376657                                         # new Random(0).nextDouble()
+
+
+def test_random_matches_reference_fixture(oracle):
+    L = oracle.lib()
+    for case in load("ref_random.json"):
+        ops = [-1] * len(case["doubles"]) + [b for b, _ in case["ints"]] + [-2] * len(case["bits31"])
+        arr = (ctypes.c_int32 * len(ops))(*ops)
+        out = (ctypes.c_int64 * len(ops))()
+        L.bsfo_random_mixed(ctypes.c_int64(case["seed"]), len(ops), arr, out)
+        nd, ni = len(case["doubles"]), len(case["ints"])
+        got_d = ["%016x" % (out[i] & 0xFFFFFFFFFFFFFFFF) for i in range(nd)]
+        assert got_d == case["doubles"]
+        assert [out[nd + i] for i in range(ni)] == [v for _, v in case["ints"]]
+        assert [out[nd + ni + i] for i in range(len(case["bits31"]))] == case["bits31"]
+
+
+def test_random_nextint_rejection_path_by_brute_force(oracle):
+    """nextInt(bound) for a large odd bound takes the rejection branch often; restate the JDK
+    algorithm with Python integers and compare long sequences."""
+    L = oracle.lib()
+    bound = 1073741825  # rejection probability ~ 1/2 per draw
+    n = 4000
+    out = (ctypes.c_int32 * n)()
+    L.bsfo_random_ints(ctypes.c_int64(99), n, bound, out)
+    s = (99 ^ 0x5DEECE66D) & ((1 << 48) - 1)
+    res = []
+    for _ in range(n):
+        while True:
+            s = (s * 0x5DEECE66D + 0xB) & ((1 << 48) - 1)
+            bits = s >> 17
+            val = bits % bound
+            if bits - val + (bound - 1) < (1 << 31):
+                break
+        res.append(val)
+    assert list(out) == res
+
+
+# ---------------------------------------------------------------- Perlin / simplex builds
+
+def _perm_abc(case_state):
+    perm = (ctypes.c_uint8 * 256)(*case_state["perm"])
+    abc = (ctypes.c_double * 3)(*[h2d(h) for h in case_state["abc"]])
+    return perm, abc
+
+
+def test_generator_states_match_reference(oracle):
+    L = oracle.lib()
+    for case in load("ref_perlin.json"):
+        for t in case["terrain"]:
+            perm = (ctypes.c_uint8 * 256)()
+            abc = (ctypes.c_double * 3)()
+            L.bsfo_perlin_state(ctypes.c_int64(case["seed"]), t["index"], perm, abc)
+            assert list(perm) == t["state"]["perm"]
+            assert [d2h(v) for v in abc] == t["state"]["abc"]
+        for kind, st in enumerate(case["climate"]):
+            perm = (ctypes.c_uint8 * 256)()
+            abc = (ctypes.c_double * 3)()
+            L.bsfo_simplex_state(ctypes.c_int64(case["seed"]), kind, 0, perm, abc)
+            assert list(perm) == st["perm"]
+            assert [d2h(v) for v in abc] == st["abc"]
+
+
+def test_noise_primitives_match_reference(oracle):
+    L = oracle.lib()
+    n = 0
+    for case in load("ref_perlin.json"):
+        for t in case["terrain"]:
+            perm, abc = _perm_abc(t["state"])
+            for x, y, z, v in t["p3"]:
+                got = L.bsfo_perlin3_point(perm, abc, ctypes.c_double(h2d(x)), ctypes.c_double(h2d(y)),
+                                           ctypes.c_double(h2d(z)))
+                assert d2h(got) == v
+                n += 1
+            for col in t["col17"]:
+                out = (ctypes.c_double * 17)()
+                L.bsfo_perlin3_column(perm, abc, ctypes.c_double(h2d(col["x"])), ctypes.c_double(h2d(col["sy"])),
+                                      ctypes.c_double(h2d(col["z"])), 17, out)
+                assert [d2h(v) for v in out] == [d2h(plus0(h2d(v))) for v in col["v"]]
+                n += 17
+            for x, z, v in t["p2"]:
+                got = L.bsfo_perlin2_point(perm, abc, ctypes.c_double(h2d(x)), ctypes.c_double(h2d(z)))
+                assert d2h(got) == d2h(plus0(h2d(v)))
+                n += 1
+            for x, z, v in t["s2"]:
+                got = L.bsfo_simplex2_point(perm, abc, ctypes.c_double(h2d(x)), ctypes.c_double(h2d(z)))
+                assert d2h(got) == v
+                n += 1
+    assert n > 400
+
+
+def test_simplex_constants_are_javas_sqrt_forms():
+    # NoiseGenerator2173.java:13-14; the device code carries these as literals
+    assert 0.5 * (math.sqrt(3.0) - 1.0) == 0.36602540378443860
+    assert (3.0 - math.sqrt(3.0)) / 6.0 == 0.21132486540518713
+
+
+def test_y_cell_quirk_against_no_cache_restatement(oracle):
+    """The legacy 3D path reuses the first y of each y cell for the corner gradients.  Restate it
+    point by point with the single-point Perlin at the cell-start fraction and compare."""
+    L = oracle.lib()
+    case = load("ref_perlin.json")[0]
+    perm, abc = _perm_abc(case["terrain"][0]["state"])
+    a, b, c = [h2d(h) for h in case["terrain"][0]["state"]["abc"]]
+    for sy in (684.412 / 32768, 684.412 / 160 / 64, 0.7, 3.1):
+        x, z = 123.4567, -987.125
+        out = (ctypes.c_double * 17)()
+        L.bsfo_perlin3_column(perm, abc, ctypes.c_double(x), ctypes.c_double(sy), ctypes.c_double(z), 17, out)
+        # cell starts
+        starts, prev = [], None
+        for y in range(17):
+            v = y * sy + b
+            idx = math.floor(v) & 255
+            if y == 0 or idx != prev:
+                start = y
+            prev = idx
+            starts.append(start)
+        differs = sum(1 for y in range(17) if starts[y] != y)
+        # where the cell starts at y itself the column equals the single-point value
+        for y in range(17):
+            if starts[y] == y:
+                pt = L.bsfo_perlin3_point(perm, abc, ctypes.c_double(x), ctypes.c_double(y * sy),
+                                          ctypes.c_double(z))
+                assert d2h(out[y]) == d2h(plus0(pt))
+        if sy < 0.05:
+            assert differs > 0  # the long-cell octave exercises the reuse path
+
+
+# ---------------------------------------------------------------- coarse grid (composition)
+
+def test_coarse_grid_matches_reference(oracle):
+    L = oracle.lib()
+    S, Y = 61, 17
+    cells, cols = S * S * Y, S * S
+    cases = load("ref_coarse.json")
+    for case in cases[:8]:
+        arrs = {k: np.zeros(cells if k in ("n1", "n2", "n3") else cols) for k in
+                ("n1", "n2", "n3", "n4", "n5", "temp", "rain")}
+        signs = np.zeros(cells, dtype=np.uint8)
+        L.bsfo_coarse_arrays(ctypes.c_int64(case["seed"]), 0, 0,
+                             *[arrs[k].ctypes.data_as(ctypes.c_void_p) for k in
+                               ("n1", "n2", "n3", "n4", "n5", "temp", "rain")],
+                             signs.ctypes.data_as(ctypes.c_void_p))
+
+        def fnv(b):
+            h = 1469598103934665603
+            for x in b:
+                h = ((h ^ x) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+            return "%016x" % h
+
+        assert int(signs.sum()) == case["positives"]
+        assert fnv(signs.tobytes()) == case["signs"]
+        for k, key in (("n1", "noise1"), ("n2", "noise2"), ("n3", "noise3"), ("n4", "noise4"), ("n5", "noise5"),
+                       ("temp", "temp"), ("rain", "rain")):
+            assert fnv(arrs[k].astype("<f8").tobytes()) == case[key], (case["seed"], k)
+        assert L.bsfo_coarse_score_signs(signs.ctypes.data_as(ctypes.c_void_p)) == case["score"]
+
+
+def test_chunk_lattice_noise_matches_reference(oracle):
+    L = oracle.lib()
+    for case in load("ref_chunk00_noise.json"):
+        n1, n2, n3 = (np.zeros(425) for _ in range(3))
+        n4, n5 = np.zeros(25), np.zeros(25)
+        L.bsfo_chunk_noise(ctypes.c_int64(case["seed"]), 0, 0,
+                           *[a.ctypes.data_as(ctypes.c_void_p) for a in (n1, n2, n3, n4, n5)])
+        for arr, key in ((n1, "n1"), (n2, "n2"), (n3, "n3"), (n4, "n4"), (n5, "n5")):
+            assert [d2h(v) for v in arr] == case[key], key
+
+
+# ---------------------------------------------------------------- P19 gate
+
+def test_p19_matches_reference(oracle):
+    L = oracle.lib()
+    for row in load("ref_p19.json"):
+        scout, f88, f96, f104, f112, l88, w88, d88, t88, l96, w96, d96, t96, score, passes, extreme = row
+        f = oracle.P19Features(f88, f96, f104, f112, l88, w88, d88, t88, l96, w96, d96, t96)
+        got = L.bsfo_p19_score(scout, ctypes.byref(f))
+        assert d2h(got) == score
+        ext = L.bsfo_p19_extreme(ctypes.byref(f))
+        assert ext == extreme
+        assert int(bool(ext) or got >= 4.02) == passes
+
+
+# ---------------------------------------------------------------- upsampler closed forms
+
+def test_upsampler_lattice_identity_and_constants(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    lat = rng.normal(0.0, 50.0, 425)
+    dens = np.zeros(32768)
+    L.bsfo_upsample_density(lat.ctypes.data_as(ctypes.c_void_p), dens.ctypes.data_as(ctypes.c_void_p))
+    # blocks on lattice positions carry the lattice value exactly (no arithmetic applied yet)
+    for i in range(4):
+        for j in range(4):
+            for k in range(16):
+                assert dens[((4 * i) * 16 + 4 * j) * 128 + 8 * k] == lat[(i * 5 + j) * 17 + k]
+    # a constant field stays constant
+    c = np.full(425, -3.25)
+    L.bsfo_upsample_density(c.ctypes.data_as(ctypes.c_void_p), dens.ctypes.data_as(ctypes.c_void_p))
+    assert np.all(dens == -3.25)
+    # an affine field with power-of-two steps is exact everywhere
+    a = np.zeros(425)
+    for i in range(5):
+        for j in range(5):
+            for k in range(17):
+                a[(i * 5 + j) * 17 + k] = 4.0 * i - 8.0 * j + 16.0 * k - 100.0
+    L.bsfo_upsample_density(a.ctypes.data_as(ctypes.c_void_p), dens.ctypes.data_as(ctypes.c_void_p))
+    for x in range(16):
+        for z in range(16):
+            for y in range(128):
+                assert dens[(x * 16 + z) * 128 + y] == 1.0 * x - 2.0 * z + 2.0 * y - 100.0
+
+
+def test_upsampler_incremental_order(oracle):
+    """Straight restatement of the incremental adds for a random lattice: the incremental result
+    differs from a closed-form lerp in the last bits, so this pins the order."""
+    L = oracle.lib()
+    rng = np.random.default_rng(11)
+    lat = rng.normal(0.0, 3.0, 425)
+    dens = np.zeros(32768)
+    L.bsfo_upsample_density(lat.ctypes.data_as(ctypes.c_void_p), dens.ctypes.data_as(ctypes.c_void_p))
+    n = lambda i, j, k: lat[(i * 5 + j) * 17 + k]  # noqa: E731
+    lerp_diff = 0
+    for (i1, j1, k1) in [(0, 0, 0), (1, 2, 7), (3, 3, 15), (2, 1, 9)]:
+        d1, d2, d3, d4 = n(i1, j1, k1), n(i1, j1 + 1, k1), n(i1 + 1, j1, k1), n(i1 + 1, j1 + 1, k1)
+        d5 = (n(i1, j1, k1 + 1) - d1) * 0.125
+        d6 = (n(i1, j1 + 1, k1 + 1) - d2) * 0.125
+        d7 = (n(i1 + 1, j1, k1 + 1) - d3) * 0.125
+        d8 = (n(i1 + 1, j1 + 1, k1 + 1) - d4) * 0.125
+        for l1 in range(8):
+            d10, d11 = d1, d2
+            d12, d13 = (d3 - d1) * 0.25, (d4 - d2) * 0.25
+            for i2 in range(4):
+                d15 = d10
+                d16 = (d11 - d10) * 0.25
+                for k2 in range(4):
+                    x, z, y = i1 * 4 + i2, j1 * 4 + k2, k1 * 8 + l1
+                    assert dens[(x * 16 + z) * 128 + y] == d15
+                    exact = n(i1, j1, k1)  # not used; closed form comparison below
+                    d15 += d16
+                d10 += d12
+                d11 += d13
+            d1 += d5
+            d2 += d6
+            d3 += d7
+            d4 += d8
+    del lerp_diff, exact
+
+
+# ---------------------------------------------------------------- island flood fill
+
+def _flood_bruteforce(stone, size_x, size_z, cands, p, min_wx, min_wz):
+    """Union-find restatement of the component statistics and isMatch/compare rules."""
+    import itertools
+    Y = 128
+    parent = {}
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    cells = [(x, y, z) for x in range(size_x) for y in range(68, Y) for z in range(size_z)
+             if stone[(x * Y + y) * size_z + z]]
+    for c in cells:
+        parent[c] = c
+    for (x, y, z) in cells:
+        for dx, dy, dz in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            nb = (x + dx, y + dy, z + dz)
+            if nb in parent:
+                ra, rb = find((x, y, z)), find(nb)
+                if ra != rb:
+                    parent[ra] = rb
+    comps = {}
+    for c in cells:
+        comps.setdefault(find(c), []).append(c)
+    best, best_key = None, None
+    seen = set()
+    for idx in cands:
+        x = idx // (Y * size_z)
+        rem = idx - x * Y * size_z
+        y, z = rem // size_z, rem % size_z
+        if not stone[idx]:
+            continue
+        r = find((x, y, z))
+        if r in seen:
+            continue
+        seen.add(r)
+        comp = comps[r]
+        blocks = len(comp)
+        colset = {(cx, cz) for cx, _, cz in comp}
+        columns = len(colset)
+        xs = [cx for cx, _, _ in comp]
+        zs = [cz for _, _, cz in comp]
+        ys = [cy for _, cy, _ in comp]
+        bottom = any(cy == 68 and stone[(cx * Y + 67) * size_z + cz] for cx, cy, cz in comp)
+        width, depth = max(xs) - min(xs) + 1, max(zs) - min(zs) + 1
+        ok = (blocks >= p.min_blocks and columns >= p.min_columns and max(ys) >= p.min_y_for_match and not bottom
+              and width >= p.min_width and depth >= p.min_depth and blocks / max(1, columns) >= p.min_avg_thickness)
+        if not ok:
+            continue
+        key = (blocks, columns, width * depth)
+        if best is None or key > best_key:
+            best, best_key = (blocks, columns, min(xs) + min_wx, max(xs) + min_wx, min(zs) + min_wz,
+                              max(zs) + min_wz, min(ys), max(ys)), key
+    del itertools
+    return best
+
+
+def test_island_flood_against_union_find(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    sx = sz = 16
+    Y = 128
+    for trial in range(6):
+        stone = np.zeros(sx * Y * sz, dtype=np.uint8)
+        # random blobs of stone above y=68 plus a ground layer at 60..70 in some trials
+        for _ in range(5):
+            cx, cy, cz = rng.integers(0, sx), rng.integers(72, 120), rng.integers(0, sz)
+            r = rng.integers(2, 5)
+            for x in range(max(0, cx - r), min(sx, cx + r)):
+                for z in range(max(0, cz - r), min(sz, cz + r)):
+                    for y in range(cy - 2, cy + 2):
+                        stone[(x * Y + y) * sz + z] = 1
+        if trial % 2:
+            for x in range(sx):
+                for z in range(4):
+                    for y in range(60, 75):
+                        stone[(x * Y + y) * sz + z] = 1
+        cands = [(x * Y + y) * sz + z for x in (4, 12) for z in (4, 12) for y in range(68, Y)
+                 if stone[(x * Y + y) * sz + z] and not stone[(x * Y + y - 1) * sz + z]]
+        p = oracle.island_params(0, 0, 1, 1, min_blocks=10, min_columns=4, min_width=2, min_depth=2,
+                                 min_avg_thickness=1.0)
+        h = oracle.IslandHit()
+        arr = (ctypes.c_int32 * max(1, len(cands)))(*cands)
+        ok = L.bsfo_island_from_stone(ctypes.c_int64(7), stone.ctypes.data_as(ctypes.c_void_p), sx, sz, 32, -48, arr,
+                                      len(cands), ctypes.byref(p), ctypes.byref(h))
+        ref = _flood_bruteforce(stone, sx, sz, cands, p, 32, -48)
+        if ref is None:
+            assert ok == 0
+        else:
+            assert ok == 1
+            assert (h.blocks, h.columns, h.min_x, h.max_x, h.min_z, h.max_z, h.min_y, h.max_y) == ref
+
+
+def test_island_gates_hand_built(oracle):
+    """A floating slab matches; the same slab grounded by one pillar to y=67 does not; a slab at
+    the window edge reports touchesSideBorder."""
+    L = oracle.lib()
+    sx = sz = 32
+    Y = 128
+
+    def run(stone):
+        cands = [(x * Y + y) * sz + z for x in (4, 12, 20, 28) for z in (4, 12, 20, 28) for y in range(68, Y)
+                 if stone[(x * Y + y) * sz + z] and not stone[(x * Y + y - 1) * sz + z]]
+        p = oracle.island_params(0, 0, 2, 2, min_blocks=200)
+        h = oracle.IslandHit()
+        arr = (ctypes.c_int32 * max(1, len(cands)))(*cands)
+        ok = L.bsfo_island_from_stone(ctypes.c_int64(1), stone.ctypes.data_as(ctypes.c_void_p), sx, sz, 0, 0, arr,
+                                      len(cands), ctypes.byref(p), ctypes.byref(h))
+        return ok, h
+
+    stone = np.zeros(sx * Y * sz, dtype=np.uint8)
+    for x in range(2, 16):
+        for z in range(2, 16):
+            for y in range(80, 84):
+                stone[(x * Y + y) * sz + z] = 1
+    ok, h = run(stone)
+    assert ok == 1 and h.blocks == 14 * 14 * 4 and h.columns == 196 and h.flags == 0
+    assert (h.min_x, h.max_x, h.min_z, h.max_z, h.min_y, h.max_y) == (2, 15, 2, 15, 80, 83)
+    assert (h.center_x, h.center_z) == (9, 9)  # Math.round(8.5) = 9
+    grounded = stone.copy()
+    for y in range(67, 80):
+        grounded[(6 * Y + y) * sz + 6] = 1
+    ok, _ = run(grounded)
+    assert ok == 0
+    edge = stone.copy()
+    for x in range(0, 2):
+        for z in range(2, 16):
+            for y in range(80, 84):
+                edge[(x * Y + y) * sz + z] = 1
+    ok, h = run(edge)
+    assert ok == 1 and h.flags == 1
+
+
+def test_island_stage1_predicate_is_superset_of_hits(oracle):
+    p = oracle.island_params(min_blocks=200)
+    hits = oracle.run_island(1600, 32, p, threads=8)
+    assert any(h[0] == 1611 for h in hits)
+    for h in hits:
+        assert oracle.island_has_candidate(h[0], p)
+
+
+def test_scout_statistic_is_octave_zero_sum(oracle):
+    """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
+    generator's octave-0 values (0.0 + v), index order x, z, y."""
+    L = oracle.lib()
+    perm = (ctypes.c_uint8 * 256)()
+    abc = (ctypes.c_double * 3)()
+    for seed in (0, 77, -3):
+        L.bsfo_perlin_state(ctypes.c_int64(seed), 0, perm, abc)
+        s = 0.0
+        for x in range(5):
+            for z in range(5):
+                col = (ctypes.c_double * 17)()
+                L.bsfo_perlin3_column(perm, abc, ctypes.c_double(x * 684.412), ctypes.c_double(684.412),
+                                      ctypes.c_double(z * 684.412), 17, col)
+                for v in col:
+                    s += v
+        assert d2h(s) == d2h(oracle.scout_statistic(seed))
