@@ -17,6 +17,7 @@
 #include "../../include/bsf.h"
 #include "bsf_host.hpp"
 #include "bsf_kernels.hpp"
+#include "bsf_stage0.hpp"
 
 namespace bsf {
 
@@ -336,27 +337,6 @@ int64_t island_pipeline(const int64_t* h_seeds, int64_t first, int64_t count, co
     return static_cast<int64_t>(hits.size());
 }
 
-template <class F>
-int guarded(F&& f) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    try {
-        f();
-        return BSF_OK;
-    } catch (const ArgError& e) {
-        set_error(e.what());
-        return BSF_ERR_ARG;
-    } catch (const CapacityError& e) {
-        set_error(e.what());
-        return BSF_ERR_CAPACITY;
-    } catch (const HipError& e) {
-        set_error(e.what());
-        return BSF_ERR_HIP;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return BSF_ERR_STATE;
-    }
-}
-
 }  // namespace bsf
 
 using namespace bsf;
@@ -364,7 +344,7 @@ using namespace bsf;
 extern "C" {
 
 int bsf_init(int device) {
-    return guarded([&] {
+    return guarded_call([&] {
         int n = 0;
         check(hipGetDeviceCount(&n), "hipGetDeviceCount");
         if (device < 0 || device >= n) throw ArgError("no such HIP device");
@@ -378,6 +358,7 @@ int bsf_init(int device) {
         g_ctx.arch = prop.gcnArchName;
         g_ctx.global_mem = prop.totalGlobalMem;
         check(upload_tables(), "upload tables");
+        check(upload_stage0_tables(), "upload stage0 tables");
         // the island flood keeps two window bitsets in LDS (up to 8 chunks in x: ~121 KiB)
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(island_flood_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024),
@@ -400,7 +381,7 @@ void bsf_shutdown(void) {
 const char* bsf_last_error(void) { return g_last_error.c_str(); }
 
 int bsf_timing_enable(int on) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         resolve_timers();
         g_timing = on != 0;
@@ -408,7 +389,7 @@ int bsf_timing_enable(int on) {
 }
 
 int bsf_timing_reset(void) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         resolve_timers();
         g_timer_stats.clear();
@@ -416,7 +397,7 @@ int bsf_timing_reset(void) {
 }
 
 int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uint64_t* units) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (!name || !total_ms || !launches || !units) throw ArgError("null stats argument");
         resolve_timers();
@@ -429,7 +410,7 @@ int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uin
 }
 
 int bsf_device_banner(char* buf, size_t cap) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (!buf || cap == 0) throw ArgError("null banner buffer");
         std::snprintf(buf, cap, "GPU: %s\nArchitecture: %s\nCompute units: %d\nGlobal memory: %g GiB\n",
@@ -439,7 +420,7 @@ int bsf_device_banner(char* buf, size_t cap) {
 }
 
 int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t cx, int32_t cz, double* out) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (count < 0 || (count > 0 && (!seeds || !out))) throw ArgError("bad density arguments");
         if (count == 0) return;
@@ -458,7 +439,7 @@ int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t cx, int32_t cz
 }
 
 int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t cx, int32_t cz, double* d_out, void* stream) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (count < 0 || (count > 0 && (!d_seeds || !d_out))) throw ArgError("bad density arguments");
         run_density(d_seeds, count, cx, cz, d_out, stream ? static_cast<hipStream_t>(stream) : g_ctx.stream);
@@ -466,7 +447,7 @@ int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t cx, i
 }
 
 int bsf_scout_range(int64_t first, int64_t count, double thr, int64_t* out, int64_t cap, int64_t* n_found) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad scout arguments");
         hipStream_t s = g_ctx.stream;
@@ -496,7 +477,7 @@ int bsf_scout_range(int64_t first, int64_t count, double thr, int64_t* out, int6
 
 int bsf_scout_range_device(int64_t first, int64_t count, double thr, int64_t* d_out, int64_t cap,
                            unsigned long long* d_count, void* stream) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         if (count < 0 || cap < 0 || !d_out || !d_count) throw ArgError("bad scout arguments");
         run_scout(first, count, thr, d_out, cap, d_count, stream ? static_cast<hipStream_t>(stream) : g_ctx.stream);
@@ -513,7 +494,7 @@ static void finish_hits(std::vector<bsf_island_hit>& hits, bsf_island_hit* out, 
 
 int bsf_island_range(int64_t first, int64_t count, const bsf_island_params* p, bsf_island_hit* out, int64_t cap,
                      int64_t* n_found) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         validate_island_params(p);
         if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad island arguments");
@@ -525,7 +506,7 @@ int bsf_island_range(int64_t first, int64_t count, const bsf_island_params* p, b
 
 int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_params* p, bsf_island_hit* out, int64_t cap,
                      int64_t* n_found) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         validate_island_params(p);
         if (count < 0 || cap < 0 || (count > 0 && !seeds) || (cap > 0 && !out) || !n_found)
@@ -538,7 +519,7 @@ int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_param
 
 int bsf_island_stage1_range(int64_t first, int64_t count, const bsf_island_params* p, int64_t* out, int64_t cap,
                             int64_t* n_found) {
-    return guarded([&] {
+    return guarded_call([&] {
         require_init();
         validate_island_params(p);
         if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad stage1 arguments");

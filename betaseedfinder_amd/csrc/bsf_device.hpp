@@ -78,6 +78,13 @@ struct LanePerm {
     __device__ __forceinline__ void set(int k, uint8_t v) const { base[off(static_cast<uint32_t>(k) & 255u)] = v; }
 };
 
+// A permutation shared by a whole workgroup (all lanes evaluate different columns of one
+// generator): 256 bytes in LDS, plain byte index.
+struct SharedPerm {
+    const uint8_t* t;
+    __device__ __forceinline__ int get(int k) const { return t[k & 255]; }
+};
+
 // A Perlin (or simplex) generator build: NoiseGeneratorPerlin173.reseed (:63-78) and
 // NoiseGenerator2173.reseed (:27-48) draw the same sequence: three nextDouble()*256
 // offsets, then a Fisher-Yates shuffle j = nextInt(256 - i) + i.
@@ -180,7 +187,8 @@ __device__ __forceinline__ Axis axis(double v) {
 
 // The 2D legacy column value (yLen == 1 branch of a(...), :188-223) at an already
 // offset x / z: the y index is 0 and y never enters.
-__device__ __forceinline__ double perlin2(LanePerm p, const Axis& ax, const Axis& az) {
+template <class Perm>
+__device__ __forceinline__ double perlin2(Perm p, const Axis& ax, const Axis& az) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
     const int p00 = p.get(xp0) + az.i;
@@ -238,8 +246,8 @@ struct YAxis {
 
 // One 3D octave at one lattice column, all levels [Y0, Y1): out[k] = value * weight
 // (the per-point addend of :281-282).
-template <int Y0, int Y1>
-__device__ __forceinline__ void perlin3_column(LanePerm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
+template <int Y0, int Y1, class Perm>
+__device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
                                                double weight, double* out) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
@@ -290,7 +298,8 @@ __device__ __forceinline__ double simplex_grad(int g, double x, double z) {
         default: return 0.0 * x + -1.0 * z;
     }
 }
-__device__ __forceinline__ double simplex2(LanePerm p, double d5, double d6) {
+template <class Perm>
+__device__ __forceinline__ double simplex2(Perm p, double d5, double d6) {
     const double f = kSimplexF, g = kSimplexG;
     const double d7 = (d5 + d6) * f;
     const int j1 = simplex_floor(d5 + d7);

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -72,6 +73,30 @@ struct LaunchTimer {
 void resolve_timers();
 
 void require_init();
+
+extern std::mutex g_mutex;
+
+// Runs f under the library lock and maps exceptions to the C ABI's status codes.
+template <class F>
+int guarded_call(F&& f) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    try {
+        f();
+        return BSF_OK;
+    } catch (const ArgError& e) {
+        set_error(e.what());
+        return BSF_ERR_ARG;
+    } catch (const CapacityError& e) {
+        set_error(e.what());
+        return BSF_ERR_CAPACITY;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return BSF_ERR_HIP;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return BSF_ERR_STATE;
+    }
+}
 
 template <class K>
 int resident_blocks(K kernel, int block, size_t dyn_lds);

@@ -384,11 +384,64 @@ __device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats&
     return areaA > areaB;
 }
 
+// Bits of m lying in a run of consecutive ones of m that contains a bit of s (s within m):
+// the carry of m + s sweeps each run upward from its seed bits; the bit-reversed pass sweeps
+// downward.
+__device__ __forceinline__ unsigned long long zfill(unsigned long long s, unsigned long long m) {
+    const unsigned long long up = ((m + s) ^ m) & m;
+    const unsigned long long rm = __builtin_bitreverse64(m);
+    const unsigned long long rs = __builtin_bitreverse64(s);
+    const unsigned long long down = __builtin_bitreverse64(((rm + rs) ^ rm) & rm);
+    return up | down | s;
+}
+
+// Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
+// (x, y in [68, 127]) with z along the bits.  Thread x sweeps its rows up and down (y and z
+// reach the fixpoint in one pass); x neighbours propagate one step per round.
+__device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
+    const int x = threadIdx.x;
+    for (;;) {
+        int changed = 0;
+        if (x < nx) {
+            unsigned long long* row = comp + x * kLevels;
+            const unsigned long long* mrow = rem + x * kLevels;
+            const unsigned long long* left = x > 0 ? row - kLevels : nullptr;
+            const unsigned long long* right = x < nx - 1 ? row + kLevels : nullptr;
+            unsigned long long prev = 0;
+            for (int y = 0; y < kLevels; ++y) {
+                const unsigned long long m = mrow[y];
+                const unsigned long long c = row[y];
+                unsigned long long nb = c | prev;
+                if (left) nb |= left[y];
+                if (right) nb |= right[y];
+                const unsigned long long n = zfill(nb & m, m);
+                if (n != c) {
+                    row[y] = n;
+                    changed = 1;
+                }
+                prev = n;
+            }
+            prev = 0;
+            for (int y = kLevels - 1; y >= 0; --y) {
+                const unsigned long long m = mrow[y];
+                const unsigned long long c = row[y];
+                const unsigned long long n = zfill((c | prev) & m, m);
+                if (n != c) {
+                    row[y] = n;
+                    changed = 1;
+                }
+                prev = n;
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
 __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
-    __shared__ int s_changed;
     __shared__ CompStats s_cs;
     __shared__ unsigned long long s_zor;
+    __shared__ int s_total;
     const int tid = threadIdx.x;
     const int nx = a.p.chunks_x * 16;
     const int nz = a.p.chunks_z * 16;
@@ -404,8 +457,8 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
 
     for (int64_t sv = blockIdx.x; sv < nsurv; sv += gridDim.x) {
         const double* lat = a.lattice + static_cast<size_t>(sv) * nd * 8;
-        // 1. upsample rows y in [67, 127]
-        for (int r = tid; r < nx * (kLevels + 1); r += 256) {
+        // 1. upsample rows y in [67, 127] (generateBareTerrain's incremental chain)
+        for (int r = tid; r < nx * (kLevels + 1); r += blockDim.x) {
             const int x = r / (kLevels + 1);
             const int y = 67 + r % (kLevels + 1);
             const unsigned long long bits = stone_row(lat, x, y, a.p.chunks_z);
@@ -413,54 +466,50 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
             else S.rem[x * kLevels + (y - kFloorY)] = bits;
         }
         __syncthreads();
+        // 2. every component that reaches a y = 68 block with stone below touches the bottom
+        //    and can never match (isMatch, :3972-4003): grow them all at once from their ground
+        //    contacts and drop them, as Java's flood would mark them visited.
+        for (int r = tid; r < rows; r += blockDim.x) {
+            const int x = r / kLevels, y = r % kLevels;
+            S.comp[r] = y == 0 ? (S.rem[r] & S.st67[x]) : 0ULL;
+        }
+        __syncthreads();
+        grow_component(S.comp, S.rem, nx);
+        if (tid == 0) s_total = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int r = tid; r < rows; r += blockDim.x) {
+            const unsigned long long left = S.rem[r] & ~S.comp[r];
+            S.rem[r] = left;
+            mine += __popcll(left);
+        }
+        atomicAdd(&s_total, mine);
+        __syncthreads();
+        // 3. all remaining stone is floating; fewer blocks than min_blocks cannot hold a match
+        if (s_total < a.p.min_blocks) continue;
 
         bool have_best = false;
         CompStats best{};
-        // 2. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
+        // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
         for (int ch = 0; ch < a.p.chunks_x * a.p.chunks_z; ++ch) {
             const int cxi = ch / a.p.chunks_z, czi = ch % a.p.chunks_z;
             for (int sc = 0; sc < 4; ++sc) {
                 const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
                 const int z = czi * 16 + ((sc & 1) ? 12 : 4);
                 for (int y = kFloorY; y < 128; ++y) {
-                    // stone here and not below (Java reads the chunk's own blocks array)
                     const unsigned long long bitz = 1ULL << z;
+                    // stone not yet visited (visited or grounded blocks are no longer in rem)
                     const unsigned long long here = S.rem[x * kLevels + (y - kFloorY)];
-                    // a visited stone block is no longer in rem: Java skips it (visited != 0)
                     if ((here & bitz) == 0) continue;
-                    // below: the original stone mask; rem only loses bits of flooded components,
-                    // and a flooded component containing the block below would contain this one.
+                    // the block below is not stone: below y = 68 read the y = 67 row; above, a
+                    // stone block below would share this block's component, so it is still in rem
                     const unsigned long long below = y == kFloorY ? S.st67[x] : S.rem[x * kLevels + (y - 1 - kFloorY)];
                     if ((below & bitz) != 0) continue;
-                    // flood this component
-                    for (int r = tid; r < rows; r += 256) S.comp[r] = 0;
+                    for (int r = tid; r < rows; r += blockDim.x) S.comp[r] = 0;
                     __syncthreads();
                     if (tid == 0) S.comp[x * kLevels + (y - kFloorY)] = bitz;
                     __syncthreads();
-                    for (;;) {
-                        if (tid == 0) s_changed = 0;
-                        __syncthreads();
-                        int changed = 0;
-                        for (int r = tid; r < rows; r += 256) {
-                            const int rx = r / kLevels, ry = r % kLevels;
-                            const unsigned long long c = S.comp[r];
-                            unsigned long long nb = c | (c << 1) | (c >> 1);
-                            if (rx > 0) nb |= S.comp[r - kLevels];
-                            if (rx < nx - 1) nb |= S.comp[r + kLevels];
-                            if (ry > 0) nb |= S.comp[r - 1];
-                            if (ry < kLevels - 1) nb |= S.comp[r + 1];
-                            const unsigned long long nw = nb & S.rem[r];
-                            if (nw != c) {
-                                S.comp[r] = nw;
-                                changed = 1;
-                            }
-                        }
-                        if (changed) s_changed = 1;
-                        __syncthreads();
-                        const int any = s_changed;
-                        __syncthreads();
-                        if (!any) break;
-                    }
+                    grow_component(S.comp, S.rem, nx);
                     // statistics of the component
                     if (tid == 0) {
                         s_cs.blocks = 0; s_cs.columns = 0;
@@ -469,7 +518,7 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
                         s_zor = 0;
                     }
                     __syncthreads();
-                    for (int rx = tid; rx < nx; rx += 256) {
+                    for (int rx = tid; rx < nx; rx += blockDim.x) {
                         unsigned long long col = 0;
                         int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
                         for (int ry = 0; ry < kLevels; ++ry) {
@@ -499,15 +548,14 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
                             if (rx == 0 || rx == nx - 1) atomicOr(&s_cs.side, 1);
                         }
                     }
-                    // remove the component from the unvisited stone
-                    for (int r = tid; r < rows; r += 256) S.rem[r] &= ~S.comp[r];
+                    for (int r = tid; r < rows; r += blockDim.x) S.rem[r] &= ~S.comp[r];
                     __syncthreads();
                     CompStats cs = s_cs;
                     const unsigned long long zor = s_zor;
                     cs.minz = __ffsll(static_cast<long long>(zor)) - 1;
                     cs.maxz = 63 - __clzll(static_cast<long long>(zor));
                     if ((zor & 1ULL) || ((zor >> (nz - 1)) & 1ULL)) cs.side = 1;
-                    // isMatch, IslandSearchEngine.java:3972-4003 (world bounds = window bounds + offsets)
+                    // isMatch, IslandSearchEngine.java:3972-4003
                     const int width = cs.maxx - cs.minx + 1;
                     const int depth = cs.maxz - cs.minz + 1;
                     const bool match = cs.blocks >= a.p.min_blocks && cs.columns >= a.p.min_columns &&

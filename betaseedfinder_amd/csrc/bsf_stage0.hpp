@@ -1,0 +1,54 @@
+// bsf_stage0.hpp -- Stage0 worker kernel arguments (stage0stream / stream modes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bsf_device.hpp"
+
+namespace bsf {
+
+// Generator ids in the per-world cache: the 66 terrain generators that are evaluated, in
+// Random order with the two consumed 4-octave generators left out, then the 10 climate ones.
+constexpr int kGenN2 = 0, kGenN3 = 16, kGenN1 = 32, kGenN4 = 40, kGenN5 = 50;
+constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
+constexpr int kCoarseRows = 61 * 17;  // (x, y) rows of 61-bit z masks
+
+enum { S0_P20 = 0, S0_UPPER = 1 };
+
+struct GenCacheArgs {
+    const int64_t* seeds;
+    int64_t count;
+    uint32_t* perms;    // [world][kGenCount][64] dwords
+    double* offsets;    // [world][kGenCount][3]
+};
+
+struct S0Args {
+    const uint32_t* perms;
+    const double* offsets;
+    int centers;                 // 1 or 8
+    int center_dx[8], center_dz[8];  // chunk offsets of the centres
+    const int64_t* items;        // item = world * centers + center (nullptr: 0..n_items-1)
+    int64_t n_items;
+    uint8_t* upper_mask;         // [item][256]
+    double* d5;                  // [item][256]
+    double* d7;                  // [item][256]
+    int* p20;                    // [item]
+    const int* p20_gated;        // [item]
+    int* upper;                  // [item]
+    int* high;                   // [item]
+    uint8_t* highest;            // [list index][256]
+    unsigned long long* coarse_bits;  // [list index][kCoarseRows]
+    int* coarse_score;           // [list index]
+};
+
+__global__ void gen_cache_kernel(GenCacheArgs a);
+template <int MODE>
+__global__ void s0_upper_kernel(S0Args a);
+__global__ void s0_lower_kernel(S0Args a);
+__global__ void s0_coarse_signs_kernel(S0Args a);
+__global__ void s0_coarse_score_kernel(S0Args a);
+
+hipError_t upload_stage0_tables();
+
+}  // namespace bsf
