@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--batch-log2", type=int, default=23)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (traffic-counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,6 +103,13 @@ def main():
 
     bsf.init(0)
     L = bsf.lib()
+    if args.copy_mib:
+        import torch
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     start, min_blocks = island_workload()
     params = bsf.island_params(-2, -2, 4, 4, min_blocks=min_blocks)
     batch = 1 << args.batch_log2
