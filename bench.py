@@ -137,6 +137,7 @@ def main():
 
     seeds = args.steps * batch
     value = seeds / elapsed
+    cache_ms, _, _ = kstats("island_cache")
     s1_ms, s1_n, s1_units = kstats("island_stage1")
     s2_ms, s2_n, s2_units = kstats("island_stage2")
     fl_ms, fl_n, _ = kstats("island_flood")
@@ -166,7 +167,7 @@ def main():
             "min_island_blocks": min_blocks,
             "hits_in_timed_region": hits,
             "stage1_survivor_fraction": round(s2_units / max(1, s1_units), 5),
-            "kernel_ms": {"island_stage1": round(s1_ms, 3), "island_stage2": round(s2_ms, 3),
+            "kernel_ms": {"island_cache": round(cache_ms, 3), "island_stage1": round(s1_ms, 3), "island_stage2": round(s2_ms, 3),
                           "island_flood": round(fl_ms, 3)},
         },
         "roofline": {

@@ -31,7 +31,7 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["bsf_kernels.hip", "bsf_stage0.hip", "bsf_api.cpp", "bsf_stage0_api.cpp", "bsf_batch.cpp"]
+LIB_SOURCES = ["bsf_kernels.hip", "bsf_stage0.hip", "bsf_island.hip", "bsf_api.cpp", "bsf_stage0_api.cpp", "bsf_batch.cpp"]
 WORKER_SOURCES = ["worker_main.cpp"]
 
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <cstring>
 #include <mutex>
@@ -17,6 +18,7 @@
 #include "../../include/bsf.h"
 #include "bsf_host.hpp"
 #include "bsf_kernels.hpp"
+#include "bsf_island.hpp"
 #include "bsf_stage0.hpp"
 
 namespace bsf {
@@ -260,11 +262,12 @@ void run_island_stage2(const int64_t* d_surv, int64_t n, const bsf_island_params
     HostColumns cols = island_window_columns(p);
     cols.upload(g_ctx.cols_noise2, g_ctx.cols_dens2, s);
     const int nd = cols.dev.nd;
-    const int64_t sub = std::max<int64_t>(64, std::min<int64_t>(n, (int64_t(1) << 30) / (int64_t(nd) * 8 * 8)));
+    // enough survivors per launch to fill every CU with several waves (one survivor per lane)
+    const int64_t sub = std::max<int64_t>(64, std::min<int64_t>(n, (int64_t(1) << 35) / (int64_t(nd) * 8 * 8)));
     auto kern = lattice_kernel<8, 16, MODE_LATTICE>;
     const size_t vals = static_cast<size_t>(scratch_vals(8, cols.dev.nc, nd));
     const int nx = p.chunks_x * 16;
-    const size_t flood_lds = static_cast<size_t>(2 * nx * 60 + nx) * sizeof(unsigned long long);
+    const size_t flood_lds = static_cast<size_t>(2 * nx * 61 + nx) * sizeof(unsigned long long);
     for (int64_t off = 0; off < n; off += sub) {
         const int64_t m = std::min(sub, n - off);
         double* d_lat = static_cast<double*>(g_ctx.lattice.ensure(static_cast<size_t>(m) * nd * 8 * sizeof(double)));
@@ -299,8 +302,116 @@ void run_island_stage2(const int64_t* d_surv, int64_t n, const bsf_island_params
     }
 }
 
-// Whole pipeline over a host-visible seed source, batched.
-int64_t island_pipeline(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p,
+// Flood the n survivors whose window lattices are in d_lat.
+void run_flood(const int64_t* d_surv_seed, int64_t n, const double* d_lat, const bsf_island_params& p,
+               bsf_island_hit* d_hits, unsigned long long* d_nhits, int64_t hit_cap, int32_t window_index,
+               hipStream_t s) {
+    const int nx = p.chunks_x * 16;
+    const size_t flood_lds = static_cast<size_t>(2 * nx * 61 + nx) * sizeof(unsigned long long);
+    FloodArgs f{};
+    f.p = p;
+    f.seeds = d_surv_seed;
+    f.d_count = nullptr;
+    f.cap = n;
+    f.lattice = d_lat;
+    f.hits = d_hits;
+    f.hit_count = d_nhits;
+    f.hit_cap = hit_cap;
+    f.window_index = window_index;
+    const int fgrid = static_cast<int>(std::min<int64_t>(n, resident_blocks(island_flood_kernel, 256, flood_lds)));
+    LaunchTimer t("island_flood", s);
+    hipLaunchKernelGGL(island_flood_kernel, dim3(fgrid), dim3(256), flood_lds, s, f);
+    check(hipGetLastError(), "flood launch");
+    t.stop(static_cast<uint64_t>(n));
+}
+
+// Cached column-parallel pipeline: per chunk of seeds, one generator-cache build, Stage 1
+// over every seed, Stage 2 + flood over the survivors, all from the same cache.
+int64_t island_pipeline_cached(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p,
+                               std::vector<bsf_island_hit>& hits, bool stage1_only,
+                               std::vector<int64_t>* stage1_out, int32_t window_index) {
+    hipStream_t s = g_ctx.stream;
+    const int64_t chunk = int64_t(1) << 17;
+    const int nd = 25 * p.chunks_x * p.chunks_z;
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t n = std::min(chunk, count - off);
+        int64_t* d_seeds = nullptr;
+        if (h_seeds) {
+            d_seeds = static_cast<int64_t*>(g_ctx.seeds.ensure(static_cast<size_t>(n) * 8));
+            check(hipMemcpyAsync(d_seeds, h_seeds + off, static_cast<size_t>(n) * 8, hipMemcpyHostToDevice, s), "H2D seeds");
+        }
+        uint32_t* perms = static_cast<uint32_t*>(g_ctx.s0_a.ensure(static_cast<size_t>(n) * kGenCount * 256));
+        double* offs = static_cast<double*>(g_ctx.s0_b.ensure(static_cast<size_t>(n) * kGenCount * 3 * 8));
+        {
+            GenCacheArgs g{d_seeds, n, perms, offs, first + off};
+            int per_cu = 0;
+            check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gen_cache_kernel, 64, 0), "occupancy");
+            const int grid = static_cast<int>(std::min<int64_t>((n + 63) / 64, std::max(1, per_cu) * g_ctx.cu_count));
+            LaunchTimer t("island_cache", s);
+            hipLaunchKernelGGL(gen_cache_kernel, dim3(grid), dim3(64), 0, s, g);
+            check(hipGetLastError(), "gen cache launch");
+            t.stop(static_cast<uint64_t>(n));
+        }
+        int64_t* d_slot = static_cast<int64_t*>(g_ctx.survivors.ensure(static_cast<size_t>(n) * 16));
+        int64_t* d_sseed = d_slot + n;
+        unsigned long long* d_cnt = static_cast<unsigned long long*>(g_ctx.counters.ensure(256));
+        check(hipMemsetAsync(d_cnt, 0, 16, s), "memset counters");
+        IslandArgs a{};
+        a.p = p;
+        a.perms = perms;
+        a.offsets = offs;
+        a.seeds = d_seeds;
+        a.first_seed = first + off;
+        a.count = n;
+        a.surv_slot = d_slot;
+        a.surv_seed = d_sseed;
+        a.surv_count = d_cnt;
+        a.surv_cap = n;
+        {
+            LaunchTimer t("island_stage1", s);
+            hipLaunchKernelGGL(island_s1_kernel, dim3(static_cast<int>(std::min<int64_t>(n, resident_blocks(island_s1_kernel, 64, 0)))),
+                               dim3(64), 0, s, a);
+            check(hipGetLastError(), "stage1 launch");
+            t.stop(static_cast<uint64_t>(n));
+        }
+        unsigned long long nsurv = 0;
+        check(hipMemcpyAsync(&nsurv, d_cnt, 8, hipMemcpyDeviceToHost, s), "D2H survivors");
+        check(hipStreamSynchronize(s), "stage1 sync");
+        if (stage1_only) {
+            std::vector<int64_t> tmp(nsurv);
+            if (nsurv) check(hipMemcpy(tmp.data(), d_sseed, nsurv * 8, hipMemcpyDeviceToHost), "D2H survivors");
+            stage1_out->insert(stage1_out->end(), tmp.begin(), tmp.end());
+            continue;
+        }
+        if (nsurv == 0) continue;
+        double* d_lat = static_cast<double*>(g_ctx.lattice.ensure(static_cast<size_t>(nsurv) * nd * 8 * sizeof(double)));
+        a.list_slot = d_slot;
+        a.list_n = static_cast<int64_t>(nsurv);
+        a.lattice = d_lat;
+        {
+            LaunchTimer t("island_stage2", s);
+            hipLaunchKernelGGL(island_s2_kernel,
+                               dim3(static_cast<int>(std::min<int64_t>(a.list_n, resident_blocks(island_s2_kernel, 320, 0)))),
+                               dim3(320), 0, s, a);
+            check(hipGetLastError(), "stage2 launch");
+            t.stop(static_cast<uint64_t>(nsurv));
+        }
+        const int64_t hit_cap = std::max<int64_t>(1024, static_cast<int64_t>(nsurv));
+        bsf_island_hit* d_hits =
+                static_cast<bsf_island_hit*>(g_ctx.hits.ensure(static_cast<size_t>(hit_cap) * sizeof(bsf_island_hit)));
+        run_flood(d_sseed, static_cast<int64_t>(nsurv), d_lat, p, d_hits, d_cnt + 1, hit_cap, window_index, s);
+        unsigned long long nh = 0;
+        check(hipMemcpyAsync(&nh, d_cnt + 1, 8, hipMemcpyDeviceToHost, s), "D2H hits");
+        check(hipStreamSynchronize(s), "flood sync");
+        std::vector<bsf_island_hit> tmp(nh);
+        if (nh) check(hipMemcpy(tmp.data(), d_hits, nh * sizeof(bsf_island_hit), hipMemcpyDeviceToHost), "D2H hits");
+        hits.insert(hits.end(), tmp.begin(), tmp.end());
+    }
+    return static_cast<int64_t>(hits.size());
+}
+
+// Whole pipeline over a host-visible seed source, batched (one seed per lane throughout).
+int64_t island_pipeline_lanes(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p,
                         std::vector<bsf_island_hit>& hits, bool stage1_only, std::vector<int64_t>* stage1_out) {
     hipStream_t s = g_ctx.stream;
     const int64_t batch = int64_t(1) << 22;
@@ -335,6 +446,13 @@ int64_t island_pipeline(const int64_t* h_seeds, int64_t first, int64_t count, co
         hits.insert(hits.end(), tmp.begin(), tmp.end());
     }
     return static_cast<int64_t>(hits.size());
+}
+
+int64_t island_pipeline(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p,
+                        std::vector<bsf_island_hit>& hits, bool stage1_only, std::vector<int64_t>* stage1_out) {
+    const char* path = std::getenv("BSF_ISLAND_PATH");
+    if (path && std::string(path) == "lanes") return island_pipeline_lanes(h_seeds, first, count, p, hits, stage1_only, stage1_out);
+    return island_pipeline_cached(h_seeds, first, count, p, hits, stage1_only, stage1_out, 0);
 }
 
 }  // namespace bsf

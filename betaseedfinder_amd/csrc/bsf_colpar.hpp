@@ -1,0 +1,159 @@
+// bsf_colpar.hpp -- column-parallel evaluation from a per-world generator cache.
+//
+// A workgroup works on one world: each lane owns one lattice column (noise x/z) or one
+// climate sample, the world's generators are staged one at a time into LDS (256-byte
+// permutation + 3 offsets) and shared by every lane, and all octave accumulators live in
+// registers.  Because every lane evaluates the same generator, the Y-cell run structure of
+// an octave is the same on all lanes, so the legacy corner reuse is a uniform branch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bsf_device.hpp"
+
+namespace bsf {
+
+// Generator ids in the per-world cache: the 66 terrain generators that are evaluated, in
+// Random order with the two consumed 4-octave generators left out, then the 10 climate ones.
+constexpr int kGenN2 = 0, kGenN3 = 16, kGenN1 = 32, kGenN4 = 40, kGenN5 = 50;
+constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
+
+struct GenStage {
+    uint32_t perm[64];
+    double off[3];
+};
+
+// Stage generator g of the current world into LDS (every thread of the block must call this).
+__device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __restrict__ perms,
+                                                const double* __restrict__ offs, int g) {
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 64) st.perm[t] = perms[g * 64 + t];
+    if (t < 3) st.off[t] = offs[g * 3 + t];
+    __syncthreads();
+}
+
+template <int Y0, int Y1>
+struct ColAcc {
+    static constexpr int NY = Y1 - Y0;
+    double n1[NY], n2[NY], n3[NY];
+    double n4, n5;
+};
+
+template <int Y0, int Y1>
+__device__ __forceinline__ void colpar_3d(GenStage& st, const uint32_t* perms, const double* offs, int g0, int octaves,
+                                          double s_xz, double s_y, double X, double Z, bool active, double* acc) {
+    constexpr int NY = Y1 - Y0;
+    double amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        stage_generator(st, perms, offs, g0 + o);
+        if (active) {
+            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
+            const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
+            YAxis<Y0, Y1> ya;
+            ya.build(sy, st.off[1]);
+            // the same value on every lane: make the branch scalar
+            const uint32_t fresh = __builtin_amdgcn_readfirstlane(ya.fresh);
+            const Axis ax = axis(X * sx + st.off[0]);
+            const Axis az = axis(Z * sx + st.off[2]);
+            double v[NY];
+            perlin3_column_cells<Y0, Y1>(P, ax, az, ya, fresh, w, v);
+#pragma unroll
+            for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+        }
+        amp /= 2.0;
+    }
+}
+
+__device__ __forceinline__ double colpar_2d(GenStage& st, const uint32_t* perms, const double* offs, int g0,
+                                            int octaves, double s, double X, double Z, bool active) {
+    double acc = 0.0, amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        stage_generator(st, perms, offs, g0 + o);
+        if (active) {
+            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
+            const double sx = s * amp;
+            const double v = perlin2(P, axis(X * sx + st.off[0]), axis(Z * sx + st.off[2])) * (1.0 / amp);
+            acc = o == 0 ? 0.0 + v : acc + v;
+        }
+        amp /= 2.0;
+    }
+    return acc;
+}
+
+// One climate group (NoiseGeneratorOctaves2173.a, :29-51) for up to N samples per lane.
+template <int N>
+__device__ __forceinline__ void colpar_climate(GenStage& st, const uint32_t* perms, const double* offs, int g0,
+                                               int octaves, double start_scale, double octave_scale, const double* BX,
+                                               const double* BZ, int n, double* acc) {
+    const double d2 = start_scale / 1.5;
+    double d6 = 1.0, d7 = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        stage_generator(st, perms, offs, g0 + o);
+        SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
+        const double sc = d2 * d7;
+        const double w = 0.55 / d6;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (j < n) {
+                const double v = simplex2(P, BX[j] * sc + st.off[0], BZ[j] * sc + st.off[1]) * w;
+                acc[j] = o == 0 ? 0.0 + v : acc[j] + v;
+            }
+        }
+        d7 *= octave_scale;
+        d6 *= 0.5;
+    }
+}
+
+// Climate of up to N samples per lane: finished temperature / rain (WorldChunkManager173.java:85-130).
+template <int N>
+__device__ __forceinline__ void colpar_climate_all(GenStage& st, const uint32_t* perms, const double* offs,
+                                                   const double* BX, const double* BZ, int n, double* temp,
+                                                   double* rain) {
+    double t[N], r[N], c[N];
+    colpar_climate<N>(st, perms, offs, kGenTemp, 4, 0.02500000037252903, 0.25, BX, BZ, n, t);
+    colpar_climate<N>(st, perms, offs, kGenRain, 4, 0.05000000074505806, 0.3333333333333333, BX, BZ, n, r);
+    colpar_climate<N>(st, perms, offs, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, n, c);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < n) climate_finish(t[j], r[j], c[j], temp[j], rain[j]);
+}
+
+// The three 3D octave groups of one lattice column over levels [Y0, Y1) (noise2, noise3,
+// noise1: generateTerrainNoise's evaluation order, BetaTerrain173.java:678-737).
+template <int Y0, int Y1>
+struct Acc3 {
+    double n1[Y1 - Y0], n2[Y1 - Y0], n3[Y1 - Y0];
+};
+
+template <int Y0, int Y1>
+__device__ __forceinline__ void colpar_noise3(GenStage& st, const uint32_t* perms, const double* offs, double X,
+                                              double Z, bool active, Acc3<Y0, Y1>& a) {
+    colpar_3d<Y0, Y1>(st, perms, offs, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2);
+    colpar_3d<Y0, Y1>(st, perms, offs, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3);
+    colpar_3d<Y0, Y1>(st, perms, offs, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, a.n1);
+}
+
+// The two 2D octave groups (noise4, noise5) of one lattice column.
+__device__ __forceinline__ void colpar_noise2d(GenStage& st, const uint32_t* perms, const double* offs, double X,
+                                               double Z, bool active, double& n4, double& n5) {
+    n4 = colpar_2d(st, perms, offs, kGenN4, 10, 1.121, X, Z, active);
+    n5 = colpar_2d(st, perms, offs, kGenN5, 16, 200.0, X, Z, active);
+}
+
+template <int Y0, int Y1, bool WITH_2D>
+__device__ __forceinline__ void colpar_noise(GenStage& st, const uint32_t* perms, const double* offs, double X,
+                                             double Z, bool active, ColAcc<Y0, Y1>& a) {
+    Acc3<Y0, Y1> t;
+    colpar_noise3<Y0, Y1>(st, perms, offs, X, Z, active, t);
+#pragma unroll
+    for (int k = 0; k < Y1 - Y0; ++k) {
+        a.n1[k] = t.n1[k];
+        a.n2[k] = t.n2[k];
+        a.n3[k] = t.n3[k];
+    }
+    if (WITH_2D) colpar_noise2d(st, perms, offs, X, Z, active, a.n4, a.n5);
+}
+
+}  // namespace bsf

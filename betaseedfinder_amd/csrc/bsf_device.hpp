@@ -211,6 +211,7 @@ struct YAxis {
     double fd[Y1 - Y0];
     double cell_f[Y1 - Y0];
     double cell_fm1[Y1 - Y0];
+    uint32_t fresh;  // bit k: level Y0 + k starts the cell run inside [Y0, Y1) (always bit 0)
 
     __device__ __forceinline__ void build(double sy, double b) {
         // walk back from Y0 to the start of its cell (only levels below Y0 that share Y0's index)
@@ -227,6 +228,7 @@ struct YAxis {
             }
         }
         int prev_i = fl0 & 255;
+        fresh = 1u;
 #pragma unroll
         for (int k = 0; k < Y1 - Y0; ++k) {
             const int y = Y0 + k;
@@ -234,7 +236,10 @@ struct YAxis {
             const int fl = java_floor(v);
             const int idx = fl & 255;
             const double f = v - static_cast<double>(fl);
-            if (k > 0 && idx != prev_i) start_f = f;
+            if (k > 0 && idx != prev_i) {
+                start_f = f;
+                fresh |= 1u << k;
+            }
             prev_i = idx;
             yi[k] = idx;
             fd[k] = fade(f);
@@ -266,6 +271,39 @@ __device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axi
         const double q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
         const double q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
         const double q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        const double r0 = lerp(ya.fd[k], q00, q01);
+        const double r1 = lerp(ya.fd[k], q10, q11);
+        out[k] = lerp(az.fd, r0, r1) * weight;
+    }
+}
+
+// Same values as perlin3_column, with the corner lerps computed once per cell run as the
+// legacy method does (:254-276).  Meant for kernels where every lane shares one generator,
+// so the cell structure (ya.fresh) is uniform and the skipped work is really skipped.
+template <int Y0, int Y1, class Perm>
+__device__ __forceinline__ void perlin3_column_cells(Perm p, const Axis& ax, const Axis& az,
+                                                     const YAxis<Y0, Y1>& ya, uint32_t fresh, double weight,
+                                                     double* out) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+    double q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
+#pragma unroll
+    for (int k = 0; k < Y1 - Y0; ++k) {
+        if ((fresh >> k) & 1u) {
+            const int yi = ya.yi[k];
+            const double yf = ya.cell_f[k];
+            const double yfm1 = ya.cell_fm1[k];
+            const int p0 = xp0 + yi;
+            const int p00 = p.get(p0) + az.i;
+            const int p01 = p.get(p0 + 1) + az.i;
+            const int p1 = xp1 + yi;
+            const int p10 = p.get(p1) + az.i;
+            const int p11 = p.get(p1 + 1) + az.i;
+            q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        }
         const double r0 = lerp(ya.fd[k], q00, q01);
         const double r1 = lerp(ya.fd[k], q10, q11);
         out[k] = lerp(az.fd, r0, r1) * weight;

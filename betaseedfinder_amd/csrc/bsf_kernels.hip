@@ -300,9 +300,10 @@ __global__ __launch_bounds__(64) void scout_kernel(ScoutArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int kFloorY = 68;
 constexpr int kLevels = 128 - kFloorY;  // 60 rows per x
+constexpr int kRowStride = kLevels + 1; // 61: odd stride spreads the x rows over all LDS banks
 
 struct FloodShared {
-    unsigned long long* rem;   // stone not yet visited, [x * kLevels + (y - 68)]
+    unsigned long long* rem;   // stone not yet visited, [x * kRowStride + (y - 68)]
     unsigned long long* comp;  // current component
     unsigned long long* st67;  // stone at y = 67 per x
 };
@@ -403,10 +404,10 @@ __device__ void grow_component(unsigned long long* comp, const unsigned long lon
     for (;;) {
         int changed = 0;
         if (x < nx) {
-            unsigned long long* row = comp + x * kLevels;
-            const unsigned long long* mrow = rem + x * kLevels;
-            const unsigned long long* left = x > 0 ? row - kLevels : nullptr;
-            const unsigned long long* right = x < nx - 1 ? row + kLevels : nullptr;
+            unsigned long long* row = comp + x * kRowStride;
+            const unsigned long long* mrow = rem + x * kRowStride;
+            const unsigned long long* left = x > 0 ? row - kRowStride : nullptr;
+            const unsigned long long* right = x < nx - 1 ? row + kRowStride : nullptr;
             unsigned long long prev = 0;
             for (int y = 0; y < kLevels; ++y) {
                 const unsigned long long m = mrow[y];
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
     const int tid = threadIdx.x;
     const int nx = a.p.chunks_x * 16;
     const int nz = a.p.chunks_z * 16;
-    const int rows = nx * kLevels;
+    const int rows = nx * kRowStride;
     FloodShared S;
     S.rem = flood_lds;
     S.comp = flood_lds + rows;
@@ -462,15 +463,19 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
             const int x = r / (kLevels + 1);
             const int y = 67 + r % (kLevels + 1);
             const unsigned long long bits = stone_row(lat, x, y, a.p.chunks_z);
-            if (y == 67) S.st67[x] = bits;
-            else S.rem[x * kLevels + (y - kFloorY)] = bits;
+            if (y == 67) {
+                S.st67[x] = bits;
+                S.rem[x * kRowStride + kLevels] = 0;  // padding slot of the x row
+            } else {
+                S.rem[x * kRowStride + (y - kFloorY)] = bits;
+            }
         }
         __syncthreads();
         // 2. every component that reaches a y = 68 block with stone below touches the bottom
         //    and can never match (isMatch, :3972-4003): grow them all at once from their ground
         //    contacts and drop them, as Java's flood would mark them visited.
         for (int r = tid; r < rows; r += blockDim.x) {
-            const int x = r / kLevels, y = r % kLevels;
+            const int x = r / kRowStride, y = r % kRowStride;
             S.comp[r] = y == 0 ? (S.rem[r] & S.st67[x]) : 0ULL;
         }
         __syncthreads();
@@ -499,15 +504,15 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
                 for (int y = kFloorY; y < 128; ++y) {
                     const unsigned long long bitz = 1ULL << z;
                     // stone not yet visited (visited or grounded blocks are no longer in rem)
-                    const unsigned long long here = S.rem[x * kLevels + (y - kFloorY)];
+                    const unsigned long long here = S.rem[x * kRowStride + (y - kFloorY)];
                     if ((here & bitz) == 0) continue;
                     // the block below is not stone: below y = 68 read the y = 67 row; above, a
                     // stone block below would share this block's component, so it is still in rem
-                    const unsigned long long below = y == kFloorY ? S.st67[x] : S.rem[x * kLevels + (y - 1 - kFloorY)];
+                    const unsigned long long below = y == kFloorY ? S.st67[x] : S.rem[x * kRowStride + (y - 1 - kFloorY)];
                     if ((below & bitz) != 0) continue;
                     for (int r = tid; r < rows; r += blockDim.x) S.comp[r] = 0;
                     __syncthreads();
-                    if (tid == 0) S.comp[x * kLevels + (y - kFloorY)] = bitz;
+                    if (tid == 0) S.comp[x * kRowStride + (y - kFloorY)] = bitz;
                     __syncthreads();
                     grow_component(S.comp, S.rem, nx);
                     // statistics of the component
@@ -522,7 +527,7 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
                         unsigned long long col = 0;
                         int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
                         for (int ry = 0; ry < kLevels; ++ry) {
-                            const unsigned long long c = S.comp[rx * kLevels + ry];
+                            const unsigned long long c = S.comp[rx * kRowStride + ry];
                             if (c) {
                                 col |= c;
                                 blk += __popcll(c);
@@ -544,7 +549,7 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
                                       static_cast<unsigned long long>(static_cast<long long>(ncol) * minWZ +
                                                                       sum_of_bit_positions(col)));
                             atomicOr(&s_zor, col);
-                            if ((S.comp[rx * kLevels] & S.st67[rx]) != 0) atomicOr(&s_cs.bottom, 1);
+                            if ((S.comp[rx * kRowStride] & S.st67[rx]) != 0) atomicOr(&s_cs.bottom, 1);
                             if (rx == 0 || rx == nx - 1) atomicOr(&s_cs.side, 1);
                         }
                     }

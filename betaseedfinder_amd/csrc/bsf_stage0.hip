@@ -61,7 +61,8 @@ __global__ __launch_bounds__(64) void gen_cache_kernel(GenCacheArgs a) {
     for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
         const int64_t w = (grp << 6) + lane;
         const bool active = w < a.count;
-        const int64_t seed = a.seeds[active ? w : a.count - 1];
+        const int64_t idx = active ? w : a.count - 1;
+        const int64_t seed = a.seeds != nullptr ? a.seeds[idx] : a.first_seed + idx;
         uint32_t* perms = a.perms + static_cast<size_t>(active ? w : 0) * kGenCount * 64;
         double* offs = a.offsets + static_cast<size_t>(active ? w : 0) * kGenCount * 3;
         Lcg r;
@@ -94,105 +95,23 @@ __global__ __launch_bounds__(64) void gen_cache_kernel(GenCacheArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// column evaluation from the cache
+// column evaluation from the cache (bsf_colpar.hpp), coarse climate semantics
 // ---------------------------------------------------------------------------
-struct GenStage {
-    uint32_t perm[64];
-    double off[3];
+// All five octave groups and the climate of one coarse-semantics column: climate sampled at
+// block coarse*4 + 2 (BetaTerrain173.java:2524-2584).
+template <int Y0, int Y1>
+struct S0Column {
+    ColAcc<Y0, Y1> acc;
+    double temp, rain;
 };
 
-// Stage generator g of a world into LDS (the whole block must call this).
-__device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __restrict__ perms,
-                                                const double* __restrict__ offs, int g) {
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < 64) st.perm[t] = perms[g * 64 + t];
-    if (t < 3) st.off[t] = offs[g * 3 + t];
-    __syncthreads();
-}
-
-// Column accumulators of one lane over levels [Y0, Y1).
-template <int Y0, int Y1>
-struct ColAcc {
-    static constexpr int NY = Y1 - Y0;
-    double n1[NY], n2[NY], n3[NY];
-    double n4, n5, t, r, c;
-};
-
-template <int Y0, int Y1>
-__device__ __forceinline__ void accumulate_3d(GenStage& st, const uint32_t* perms, const double* offs, int g0,
-                                              int octaves, double s_xz, double s_y, double X, double Z, bool active,
-                                              double* acc) {
-    constexpr int NY = Y1 - Y0;
-    double amp = 1.0;
-    for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
-        if (active) {
-            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
-            const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
-            YAxis<Y0, Y1> ya;
-            ya.build(sy, st.off[1]);
-            const Axis ax = axis(X * sx + st.off[0]);
-            const Axis az = axis(Z * sx + st.off[2]);
-            double v[NY];
-            perlin3_column<Y0, Y1>(P, ax, az, ya, w, v);
-#pragma unroll
-            for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
-        }
-        amp /= 2.0;
-    }
-}
-
-__device__ __forceinline__ double accumulate_2d(GenStage& st, const uint32_t* perms, const double* offs, int g0,
-                                                int octaves, double s, double X, double Z, bool active) {
-    double acc = 0.0, amp = 1.0;
-    for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
-        if (active) {
-            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
-            const double sx = s * amp;
-            const double v = perlin2(P, axis(X * sx + st.off[0]), axis(Z * sx + st.off[2])) * (1.0 / amp);
-            acc = o == 0 ? 0.0 + v : acc + v;
-        }
-        amp /= 2.0;
-    }
-    return acc;
-}
-
-__device__ __forceinline__ double accumulate_climate(GenStage& st, const uint32_t* perms, const double* offs, int g0,
-                                                     int octaves, double start_scale, double octave_scale, double BX,
-                                                     double BZ, bool active) {
-    const double d2 = start_scale / 1.5;
-    double d6 = 1.0, d7 = 1.0, acc = 0.0;
-    for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
-        if (active) {
-            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
-            const double sc = d2 * d7;
-            const double v = simplex2(P, BX * sc + st.off[0], BZ * sc + st.off[1]) * (0.55 / d6);
-            acc = o == 0 ? 0.0 + v : acc + v;
-        }
-        d7 *= octave_scale;
-        d6 *= 0.5;
-    }
-    return acc;
-}
-
-// All five octave groups and the climate of one coarse-semantics column (climate sampled at
-// block coarse*4 + 2, BetaTerrain173.java:2524-2584).
 template <int Y0, int Y1, bool WITH_SHAPE>
 __device__ __forceinline__ void evaluate_cached_column(GenStage& st, const uint32_t* perms, const double* offs,
-                                                       double X, double Z, bool active, ColAcc<Y0, Y1>& a) {
-    accumulate_3d<Y0, Y1>(st, perms, offs, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2);
-    accumulate_3d<Y0, Y1>(st, perms, offs, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3);
-    accumulate_3d<Y0, Y1>(st, perms, offs, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, a.n1);
+                                                       double X, double Z, bool active, S0Column<Y0, Y1>& c) {
+    colpar_noise<Y0, Y1, WITH_SHAPE>(st, perms, offs, X, Z, active, c.acc);
     if (WITH_SHAPE) {
-        a.n4 = accumulate_2d(st, perms, offs, kGenN4, 10, 1.121, X, Z, active);
-        a.n5 = accumulate_2d(st, perms, offs, kGenN5, 16, 200.0, X, Z, active);
         const double BX = X * 4.0 + 2.0, BZ = Z * 4.0 + 2.0;
-        a.t = accumulate_climate(st, perms, offs, kGenTemp, 4, 0.02500000037252903, 0.25, BX, BZ, active);
-        a.r = accumulate_climate(st, perms, offs, kGenRain, 4, 0.05000000074505806, 0.3333333333333333, BX, BZ, active);
-        a.c = accumulate_climate(st, perms, offs, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, active);
+        colpar_climate_all<1>(st, perms, offs, &BX, &BZ, active ? 1 : 0, &c.temp, &c.rain);
     }
 }
 
@@ -256,12 +175,11 @@ __global__ __launch_bounds__(256) void s0_upper_kernel(S0Args a) {
         const double X = -28.0 + static_cast<double>((col >> 4) * 4 + a.center_dx[center] * 4);
         const double Z = -28.0 + static_cast<double>((col & 15) * 4 + a.center_dz[center] * 4);
         if (tid == 0) s_count = 0;
-        ColAcc<11, 16> acc;
-        evaluate_cached_column<11, 16, true>(st, perms, offs, X, Z, active, acc);
+        S0Column<11, 16> cc;
+        evaluate_cached_column<11, 16, true>(st, perms, offs, X, Z, active, cc);
+        const ColAcc<11, 16>& acc = cc.acc;
         if (active) {
-            double temp, rain;
-            climate_finish(acc.t, acc.r, acc.c, temp, rain);
-            const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
+            const ColumnShape cs = column_shape(cc.temp, cc.rain, acc.n4, acc.n5);
             uint8_t mask = 0;
 #pragma unroll
             for (int k = 0; k < 5; ++k)
@@ -308,8 +226,9 @@ __global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
         const int col = active ? s_active[tid] : 0;
         const double X = -28.0 + static_cast<double>((col >> 4) * 4 + a.center_dx[center] * 4);
         const double Z = -28.0 + static_cast<double>((col & 15) * 4 + a.center_dz[center] * 4);
-        ColAcc<0, 11> acc;
-        evaluate_cached_column<0, 11, false>(st, perms, offs, X, Z, active, acc);
+        S0Column<0, 11> cc;
+        evaluate_cached_column<0, 11, false>(st, perms, offs, X, Z, active, cc);
+        const ColAcc<0, 11>& acc = cc.acc;
         if (active) {
             const size_t ci = static_cast<size_t>(item) * 256 + col;
             ColumnShape cs;
@@ -348,12 +267,11 @@ __global__ __launch_bounds__(256) void s0_coarse_signs_kernel(S0Args a) {
             const int cx = active ? col / 61 : 0, cz = active ? col % 61 : 0;
             const double X = static_cast<double>(-28 + cx + a.center_dx[center] * 4);
             const double Z = static_cast<double>(-28 + cz + a.center_dz[center] * 4);
-            ColAcc<0, 16> acc;
-            evaluate_cached_column<0, 16, true>(st, perms, offs, X, Z, active, acc);
+            S0Column<0, 16> cc;
+            evaluate_cached_column<0, 16, true>(st, perms, offs, X, Z, active, cc);
+            const ColAcc<0, 16>& acc = cc.acc;
             if (active) {
-                double temp, rain;
-                climate_finish(acc.t, acc.r, acc.c, temp, rain);
-                const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
+                const ColumnShape cs = column_shape(cc.temp, cc.rain, acc.n4, acc.n5);
 #pragma unroll
                 for (int y = 0; y < 16; ++y)
                     if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0)

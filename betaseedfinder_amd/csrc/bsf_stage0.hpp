@@ -4,23 +4,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bsf_colpar.hpp"
 #include "bsf_device.hpp"
 
 namespace bsf {
 
-// Generator ids in the per-world cache: the 66 terrain generators that are evaluated, in
-// Random order with the two consumed 4-octave generators left out, then the 10 climate ones.
-constexpr int kGenN2 = 0, kGenN3 = 16, kGenN1 = 32, kGenN4 = 40, kGenN5 = 50;
-constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
 constexpr int kCoarseRows = 61 * 17;  // (x, y) rows of 61-bit z masks
 
 enum { S0_P20 = 0, S0_UPPER = 1 };
 
 struct GenCacheArgs {
-    const int64_t* seeds;
+    const int64_t* seeds;   // explicit seeds, or nullptr for first_seed + i
     int64_t count;
     uint32_t* perms;    // [world][kGenCount][64] dwords
     double* offsets;    // [world][kGenCount][3]
+    int64_t first_seed;
 };
 
 struct S0Args {

@@ -193,7 +193,7 @@ void run_p20(const int64_t* h_seeds, int count, int centers, S0Args& a, std::vec
     check(hipMemcpyAsync(d_seeds, h_seeds, static_cast<size_t>(count) * 8, hipMemcpyHostToDevice, s), "H2D seeds");
     uint32_t* perms = static_cast<uint32_t*>(g_ctx.s0_a.ensure(static_cast<size_t>(count) * kGenCount * 256));
     double* offs = static_cast<double*>(g_ctx.s0_b.ensure(static_cast<size_t>(count) * kGenCount * 3 * 8));
-    GenCacheArgs g{d_seeds, count, perms, offs};
+    GenCacheArgs g{d_seeds, count, perms, offs, 0};
     {
         LaunchTimer t("stage0_cache", s);
         hipLaunchKernelGGL(gen_cache_kernel, dim3(grid_for(gen_cache_kernel, 64, (count + 63) / 64)), dim3(64), 0, s, g);
