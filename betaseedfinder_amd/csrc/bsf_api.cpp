@@ -390,9 +390,10 @@ int64_t island_pipeline_cached(const int64_t* h_seeds, int64_t first, int64_t co
         a.lattice = d_lat;
         {
             LaunchTimer t("island_stage2", s);
+            const size_t clim_lds = static_cast<size_t>(2 * nd) * sizeof(double);
             hipLaunchKernelGGL(island_s2_kernel,
-                               dim3(static_cast<int>(std::min<int64_t>(a.list_n, resident_blocks(island_s2_kernel, 320, 0)))),
-                               dim3(320), 0, s, a);
+                               dim3(static_cast<int>(std::min<int64_t>(a.list_n, resident_blocks(island_s2_kernel, 64, clim_lds)))),
+                               dim3(64), clim_lds, s, a);
             check(hipGetLastError(), "stage2 launch");
             t.stop(static_cast<uint64_t>(nsurv));
         }

@@ -94,17 +94,19 @@ __device__ __forceinline__ void island_s2_emit(const IslandArgs& a, int64_t it, 
     }
 }
 
-constexpr int kS2Threads = 320;
+constexpr int kS2Threads = 64;
 constexpr int kS2MaxDens = 8 * 8 * 25;
-constexpr int kS2DensPerLane = (kS2MaxDens + kS2Threads - 1) / kS2Threads;  // 5
 
+// One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
+// of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
 __global__ __launch_bounds__(kS2Threads) void island_s2_kernel(IslandArgs a) {
     __shared__ GenStage st;
-    __shared__ double s_temp[kS2MaxDens];
-    __shared__ double s_rain[kS2MaxDens];
+    extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
     const int cx_n = a.p.chunks_x, cz_n = a.p.chunks_z;
     const int nd = 25 * cx_n * cz_n;
+    double* s_temp = s_clim;
+    double* s_rain = s_clim + nd;
     const int nzl = 4 * cz_n + 1;
     const int nc = (4 * cx_n + 1) * nzl;
     for (int64_t it = blockIdx.x; it < a.list_n; it += gridDim.x) {
@@ -112,30 +114,18 @@ __global__ __launch_bounds__(kS2Threads) void island_s2_kernel(IslandArgs a) {
         const uint32_t* perms = a.perms + static_cast<size_t>(slot) * kGenCount * 64;
         const double* offs = a.offsets + static_cast<size_t>(slot) * kGenCount * 3;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)
-        {
-            double BX[kS2DensPerLane], BZ[kS2DensPerLane], t[kS2DensPerLane], r[kS2DensPerLane];
-            int n = 0;
-#pragma unroll
-            for (int j = 0; j < kS2DensPerLane; ++j) {
-                const int d = tid + j * kS2Threads;
-                if (d < nd) {
-                    const int ch = d / 25, il = d % 25;
-                    const int cx = a.p.chunk_x0 + ch / cz_n, cz = a.p.chunk_z0 + ch % cz_n;
-                    BX[j] = static_cast<double>(cx * 16 + 3 * (il / 5) + 1);
-                    BZ[j] = static_cast<double>(cz * 16 + 3 * (il % 5) + 1);
-                    n = j + 1;
-                } else {
-                    BX[j] = BZ[j] = 0.0;
-                }
-            }
-            colpar_climate_all<kS2DensPerLane>(st, perms, offs, BX, BZ, n, t, r);
-#pragma unroll
-            for (int j = 0; j < kS2DensPerLane; ++j) {
-                const int d = tid + j * kS2Threads;
-                if (d < nd) {
-                    s_temp[d] = t[j];
-                    s_rain[d] = r[j];
-                }
+        for (int base = 0; base < nd; base += kS2Threads) {
+            const int d = base + tid;
+            const bool active = d < nd;
+            const int ch = active ? d / 25 : 0, il = active ? d % 25 : 0;
+            const int cx = a.p.chunk_x0 + ch / cz_n, cz = a.p.chunk_z0 + ch % cz_n;
+            const double BX = static_cast<double>(cx * 16 + 3 * (il / 5) + 1);
+            const double BZ = static_cast<double>(cz * 16 + 3 * (il % 5) + 1);
+            double t = 0.0, r = 0.0;
+            colpar_climate_all<1>(st, perms, offs, &BX, &BZ, active ? 1 : 0, &t, &r);
+            if (active) {
+                s_temp[d] = t;
+                s_rain[d] = r;
             }
         }
         // noise lattice, one column per lane; stage_generator's barriers publish s_temp/s_rain
@@ -150,6 +140,7 @@ __global__ __launch_bounds__(kS2Threads) void island_s2_kernel(IslandArgs a) {
             __syncthreads();
             if (active) island_s2_emit<8, 16>(a, it, nd, u, v, acc.n1, acc.n2, acc.n3, acc.n4, acc.n5, s_temp, s_rain);
         }
+        __syncthreads();
     }
 }
 

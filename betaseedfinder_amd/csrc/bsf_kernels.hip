@@ -308,14 +308,16 @@ struct FloodShared {
     unsigned long long* st67;  // stone at y = 67 per x
 };
 
-// Block density chain of generateBareTerrain (BetaTerrain173.java:599-664) for one (x, y)
-// row across the window, from the window lattice lat[(chunk*25 + i*5 + l)*8 + (k - 8)],
-// level 16 = -10.0.  Returns the stone z-mask.
-__device__ unsigned long long stone_row(const double* __restrict__ lat, int x, int y, int cz_n) {
+// The eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks) from one lattice
+// y-cell, walking generateBareTerrain's incremental chain (BetaTerrain173.java:599-664) once:
+// d1..d4 step by d5..d8 per level, d10/d11 step i2 times by d12/d13 along x, d15 steps by
+// d16 along z.  The i2 steps are all formed and the right one selected, so lanes with
+// different x stay converged.
+__device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, unsigned long long rows[8]) {
     const int cxi = x >> 4, lx = x & 15;
     const int i1 = lx >> 2, i2 = lx & 3;
-    const int k1 = y >> 3, l1 = y & 7;
-    unsigned long long bits = 0;
+#pragma unroll
+    for (int l1 = 0; l1 < 8; ++l1) rows[l1] = 0;
     for (int czi = 0; czi < cz_n; ++czi) {
         const double* cl = lat + static_cast<size_t>(cxi * cz_n + czi) * 25 * 8;
         for (int j1 = 0; j1 < 4; ++j1) {
@@ -327,28 +329,31 @@ __device__ unsigned long long stone_row(const double* __restrict__ lat, int x, i
             const double d6 = (L(i1, j1 + 1, k1 + 1) - d2) * 0.125;
             const double d7 = (L(i1 + 1, j1, k1 + 1) - d3) * 0.125;
             const double d8 = (L(i1 + 1, j1 + 1, k1 + 1) - d4) * 0.125;
-            for (int s = 0; s < l1; ++s) {
+            const int shift = czi * 16 + j1 * 4;
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const double d12 = (d3 - d1) * 0.25;
+                const double d13 = (d4 - d2) * 0.25;
+                const double a1 = d1 + d12, a2 = a1 + d12, a3 = a2 + d12;
+                const double b1 = d2 + d13, b2 = b1 + d13, b3 = b2 + d13;
+                const double d10 = i2 == 0 ? d1 : (i2 == 1 ? a1 : (i2 == 2 ? a2 : a3));
+                const double d11 = i2 == 0 ? d2 : (i2 == 1 ? b1 : (i2 == 2 ? b2 : b3));
+                const double d16 = (d11 - d10) * 0.25;
+                double d15 = d10;
+                unsigned long long bits = 0;
+#pragma unroll
+                for (int k2 = 0; k2 < 4; ++k2) {
+                    if (d15 > 0.0) bits |= 1ULL << k2;
+                    d15 += d16;
+                }
+                rows[l1] |= bits << shift;
                 d1 += d5;
                 d2 += d6;
                 d3 += d7;
                 d4 += d8;
             }
-            double d10 = d1, d11 = d2;
-            const double d12 = (d3 - d1) * 0.25;
-            const double d13 = (d4 - d2) * 0.25;
-            for (int s = 0; s < i2; ++s) {
-                d10 += d12;
-                d11 += d13;
-            }
-            double d15 = d10;
-            const double d16 = (d11 - d10) * 0.25;
-            for (int k2 = 0; k2 < 4; ++k2) {
-                if (d15 > 0.0) bits |= 1ULL << (czi * 16 + j1 * 4 + k2);
-                d15 += d16;
-            }
         }
     }
-    return bits;
 }
 
 struct CompStats {
@@ -397,17 +402,19 @@ __device__ __forceinline__ unsigned long long zfill(unsigned long long s, unsign
 }
 
 // Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
-// (x, y in [68, 127]) with z along the bits.  Thread x sweeps its rows up and down (y and z
-// reach the fixpoint in one pass); x neighbours propagate one step per round.
+// (x, y in [68, 127]) with z along the bits.  Rounds alternate two sweeps: thread x walks its
+// rows up and down in y, then thread y walks its rows up and down in x; z runs fill in one
+// step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is the
+// component and the order of the sweeps cannot change it.
 __device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
-    const int x = threadIdx.x;
+    const int t = threadIdx.x;
     for (;;) {
         int changed = 0;
-        if (x < nx) {
-            unsigned long long* row = comp + x * kRowStride;
-            const unsigned long long* mrow = rem + x * kRowStride;
-            const unsigned long long* left = x > 0 ? row - kRowStride : nullptr;
-            const unsigned long long* right = x < nx - 1 ? row + kRowStride : nullptr;
+        if (t < nx) {
+            unsigned long long* row = comp + t * kRowStride;
+            const unsigned long long* mrow = rem + t * kRowStride;
+            const unsigned long long* left = t > 0 ? row - kRowStride : nullptr;
+            const unsigned long long* right = t < nx - 1 ? row + kRowStride : nullptr;
             unsigned long long prev = 0;
             for (int y = 0; y < kLevels; ++y) {
                 const unsigned long long m = mrow[y];
@@ -429,6 +436,36 @@ __device__ void grow_component(unsigned long long* comp, const unsigned long lon
                 const unsigned long long n = zfill((c | prev) & m, m);
                 if (n != c) {
                     row[y] = n;
+                    changed = 1;
+                }
+                prev = n;
+            }
+        }
+        __syncthreads();
+        if (t < kLevels) {
+            unsigned long long prev = 0;
+            for (int x = 0; x < nx; ++x) {
+                const int r = x * kRowStride + t;
+                const unsigned long long m = rem[r];
+                const unsigned long long c = comp[r];
+                unsigned long long nb = c | prev;
+                if (t > 0) nb |= comp[r - 1];
+                if (t < kLevels - 1) nb |= comp[r + 1];
+                const unsigned long long n = zfill(nb & m, m);
+                if (n != c) {
+                    comp[r] = n;
+                    changed = 1;
+                }
+                prev = n;
+            }
+            prev = 0;
+            for (int x = nx - 1; x >= 0; --x) {
+                const int r = x * kRowStride + t;
+                const unsigned long long m = rem[r];
+                const unsigned long long c = comp[r];
+                const unsigned long long n = zfill((c | prev) & m, m);
+                if (n != c) {
+                    comp[r] = n;
                     changed = 1;
                 }
                 prev = n;
@@ -458,17 +495,18 @@ __global__ __launch_bounds__(256) void island_flood_kernel(FloodArgs a) {
 
     for (int64_t sv = blockIdx.x; sv < nsurv; sv += gridDim.x) {
         const double* lat = a.lattice + static_cast<size_t>(sv) * nd * 8;
-        // 1. upsample rows y in [67, 127] (generateBareTerrain's incremental chain)
-        for (int r = tid; r < nx * (kLevels + 1); r += blockDim.x) {
-            const int x = r / (kLevels + 1);
-            const int y = 67 + r % (kLevels + 1);
-            const unsigned long long bits = stone_row(lat, x, y, a.p.chunks_z);
-            if (y == 67) {
-                S.st67[x] = bits;
-                S.rem[x * kRowStride + kLevels] = 0;  // padding slot of the x row
-            } else {
-                S.rem[x * kRowStride + (y - kFloorY)] = bits;
+        // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
+        for (int task = tid; task < nx * 8; task += blockDim.x) {
+            const int x = task >> 3, k1 = 8 + (task & 7);
+            unsigned long long rows[8];
+            stone_cell_rows(lat, x, k1, a.p.chunks_z, rows);
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const int y = k1 * 8 + l1;
+                if (y >= kFloorY) S.rem[x * kRowStride + (y - kFloorY)] = rows[l1];
+                else if (y == kFloorY - 1) S.st67[x] = rows[l1];
             }
+            if (k1 == 8) S.rem[x * kRowStride + kLevels] = 0;  // padding slot of the x row
         }
         __syncthreads();
         // 2. every component that reaches a y = 68 block with stone below touches the bottom

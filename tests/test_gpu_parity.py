@@ -54,3 +54,17 @@ def test_island_hits_match_oracle(bsf, oracle):
     gpu = bsf.island_range(first, count, p)
     ref = oracle.run_island(first, count, po, threads=16)
     assert gpu == ref
+
+
+@pytest.mark.parametrize("window", [(0, 0, 1, 1), (-3, 1, 3, 2), (5, -7, 2, 4), (-1, -4, 8, 3)])
+def test_island_windows_match_oracle(bsf, oracle, window):
+    x0, z0, w, h = window
+    kw = dict(chunk_x0=x0, chunk_z0=z0, chunks_x=w, chunks_z=h, min_blocks=100, min_columns=40,
+              min_width=6, min_depth=6)
+    first, count = 20000, 768
+    gpu = bsf.island_range(first, count, bsf.island_params(**kw))
+    ref = oracle.run_island(first, count, oracle.island_params(**kw), threads=16)
+    assert gpu == ref
+    s1 = set(bsf.island_stage1_range(first, 256, bsf.island_params(**kw)).tolist())
+    ref1 = {s for s in range(first, first + 256) if oracle.island_has_candidate(s, oracle.island_params(**kw))}
+    assert s1 == ref1
