@@ -1,0 +1,271 @@
+// worker_main.cpp -- BetaSeedFinderWorker: the drop-in search worker process for MI355X.
+//
+// The Java launcher (src/GpuStage0Scout.java:143-219, src/GpuP20Scout.java:45-93) spawns the
+// worker with positional argv, reads an 8-byte magic from stdout, then alternates
+//   request  : LE u32 count, count x LE i64 seeds          (count == 0 or EOF: orderly exit)
+//   response : LE u32 count, count x centers x record bytes
+// while it pumps stderr lines into its log.  This program keeps those arguments, magics,
+// record layouts and stderr lines (gpu_p20_benchmark.cpp:7251-7685, 7689-7734) and runs every
+// batch through libbsf (include/bsf.h).  Errors end the process with "ERROR: <what>" on stderr
+// and exit status 1, as the reference's main does (:7922-7925).
+//
+// Modes:
+//   stream [capacity=32768]
+//   stage0stream [capacity=32768] [general|mega|record60|record80] [research|lean] [centers=1|8]
+//   density [capacity=32768] [chunkX=0] [chunkZ=0]      magic DNSTY425: count x 425 LE f64
+//   island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]   hit lines (text)
+#include <cinttypes>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bsf.h"
+
+namespace {
+
+struct WorkerError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(int rc, const char* what) {
+    if (rc != BSF_OK) throw WorkerError(std::string(what) + ": " + bsf_last_error());
+}
+
+// tryReadLe32 (gpu_p20_benchmark.cpp:1010-1020): clean EOF before the header -> false.
+bool try_read_le32(uint32_t& v) {
+    unsigned char b[4];
+    const size_t got = std::fread(b, 1, 4, stdin);
+    if (got == 0 && std::feof(stdin)) return false;
+    if (got != 4) throw WorkerError("truncated stream request header");
+    v = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 8) | (static_cast<uint32_t>(b[2]) << 16) |
+        (static_cast<uint32_t>(b[3]) << 24);
+    return true;
+}
+
+void read_seeds(uint32_t count, std::vector<int64_t>& seeds, std::vector<unsigned char>& raw, const char* what) {
+    raw.resize(static_cast<size_t>(count) * 8);
+    if (count && std::fread(raw.data(), 1, raw.size(), stdin) != raw.size()) throw WorkerError(what);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint64_t v = 0;
+        for (int b = 7; b >= 0; --b) v = (v << 8) | raw[static_cast<size_t>(i) * 8 + b];
+        seeds[i] = static_cast<int64_t>(v);
+    }
+}
+
+bool write_all(const void* p, size_t n) { return n == 0 || std::fwrite(p, 1, n, stdout) == n; }
+
+bool write_le32(uint32_t v) {
+    const unsigned char b[4] = {static_cast<unsigned char>(v), static_cast<unsigned char>(v >> 8),
+                                static_cast<unsigned char>(v >> 16), static_cast<unsigned char>(v >> 24)};
+    return write_all(b, 4);
+}
+
+void print_device() {
+    char banner[1024];
+    check(bsf_device_banner(banner, sizeof(banner)), "device banner");
+    std::fputs(banner, stderr);
+}
+
+int parse_int(const char* s, const char* what) {
+    char* end = nullptr;
+    const long v = std::strtol(s, &end, 10);
+    if (!s[0] || *end) throw WorkerError(std::string("invalid ") + what + ": " + s);
+    return static_cast<int>(v);
+}
+
+// streamFilter (gpu_p20_benchmark.cpp:7614-7685): one byte per seed, the P20 count clamped.
+int stream_mode(int capacity) {
+    if (capacity < 1) throw WorkerError("stream capacity must be >= 1");
+    print_device();
+    std::fprintf(stderr, "P20 stream capacity: %d seeds\n", capacity);
+    std::vector<int64_t> seeds(capacity);
+    std::vector<unsigned char> raw;
+    std::vector<int32_t> counts(capacity);
+    std::vector<unsigned char> out(capacity);
+    if (!write_all("P20STR01", 8) || std::fflush(stdout) != 0) throw WorkerError("failed writing stream magic");
+    uint64_t total = 0, passed = 0;
+    for (;;) {
+        uint32_t count = 0;
+        if (!try_read_le32(count) || count == 0) break;
+        if (count > static_cast<uint32_t>(capacity)) throw WorkerError("stream batch exceeds configured capacity");
+        read_seeds(count, seeds, raw, "truncated stream seed batch");
+        check(bsf_p20_counts(seeds.data(), static_cast<int32_t>(count), counts.data()), "P20 batch");
+        uint64_t p = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const int v = counts[i];
+            out[i] = static_cast<unsigned char>(v < 0 ? 0 : (v > 255 ? 255 : v));
+            if (v > 0) ++p;
+        }
+        if (!write_le32(count) || !write_all(out.data(), count) || std::fflush(stdout) != 0)
+            throw WorkerError("failed writing stream response");
+        total += count;
+        passed += p;
+    }
+    std::fprintf(stderr, "P20 stream closed | seeds=%" PRIu64 " | passed=%" PRIu64 "\n", total, passed);
+    return 0;
+}
+
+struct ProfileInfo {
+    const char* arg;
+    const char* display;
+    int p20, upper, high;
+    double p19;
+    int coarse;
+    bool mega;
+};
+const ProfileInfo kProfiles[4] = {
+    {"general", "GENERAL", 1, 5, 5, 4.02, 85, false},
+    {"mega", "MEGA 30k+", 1, 8, 6, 4.02, 85, true},
+    {"record60", "RECORD 60k+ (empirical)", 3, 19, 12, 6.70, 95, false},
+    {"record80", "WORLD RECORD COARSE 700 (compact-thick safe)", 3, 19, 20, 6.70, 700, false},
+};
+
+// streamStage0Filter (gpu_p20_benchmark.cpp:7310-7612).
+int stage0_mode(int capacity, const std::string& profile, const std::string& telemetry, int centers) {
+    int pid = -1;
+    for (int i = 0; i < 4; ++i)
+        if (profile == kProfiles[i].arg) pid = i;
+    if (pid < 0) throw WorkerError("hunt profile must be general, mega, record60, or record80");
+    if (telemetry != "research" && telemetry != "lean") throw WorkerError("stage0stream telemetry must be research or lean");
+    if (capacity < 1) throw WorkerError("Stage0 stream capacity must be >= 1");
+    if (centers != 1 && centers != 8) throw WorkerError("Stage0 stream center count must be 1 or 8");
+    const bool research = telemetry == "research";
+    const ProfileInfo& pr = kProfiles[pid];
+    print_device();
+    std::fprintf(stderr, "Stage0+P19+coarse stream capacity: %d worlds / %lld regions\n", capacity,
+                 static_cast<long long>(capacity) * centers);
+    std::fprintf(stderr, "Multi-center: %d exact radius-7 regions/world | 15-chunk spacing\n", centers);
+    std::fprintf(stderr, "Profile: %s | P20>=%d | Upper>=%d | High>=%d | P19>=%g/extreme | Coarse>=%d%s\n",
+                 pr.display, pr.p20, pr.upper, pr.high, pr.p19, pr.coarse,
+                 pr.mega ? " | Mega topology reject ON" : " | topology reject OFF");
+    if (pid >= 2) std::fprintf(stderr, "WARNING: record profile is empirically validated, not mathematically lossless.\n");
+    std::fprintf(stderr, "Stage0+P19+coarse pipeline: exact P20 -> full upper survivors -> lower reentry -> exact "
+                         "cached Stage0.75 gate -> optional Mega topology reject -> exact radius-7 coarse score\n");
+    std::fprintf(stderr, "Response telemetry: %s\n", research ? "RESEARCH 34 bytes/region" : "LEAN 16 bytes/region + exact P19 double");
+    const size_t rb = research ? 34 : 16;
+    std::vector<int64_t> seeds(capacity);
+    std::vector<unsigned char> raw;
+    std::vector<uint8_t> out(static_cast<size_t>(capacity) * centers * rb);
+    if (!write_all("ST0R3816", 8) || std::fflush(stdout) != 0) throw WorkerError("failed writing stream magic");
+    bsf_stage0_counters c{};
+    for (;;) {
+        uint32_t count = 0;
+        if (!try_read_le32(count) || count == 0) break;
+        if (count > static_cast<uint32_t>(capacity))
+            throw WorkerError("Stage0 stream batch exceeds configured world capacity");
+        read_seeds(count, seeds, raw, "truncated Stage0 stream seed batch");
+        check(bsf_stage0_records(seeds.data(), static_cast<int32_t>(count), pid, research ? 1 : 0, centers, out.data(), &c),
+              "Stage0 batch");
+        const size_t n = static_cast<size_t>(count) * centers * rb;
+        if (!write_le32(count) || !write_all(out.data(), n) || std::fflush(stdout) != 0) {
+            std::fprintf(stderr, "Stage0+P19+coarse stream parent closed response pipe; exiting cleanly\n");
+            break;
+        }
+    }
+    std::fprintf(stderr,
+                 "Stage0+P19+coarse stream closed | worlds=%" PRIu64 " | regions=%" PRIu64 " | P20Passed=%" PRIu64
+                 " | Stage1Passed=%" PRIu64 " | Stage0.5Passed=%" PRIu64 " | Stage0.75Evaluated=%" PRIu64
+                 " | Stage0.75Passed=%" PRIu64 " | Stage0.75Rejected=%" PRIu64 " | MegaTopologyRejected=%" PRIu64
+                 " | CoarseEvaluated=%" PRIu64 " | CoarsePassedTarget=%" PRIu64 "\n",
+                 c.worlds, c.regions, c.p20_passed, c.stage1_passed, c.stage05_passed, c.stage075_evaluated,
+                 c.stage075_passed, c.stage075_rejected, c.mega_topology_rejected, c.coarse_evaluated, c.coarse_passed);
+    return 0;
+}
+
+// density: the 5x17x5 generateTerrainNoise lattice of one chunk for every seed of a batch.
+int density_mode(int capacity, int cx, int cz) {
+    if (capacity < 1) throw WorkerError("density capacity must be >= 1");
+    print_device();
+    std::fprintf(stderr, "Density stream capacity: %d seeds | chunk (%d, %d) | 425 f64 per seed\n", capacity, cx, cz);
+    std::vector<int64_t> seeds(capacity);
+    std::vector<unsigned char> raw;
+    std::vector<double> out(static_cast<size_t>(capacity) * 425);
+    if (!write_all("DNSTY425", 8) || std::fflush(stdout) != 0) throw WorkerError("failed writing stream magic");
+    uint64_t total = 0;
+    for (;;) {
+        uint32_t count = 0;
+        if (!try_read_le32(count) || count == 0) break;
+        if (count > static_cast<uint32_t>(capacity)) throw WorkerError("density batch exceeds configured capacity");
+        read_seeds(count, seeds, raw, "truncated density seed batch");
+        check(bsf_density_grid(seeds.data(), count, cx, cz, out.data()), "density batch");
+        // doubles go out little-endian: this host is little-endian (x86-64)
+        if (!write_le32(count) || !write_all(out.data(), static_cast<size_t>(count) * 425 * 8) || std::fflush(stdout) != 0)
+            throw WorkerError("failed writing density response");
+        total += count;
+    }
+    std::fprintf(stderr, "Density stream closed | seeds=%" PRIu64 "\n", total);
+    return 0;
+}
+
+// island: the full floating-island pipeline over a seed range, one text line per hit.
+int island_mode(int argc, char** argv) {
+    if (argc < 4) throw WorkerError("island requires <first_seed> <count>");
+    const int64_t first = std::strtoll(argv[2], nullptr, 10);
+    const int64_t count = std::strtoll(argv[3], nullptr, 10);
+    bsf_island_params p{-2, -2, 4, 4, 1000, 100, 60, 10, 10, 2.0};
+    if (argc >= 9) {
+        p.chunk_x0 = parse_int(argv[4], "chunkX0");
+        p.chunk_z0 = parse_int(argv[5], "chunkZ0");
+        p.chunks_x = parse_int(argv[6], "chunksX");
+        p.chunks_z = parse_int(argv[7], "chunksZ");
+        p.min_blocks = parse_int(argv[8], "minBlocks");
+    }
+    print_device();
+    std::vector<bsf_island_hit> hits(1 << 20);
+    int64_t n = 0;
+    check(bsf_island_range(first, count, &p, hits.data(), static_cast<int64_t>(hits.size()), &n), "island range");
+    std::printf("seed,chunk_x,chunk_z,min_x,max_x,min_z,max_z,min_y,max_y,blocks,columns,center_x,center_z,touches_side\n");
+    for (int64_t i = 0; i < n && i < static_cast<int64_t>(hits.size()); ++i) {
+        const bsf_island_hit& h = hits[i];
+        std::printf("%" PRId64 ",%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d\n", h.seed, h.chunk_x, h.chunk_z, h.min_x,
+                    h.max_x, h.min_z, h.max_z, h.min_y, h.max_y, h.blocks, h.columns, h.center_x, h.center_z,
+                    h.flags & 1);
+    }
+    std::fprintf(stderr, "Island range closed | seeds=%" PRId64 " | hits=%" PRId64 "\n", count, n);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        if (argc < 2) {
+            std::fprintf(stderr,
+                         "Usage:\n"
+                         "  BetaSeedFinderWorker stream [capacity=32768]\n"
+                         "  BetaSeedFinderWorker stage0stream [capacity=32768] [general|mega|record60|record80] "
+                         "[research|lean] [centers=1|8]\n"
+                         "  BetaSeedFinderWorker density [capacity=32768] [chunkX=0] [chunkZ=0]\n"
+                         "  BetaSeedFinderWorker island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]\n");
+            return 1;
+        }
+        const std::string mode = argv[1];
+        const char* dev = std::getenv("BSF_DEVICE");
+        check(bsf_init(dev ? std::atoi(dev) : 0), "init");
+        std::setvbuf(stdout, nullptr, _IOFBF, 1 << 20);
+        int rc = 1;
+        if (mode == "stream") {
+            rc = stream_mode(argc > 2 ? parse_int(argv[2], "capacity") : 32768);
+        } else if (mode == "stage0stream") {
+            rc = stage0_mode(argc > 2 ? parse_int(argv[2], "capacity") : 32768, argc > 3 ? argv[3] : "general",
+                             argc > 4 ? argv[4] : "research", argc > 5 ? parse_int(argv[5], "centers") : 1);
+        } else if (mode == "density") {
+            rc = density_mode(argc > 2 ? parse_int(argv[2], "capacity") : 32768, argc > 3 ? parse_int(argv[3], "chunkX") : 0,
+                              argc > 4 ? parse_int(argv[4], "chunkZ") : 0);
+        } else if (mode == "island") {
+            rc = island_mode(argc, argv);
+        } else {
+            throw WorkerError("unknown mode: " + mode);
+        }
+        std::fflush(stdout);
+        bsf_shutdown();
+        return rc;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ERROR: %s\n", e.what());
+        return 1;
+    }
+}
