@@ -48,14 +48,16 @@ def island_workload():
     return start, min_blocks
 
 
-# Algorithmic FP64 operations (add, sub, mul, div, compare, int<->double conversion) per
-# construct of the Stage-1 kernel, counted from the source (bsf_device.hpp / bsf_kernels.hip).
+# Algorithmic FP64 operations (add, sub, mul, div, compare, int<->double conversion) of one
+# Stage-1 seed, counted per construct of the Java algorithm with cached axes
+# (NoiseGeneratorPerlin173.addNoiseCachedAxes*, :399-507): the x and z axes once per column and
+# octave, the y axis once per octave, the eight corner gradients + four x-lerps once per y
+# cell (the legacy Y-cell reuse), and per point the y/z lerps, the weight and the accumulate.
 OPS = {
-    "build": 18,            # 3 x nextDouble (2 cvt, mul, add, mul) x 256.0
     "axis": 16,             # coordinate (cvt, mul, add) + Java floor (2 cvt, cmp) + frac (cvt, sub) + fm1 + fade (7)
     "yaxis_level": 16,      # level coordinate, floor, frac, fade, cell fm1
-    "yaxis_setup": 14,      # Y0 coordinate + one cell walk-back step
-    "point3": 31,           # 8 gradients, 7 lerps, weight, accumulate
+    "corners": 20,          # 8 gradients (one add each) + 4 x-lerps (3 each)
+    "point3": 11,           # 2 y-lerps + 1 z-lerp (3 each), weight, accumulate
     "point2": 18,           # 1 legacy + 3 gradients, 3 lerps, weight, accumulate (+0.0*x)
     "simplex": 70,          # one simplex2 sample with all three corners live, weight, accumulate
     "finish": 16,           # climate post-processing
@@ -63,15 +65,22 @@ OPS = {
     "density_level": 15,    # one lattice density
     "chain_level": 18,      # candidate y-chain per lattice level (sub, mul, 8 x (cmp, add))
 }
+NOISE_OCTAVES = ([("n2", 684.412 / 2 ** o) for o in range(16)] + [("n3", 684.412 / 2 ** o) for o in range(16)] +
+                 [("n1", 684.412 / 160.0 / 2 ** o) for o in range(8)])
 
 
 def stage1_fp64_ops_per_seed(n_cols=64, levels=8):
-    o3d = 40 * (OPS["yaxis_setup"] + levels * OPS["yaxis_level"] + n_cols * (2 * OPS["axis"] + levels * OPS["point3"]))
+    """Expected algorithmic FP64 operations of one Stage-1 seed (64 sample columns, 8 levels).
+    A y cell of scale s covers 1/s levels, so 1 + (levels - 1) * min(1, s) corner sets are
+    computed per column and octave."""
+    o3d = 0.0
+    for _, sy in NOISE_OCTAVES:
+        cells = 1 + (levels - 1) * min(1.0, sy)
+        o3d += levels * OPS["yaxis_level"] + n_cols * (2 * OPS["axis"] + cells * OPS["corners"] + levels * OPS["point3"])
     o2d = 26 * n_cols * (2 * OPS["axis"] + OPS["point2"])
     clim = 10 * n_cols * OPS["simplex"]
-    builds = 84 * OPS["build"]
     combine = n_cols * (OPS["finish"] + OPS["shape"] + levels * (OPS["density_level"] + OPS["chain_level"]))
-    return o3d + o2d + clim + builds + combine
+    return int(round(o3d + o2d + clim + combine))
 
 
 def main():
@@ -172,7 +181,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "lattice_kernel<8,16,MODE_CANDIDATE> (island Stage 1)",
+            "kernel": "island_s1_kernel (island Stage 1: 64 sample columns per seed)",
             "achieved": round(achieved_tflops, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
