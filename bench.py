@@ -128,6 +128,7 @@ def main():
 
     for k in range(args.warmup):
         step(k)
+    stats0 = bsf.engine_stats()
     L.bsf_timing_reset()
     L.bsf_timing_enable(1)
     sync()
@@ -144,11 +145,12 @@ def main():
         L.bsf_kernel_stats(name.encode(), ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(units))
         return ms.value, launches.value, units.value
 
+    stats1 = bsf.engine_stats()
     seeds = args.steps * batch
     value = seeds / elapsed
     cache_ms, _, _ = kstats("island_cache")
     s1_ms, s1_n, s1_units = kstats("island_stage1")
-    s2_ms, s2_n, s2_units = kstats("island_stage2")
+    s2_ms, _, _ = kstats("island_stage2")
     fl_ms, fl_n, _ = kstats("island_flood")
     ops_seed = stage1_fp64_ops_per_seed()
     avg_launch_s = s1_ms / 1e3 / max(1, s1_n)
@@ -175,7 +177,8 @@ def main():
             "first_seed": start,
             "min_island_blocks": min_blocks,
             "hits_in_timed_region": hits,
-            "stage1_survivor_fraction": round(s2_units / max(1, s1_units), 5),
+            "stage1_survivor_fraction": round((stats1[2] - stats0[2]) / max(1, stats1[1] - stats0[1]), 5),
+            "passes": stats1[0] - stats0[0],
             "kernel_ms": {"island_cache": round(cache_ms, 3), "island_stage1": round(s1_ms, 3), "island_stage2": round(s2_ms, 3),
                           "island_flood": round(fl_ms, 3)},
         },

@@ -13,6 +13,13 @@ from .api import (  # noqa: F401
     BsfError,
     IslandHit,
     IslandParams,
+    Server,
+    autotune,
+    engine_configure,
+    engine_stats,
+    get_tuning,
+    island_batch,
+    set_tuning,
     Stage0Counters,
     density_grid,
     device_banner,

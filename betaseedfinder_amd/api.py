@@ -162,3 +162,138 @@ def stage0_records(seeds, profile="general", research=True, centers=8):
                                     PROFILE_NAMES[profile], int(research), centers,
                                     out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(c)))
     return out.reshape(len(s), centers, rb), c
+
+
+# ---------------------------------------------------------------------------
+# batched jobs, engine shape, autotuner
+# ---------------------------------------------------------------------------
+class IslandJob(ctypes.Structure):
+    _fields_ = [("first_seed", ctypes.c_int64), ("count", ctypes.c_int64), ("p", IslandParams)]
+
+
+class IslandTuning(ctypes.Structure):
+    _fields_ = [("worlds_per_wave", ctypes.c_int32), ("gate", ctypes.c_int32), ("seeds_per_sec", ctypes.c_double),
+                ("candidates", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def island_batch(jobs, cap=1 << 20):
+    """jobs: list of (first_seed, count, IslandParams).  Returns (hits sorted by (job, seed),
+    per-job hit counts); each hit's window_index is its job's index."""
+    init()
+    arr = (IslandJob * max(1, len(jobs)))()
+    for i, (first, count, p) in enumerate(jobs):
+        arr[i].first_seed = first
+        arr[i].count = count
+        arr[i].p = p
+    out = (IslandHit * cap)()
+    n = ctypes.c_int64(0)
+    per_job = np.zeros(max(1, len(jobs)), dtype=np.int64)
+    _check(lib().bsf_island_batch(arr, ctypes.c_int32(len(jobs)), out, ctypes.c_int64(cap), ctypes.byref(n),
+                                  per_job.ctypes.data_as(ctypes.c_void_p)))
+    return [out[i].as_tuple() for i in range(n.value)], per_job[:len(jobs)].copy()
+
+
+def engine_configure(pass_slots=1 << 17, max_jobs=1024, min_quantum=1024):
+    init()
+    _check(lib().bsf_island_engine_configure(ctypes.c_int64(pass_slots), ctypes.c_int32(max_jobs),
+                                             ctypes.c_int32(min_quantum)))
+
+
+def engine_stats():
+    """(passes, seeds, survivors, hits) of the library's island engine."""
+    init()
+    v = [ctypes.c_uint64(0) for _ in range(4)]
+    _check(lib().bsf_island_engine_stats(*[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def autotune(params, first_seed, trial_seeds):
+    init()
+    t = IslandTuning()
+    _check(lib().bsf_island_autotune(ctypes.byref(params), ctypes.c_int64(first_seed), ctypes.c_int64(trial_seeds),
+                                     ctypes.byref(t)))
+    return t
+
+
+def get_tuning(params):
+    init()
+    t = IslandTuning()
+    _check(lib().bsf_island_get_tuning(ctypes.byref(params), ctypes.byref(t)))
+    return t
+
+
+def set_tuning(params, worlds_per_wave, gate=1):
+    init()
+    t = IslandTuning(worlds_per_wave, gate, 0.0, 0, 0)
+    _check(lib().bsf_island_set_tuning(ctypes.byref(params), ctypes.byref(t)))
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+EVENT_ADMITTED, EVENT_HIT, EVENT_DONE, EVENT_REJECTED = 1, 2, 3, 4
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("state_bytes", ctypes.c_int64), ("hit_buffer_bytes", ctypes.c_int64),
+                ("queue_limit", ctypes.c_int64), ("min_quantum", ctypes.c_int32), ("fifo", ctypes.c_int32)]
+
+
+class ServerEvent(ctypes.Structure):
+    _fields_ = [("job_id", ctypes.c_int64), ("kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("t_ms", ctypes.c_double), ("hit", IslandHit)]
+
+
+class ServerJobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32), ("submit_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("first_hit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("seeds_done", ctypes.c_int64), ("hits", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "admitted", "rejected", "completed", "queued", "resident",
+                                               "passes", "seeds", "hits", "pass_slots", "max_jobs")]
+
+
+class Server:
+    """Request loop over libbsf's serving ABI (bsf_server_*)."""
+
+    def __init__(self, state_bytes=1024 * 112, hit_buffer_bytes=(1 << 17) * 128, queue_limit=1 << 20,
+                 min_quantum=1024, fifo=False):
+        init()
+        cfg = ServerConfig(state_bytes, hit_buffer_bytes, queue_limit, min_quantum, 1 if fifo else 0)
+        self._h = ctypes.c_void_p()
+        _check(lib().bsf_server_start(ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def submit(self, first_seed, count, params):
+        job = IslandJob(first_seed, count, params)
+        jid = ctypes.c_int64(0)
+        _check(lib().bsf_server_submit(self._h, ctypes.byref(job), ctypes.byref(jid)))
+        return jid.value
+
+    def poll(self, timeout_ms=0, cap=4096):
+        ev = (ServerEvent * cap)()
+        n = ctypes.c_int64(0)
+        _check(lib().bsf_server_poll(self._h, ev, ctypes.c_int64(cap), ctypes.byref(n), ctypes.c_int32(timeout_ms)))
+        return [ev[i] for i in range(n.value)]
+
+    def job(self, job_id):
+        info = ServerJobInfo()
+        _check(lib().bsf_server_get_job(self._h, ctypes.c_int64(job_id), ctypes.byref(info)))
+        return info
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().bsf_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def stop(self, drain=True):
+        if self._h:
+            h, self._h = self._h, None
+            _check(lib().bsf_server_stop(h, ctypes.c_int32(1 if drain else 0)))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop(drain=exc[0] is None)

@@ -1,10 +1,11 @@
 // bsf_colpar.hpp -- column-parallel evaluation from a per-world generator cache.
 //
-// A workgroup works on one world: each lane owns one lattice column (noise x/z) or one
-// climate sample, the world's generators are staged one at a time into LDS (256-byte
-// permutation + 3 offsets) and shared by every lane, and all octave accumulators live in
-// registers.  Because every lane evaluates the same generator, the Y-cell run structure of
-// an octave is the same on all lanes, so the legacy corner reuse is a uniform branch.
+// A wave works on one world (or on a few worlds side by side): each lane owns one lattice
+// column (noise x/z) or one climate sample, the generators are staged one at a time into
+// LDS (256-byte permutation + 3 offsets) and shared by the lanes of their world, and all
+// octave accumulators live in registers.  When the whole workgroup evaluates one world, the
+// Y-cell run structure of an octave is the same on every lane and the legacy corner reuse
+// is a uniform branch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,6 +35,49 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
     __syncthreads();
 }
 
+// Stager for one world per workgroup: every lane reads the same staged generator.
+struct OneWorld {
+    static constexpr bool kUniform = true;
+    GenStage* st;
+    const uint32_t* perms;  // the world's cache entry [kGenCount][64]
+    const double* offs;     // [kGenCount][3]
+    __device__ __forceinline__ void stage(int g) const { stage_generator(*st, perms, offs, g); }
+    __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
+    __device__ __forceinline__ double off(int i) const { return st->off[i]; }
+};
+
+// Stager for up to kMaxWorldsPerWave consecutive cache slots in one 64-lane workgroup: lane
+// group `sub` evaluates world first_slot + sub.  Used when a world needs few columns.
+constexpr int kMaxWorldsPerWave = 16;
+struct GenStageMulti {
+    uint32_t perm[kMaxWorldsPerWave][64];
+    double off[kMaxWorldsPerWave][3];
+};
+struct MultiWorld {
+    static constexpr bool kUniform = false;
+    GenStageMulti* st;
+    const uint32_t* perms;  // cache entry of the first world; world j at + j * kGenCount * 64
+    const double* offs;
+    int worlds;             // staged worlds (1..kMaxWorldsPerWave)
+    int sub;                // this lane's world
+    __device__ __forceinline__ void stage(int g) const {
+        __syncthreads();
+        for (int j = threadIdx.x; j < worlds * 64; j += blockDim.x) {
+            const int w = j >> 6, t = j & 63;
+            st->perm[w][t] = perms[(static_cast<size_t>(w) * kGenCount + g) * 64 + t];
+        }
+        for (int j = threadIdx.x; j < worlds * 3; j += blockDim.x) {
+            const int w = j / 3, t = j % 3;
+            st->off[w][t] = offs[(static_cast<size_t>(w) * kGenCount + g) * 3 + t];
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ SharedPerm perm() const {
+        return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm[sub])};
+    }
+    __device__ __forceinline__ double off(int i) const { return st->off[sub][i]; }
+};
+
 template <int Y0, int Y1>
 struct ColAcc {
     static constexpr int NY = Y1 - Y0;
@@ -41,22 +85,22 @@ struct ColAcc {
     double n4, n5;
 };
 
-template <int Y0, int Y1>
-__device__ __forceinline__ void colpar_3d(GenStage& st, const uint32_t* perms, const double* offs, int g0, int octaves,
-                                          double s_xz, double s_y, double X, double Z, bool active, double* acc) {
+template <int Y0, int Y1, class S>
+__device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
+                                          double Z, bool active, double* acc) {
     constexpr int NY = Y1 - Y0;
     double amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
+        s.stage(g0 + o);
         if (active) {
-            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
+            const SharedPerm P = s.perm();
             const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
             YAxis<Y0, Y1> ya;
-            ya.build(sy, st.off[1]);
-            // the same value on every lane: make the branch scalar
-            const uint32_t fresh = __builtin_amdgcn_readfirstlane(ya.fresh);
-            const Axis ax = axis(X * sx + st.off[0]);
-            const Axis az = axis(Z * sx + st.off[2]);
+            ya.build(sy, s.off(1));
+            // one world per workgroup: the same value on every lane, so make the branch scalar
+            const uint32_t fresh = S::kUniform ? __builtin_amdgcn_readfirstlane(ya.fresh) : ya.fresh;
+            const Axis ax = axis(X * sx + s.off(0));
+            const Axis az = axis(Z * sx + s.off(2));
             double v[NY];
             perlin3_column_cells<Y0, Y1>(P, ax, az, ya, fresh, w, v);
 #pragma unroll
@@ -66,15 +110,15 @@ __device__ __forceinline__ void colpar_3d(GenStage& st, const uint32_t* perms, c
     }
 }
 
-__device__ __forceinline__ double colpar_2d(GenStage& st, const uint32_t* perms, const double* offs, int g0,
-                                            int octaves, double s, double X, double Z, bool active) {
+template <class S>
+__device__ __forceinline__ double colpar_2d(const S& s, int g0, int octaves, double sc, double X, double Z,
+                                            bool active) {
     double acc = 0.0, amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
+        s.stage(g0 + o);
         if (active) {
-            SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
-            const double sx = s * amp;
-            const double v = perlin2(P, axis(X * sx + st.off[0]), axis(Z * sx + st.off[2])) * (1.0 / amp);
+            const double sx = sc * amp;
+            const double v = perlin2(s.perm(), axis(X * sx + s.off(0)), axis(Z * sx + s.off(2))) * (1.0 / amp);
             acc = o == 0 ? 0.0 + v : acc + v;
         }
         amp /= 2.0;
@@ -83,21 +127,21 @@ __device__ __forceinline__ double colpar_2d(GenStage& st, const uint32_t* perms,
 }
 
 // One climate group (NoiseGeneratorOctaves2173.a, :29-51) for up to N samples per lane.
-template <int N>
-__device__ __forceinline__ void colpar_climate(GenStage& st, const uint32_t* perms, const double* offs, int g0,
-                                               int octaves, double start_scale, double octave_scale, const double* BX,
-                                               const double* BZ, int n, double* acc) {
+template <int N, class S>
+__device__ __forceinline__ void colpar_climate(const S& s, int g0, int octaves, double start_scale,
+                                               double octave_scale, const double* BX, const double* BZ, int n,
+                                               double* acc) {
     const double d2 = start_scale / 1.5;
     double d6 = 1.0, d7 = 1.0;
     for (int o = 0; o < octaves; ++o) {
-        stage_generator(st, perms, offs, g0 + o);
-        SharedPerm P{reinterpret_cast<const uint8_t*>(st.perm)};
+        s.stage(g0 + o);
+        const SharedPerm P = s.perm();
         const double sc = d2 * d7;
         const double w = 0.55 / d6;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             if (j < n) {
-                const double v = simplex2(P, BX[j] * sc + st.off[0], BZ[j] * sc + st.off[1]) * w;
+                const double v = simplex2(P, BX[j] * sc + s.off(0), BZ[j] * sc + s.off(1)) * w;
                 acc[j] = o == 0 ? 0.0 + v : acc[j] + v;
             }
         }
@@ -107,53 +151,30 @@ __device__ __forceinline__ void colpar_climate(GenStage& st, const uint32_t* per
 }
 
 // Climate of up to N samples per lane: finished temperature / rain (WorldChunkManager173.java:85-130).
-template <int N>
-__device__ __forceinline__ void colpar_climate_all(GenStage& st, const uint32_t* perms, const double* offs,
-                                                   const double* BX, const double* BZ, int n, double* temp,
-                                                   double* rain) {
+template <int N, class S>
+__device__ __forceinline__ void colpar_climate_all(const S& s, const double* BX, const double* BZ, int n,
+                                                   double* temp, double* rain) {
     double t[N], r[N], c[N];
-    colpar_climate<N>(st, perms, offs, kGenTemp, 4, 0.02500000037252903, 0.25, BX, BZ, n, t);
-    colpar_climate<N>(st, perms, offs, kGenRain, 4, 0.05000000074505806, 0.3333333333333333, BX, BZ, n, r);
-    colpar_climate<N>(st, perms, offs, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, n, c);
+    colpar_climate<N>(s, kGenTemp, 4, 0.02500000037252903, 0.25, BX, BZ, n, t);
+    colpar_climate<N>(s, kGenRain, 4, 0.05000000074505806, 0.3333333333333333, BX, BZ, n, r);
+    colpar_climate<N>(s, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, n, c);
 #pragma unroll
     for (int j = 0; j < N; ++j)
         if (j < n) climate_finish(t[j], r[j], c[j], temp[j], rain[j]);
 }
 
 // The three 3D octave groups of one lattice column over levels [Y0, Y1) (noise2, noise3,
-// noise1: generateTerrainNoise's evaluation order, BetaTerrain173.java:678-737).
-template <int Y0, int Y1>
-struct Acc3 {
-    double n1[Y1 - Y0], n2[Y1 - Y0], n3[Y1 - Y0];
-};
-
-template <int Y0, int Y1>
-__device__ __forceinline__ void colpar_noise3(GenStage& st, const uint32_t* perms, const double* offs, double X,
-                                              double Z, bool active, Acc3<Y0, Y1>& a) {
-    colpar_3d<Y0, Y1>(st, perms, offs, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2);
-    colpar_3d<Y0, Y1>(st, perms, offs, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3);
-    colpar_3d<Y0, Y1>(st, perms, offs, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, a.n1);
-}
-
-// The two 2D octave groups (noise4, noise5) of one lattice column.
-__device__ __forceinline__ void colpar_noise2d(GenStage& st, const uint32_t* perms, const double* offs, double X,
-                                               double Z, bool active, double& n4, double& n5) {
-    n4 = colpar_2d(st, perms, offs, kGenN4, 10, 1.121, X, Z, active);
-    n5 = colpar_2d(st, perms, offs, kGenN5, 16, 200.0, X, Z, active);
-}
-
-template <int Y0, int Y1, bool WITH_2D>
-__device__ __forceinline__ void colpar_noise(GenStage& st, const uint32_t* perms, const double* offs, double X,
-                                             double Z, bool active, ColAcc<Y0, Y1>& a) {
-    Acc3<Y0, Y1> t;
-    colpar_noise3<Y0, Y1>(st, perms, offs, X, Z, active, t);
-#pragma unroll
-    for (int k = 0; k < Y1 - Y0; ++k) {
-        a.n1[k] = t.n1[k];
-        a.n2[k] = t.n2[k];
-        a.n3[k] = t.n3[k];
+// noise1: generateTerrainNoise's evaluation order, BetaTerrain173.java:678-737), then the
+// two 2D groups (noise4, noise5) when WITH_2D.
+template <int Y0, int Y1, bool WITH_2D, class S>
+__device__ __forceinline__ void colpar_noise(const S& s, double X, double Z, bool active, ColAcc<Y0, Y1>& a) {
+    colpar_3d<Y0, Y1>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2);
+    colpar_3d<Y0, Y1>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3);
+    colpar_3d<Y0, Y1>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, a.n1);
+    if (WITH_2D) {
+        a.n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
+        a.n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
     }
-    if (WITH_2D) colpar_noise2d(st, perms, offs, X, Z, active, a.n4, a.n5);
 }
 
 }  // namespace bsf

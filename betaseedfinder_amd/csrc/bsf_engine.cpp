@@ -1,0 +1,460 @@
+// bsf_engine.cpp -- the island search engine's host side (see bsf_engine.hpp).
+//
+// A pass is launched as one captured hipGraph (or eagerly, when launch timing is on):
+//   reset -> build -> cache -> Stage 1 -> Stage 2 -> flood x 4
+// followed by asynchronous copies of its counters, the job table and the first hits into
+// pinned memory.  Two result buffer sets alternate, so pass k+1 runs while the host reads
+// pass k.
+#include "bsf_engine.hpp"
+
+#include "bsf_cache.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace bsf {
+
+namespace {
+
+template <class K>
+int occupancy_grid(K kernel, int block, size_t dyn_lds, int cap_per_cu) {
+    int per_cu = 0;
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_lds), "occupancy query");
+    if (per_cu < 1) per_cu = 1;
+    if (cap_per_cu > 0 && per_cu > cap_per_cu) per_cu = cap_per_cu;
+    return per_cu * g_ctx.cu_count;
+}
+
+size_t flood_lds_bytes(int cls) {
+    switch (cls) {
+        case kFloodClassSmall: return flood_words(64, 1) * 8;
+        case kFloodClassWideX: return flood_words(128, 1) * 8;
+        case kFloodClassWideZ: return flood_words(64, 2) * 8;
+        default: return 0;
+    }
+}
+
+template <class T>
+T* pinned(size_t n) {
+    void* p = nullptr;
+    check(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<T*>(p);
+}
+
+int bucket_of(int nd) { return nd <= kNdBucket[0] ? 0 : 1; }
+
+}  // namespace
+
+void IslandEngine::release() {
+    drop_graphs();
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &perms_, &offs_, &surv_slot_, &surv_seed_, &surv_job_, &lattice_,
+                            &flood_list_, &flood_scratch_, &ctr_[0], &ctr_[1], &hits_[0], &hits_[1]})
+        b->release();
+    auto free_pinned = [](void* p) {
+        if (p) (void)hipHostFree(p);
+    };
+    free_pinned(h_jobs_);
+    h_jobs_ = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        free_pinned(h_snap_[i]);
+        free_pinned(h_ctr_[i]);
+        free_pinned(h_hits_[i]);
+        free_pinned(h_stage1_[i]);
+        h_snap_[i] = nullptr;
+        h_ctr_[i] = nullptr;
+        h_hits_[i] = nullptr;
+        h_stage1_[i] = nullptr;
+        if (done_[i]) (void)hipEventDestroy(done_[i]);
+        done_[i] = nullptr;
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
+    stream_ = copy_stream_ = nullptr;
+    host_jobs_.clear();
+    cfg_slots_ = 0;
+    cfg_jobs_ = 0;
+    inflight_n_ = 0;
+    next_ab_ = oldest_ab_ = 0;
+    host_rr_ = 0;
+}
+
+void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, bool timers) {
+    if (pass_slots < 64 || pass_slots > (int64_t(1) << 22)) throw ArgError("pass size must be 64 .. 2^22 seeds");
+    if (max_jobs < 1 || max_jobs > 65536) throw ArgError("job table size must be 1 .. 65536");
+    if (min_quantum < 1) throw ArgError("minimum share must be >= 1");
+    release();
+    cfg_slots_ = pass_slots;
+    cfg_jobs_ = max_jobs;
+    cfg_quantum_ = min_quantum;
+    timers_ = timers;
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t P = static_cast<size_t>(pass_slots);
+    jobs_.ensure(static_cast<size_t>(max_jobs) * sizeof(IslandJob));
+    segs_.ensure(2 * static_cast<size_t>(max_jobs) * sizeof(PassSeg));
+    desc_.ensure(sizeof(PassDesc));
+    perms_.ensure(P * kCachePermBytes);
+    offs_.ensure(P * kGenCount * 3 * sizeof(double));
+    surv_slot_.ensure(P * sizeof(int64_t));
+    surv_seed_.ensure(P * sizeof(int64_t));
+    surv_job_.ensure(P * sizeof(int32_t));
+    lattice_.ensure(P * static_cast<size_t>(kNdBucket[1]) * 8 * sizeof(double));
+    flood_list_.ensure(static_cast<size_t>(kFloodClasses) * P * sizeof(int32_t));
+    for (int i = 0; i < 2; ++i) {
+        ctr_[i].ensure(kCounters * sizeof(unsigned long long));
+        hits_[i].ensure(P * sizeof(bsf_island_hit));
+        check(hipMemset(ctr_[i].ptr, 0, kCounters * sizeof(unsigned long long)), "hipMemset");
+        h_snap_[i] = pinned<IslandJob>(static_cast<size_t>(max_jobs));
+        h_ctr_[i] = pinned<unsigned long long>(kCounters);
+        h_hits_[i] = pinned<bsf_island_hit>(P);
+        h_stage1_[i] = pinned<int64_t>(P);
+        check(hipEventCreateWithFlags(&done_[i], hipEventDisableTiming), "hipEventCreate");
+    }
+    h_jobs_ = pinned<IslandJob>(static_cast<size_t>(max_jobs));
+    std::memset(h_jobs_, 0, static_cast<size_t>(max_jobs) * sizeof(IslandJob));
+    check(hipMemset(jobs_.ptr, 0, static_cast<size_t>(max_jobs) * sizeof(IslandJob)), "hipMemset");
+    check(hipMemset(desc_.ptr, 0, sizeof(PassDesc)), "hipMemset");
+    host_jobs_.assign(static_cast<size_t>(max_jobs), HostJob{});
+
+    grid_cache_ = occupancy_grid(pass_cache_kernel, 64, 0, 0);
+    grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
+    grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
+    for (int b = 0; b < 2; ++b)
+        grid_s2_[b] = occupancy_grid(pass_stage2_kernel, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
+    grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, 256, flood_lds_bytes(0), 0);
+    grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, 256, flood_lds_bytes(1), 0);
+    grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, 256, flood_lds_bytes(2), 0);
+    grid_flood_[kFloodClassLarge] = occupancy_grid(pass_flood_kernel<2, true>, 256, 0, 2);
+    flood_scratch_words_ = static_cast<int64_t>(flood_words(128, 2));
+    flood_scratch_.ensure(static_cast<size_t>(grid_flood_[kFloodClassLarge]) * flood_scratch_words_ * 8);
+    check(hipDeviceSynchronize(), "engine setup");
+}
+
+int IslandEngine::free_slots() const {
+    int n = 0;
+    for (const HostJob& j : host_jobs_) n += !j.used;
+    return n;
+}
+
+bool IslandEngine::has_work() const {
+    for (const HostJob& j : host_jobs_)
+        if (j.used && j.cursor < j.count) return true;
+    return false;
+}
+
+int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64_t* d_seeds, int64_t count,
+                        const bsf_island_params& p, const Tuning& tuning) {
+    int slot = -1;
+    for (int i = 0; i < cfg_jobs_; ++i)
+        if (!host_jobs_[i].used) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) return -1;
+    IslandJob j{};
+    j.first_seed = first_seed;
+    j.seeds = d_seeds;
+    j.count = count;
+    j.cursor = 0;
+    j.hits = 0;
+    j.p = p;
+    j.tag = tag;
+    j.gate = tuning.gate;
+    j.worlds_per_wave = tuning.gate ? std::max(1, std::min(tuning.worlds_per_wave, kMaxWorldsPerWave)) : 64;
+    if (tuning.gate && j.worlds_per_wave * 4 * p.chunks_x * p.chunks_z > 64) j.worlds_per_wave = 1;
+    j.flood_class = flood_class_of(p);
+    h_jobs_[slot] = j;
+    HostJob& h = host_jobs_[slot];
+    h = HostJob{};
+    h.used = true;
+    h.id = id;
+    h.tag = tag;
+    h.count = count;
+    h.worlds_per_wave = j.worlds_per_wave;
+    h.upload = true;
+    h.nd = 25 * p.chunks_x * p.chunks_z;
+    return slot;
+}
+
+void IslandEngine::retire(int slot) {
+    HostJob& h = host_jobs_[slot];
+    if (!h.used) return;
+    if (h.done < h.count) throw ArgError("retiring a job that is still running");
+    h = HostJob{};
+    // the device entry keeps cursor == count until a new job overwrites it: pass_build skips it
+}
+
+// Host replica of pass_build_kernel's allocation rule (same order, same arithmetic).
+int64_t IslandEngine::plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
+                                int64_t seeds_kind[2]) {
+    const int E = cfg_jobs_;
+    int64_t active = 0;
+    for (const HostJob& j : host_jobs_) active += (j.used && j.cursor < j.count) ? 1 : 0;
+    if (active == 0) return 0;
+    const int64_t P = cfg_slots_;
+    int64_t q = (P + active - 1) / active;
+    if (q < cfg_quantum_) q = cfg_quantum_;
+    int64_t o = 0, seeds = 0;
+    int last = -1;
+    for (int r = 0; r < E; ++r) {
+        const int ji = (host_rr_ + r) % E;
+        HostJob& j = host_jobs_[ji];
+        const int64_t rem = j.used ? j.count - j.cursor : 0;
+        const int64_t want = rem > 0 ? std::min(rem, q) : 0;
+        int64_t take = P - o;
+        take = take < 0 ? 0 : std::min(take, want);
+        if (take > 0) {
+            j.cursor += take;
+            seeds += take;
+            seeds_kind[stage1_kind(j.worlds_per_wave)] += take;
+            cursors.emplace_back(ji, j.cursor);
+            if (j.cursor == j.count) finished.push_back(ji);
+            last = r;
+        }
+        o += want;
+    }
+    if (last >= 0) host_rr_ = (host_rr_ + last + 1) % E;
+    return seeds;
+}
+
+EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
+    EngineArgs a{};
+    a.jobs = static_cast<IslandJob*>(jobs_.ptr);
+    a.max_jobs = cfg_jobs_;
+    a.min_quantum = cfg_quantum_;
+    a.pass_slots = cfg_slots_;
+    a.desc = static_cast<PassDesc*>(desc_.ptr);
+    a.segs = static_cast<PassSeg*>(segs_.ptr);
+    a.perms = static_cast<uint32_t*>(perms_.ptr);
+    a.offsets = static_cast<double*>(offs_.ptr);
+    a.ctr = static_cast<unsigned long long*>(ctr_[ab].ptr);
+    a.surv_slot = static_cast<int64_t*>(surv_slot_.ptr);
+    a.surv_seed = static_cast<int64_t*>(surv_seed_.ptr);
+    a.surv_job = static_cast<int32_t*>(surv_job_.ptr);
+    a.lattice = static_cast<double*>(lattice_.ptr);
+    a.lat_stride = static_cast<int64_t>(kNdBucket[1]) * 8;
+    a.flood_list = static_cast<int32_t*>(flood_list_.ptr);
+    a.hits = static_cast<bsf_island_hit*>(hits_[ab].ptr);
+    a.flood_scratch = static_cast<unsigned long long*>(flood_scratch_.ptr);
+    a.flood_scratch_words = flood_scratch_words_;
+    a.stage1_only = stage1_only ? 1 : 0;
+    return a;
+}
+
+void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
+    const EngineArgs a = args_for(ab, stage1_only);
+    hipStream_t s = stream_;
+    const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
+    hipLaunchKernelGGL(pass_reset_kernel, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(pass_build_kernel, dim3(1), dim3(256), 0, s, a);
+    check(hipGetLastError(), "pass build launch");
+    {
+        LaunchTimer t("island_cache", s, timers_);
+        hipLaunchKernelGGL(pass_cache_kernel, dim3(grid_cache_), dim3(64), 0, s, a);
+        check(hipGetLastError(), "cache launch");
+        t.stop(seeds);
+    }
+    {
+        // units: the pass's seeds of one-world-per-wave jobs (the other form times separately)
+        LaunchTimer t("island_stage1", s, timers_);
+        auto s1_one = pass_stage1_kernel<0>;
+        hipLaunchKernelGGL(s1_one, dim3(grid_s1_[0]), dim3(64), 0, s, a);
+        check(hipGetLastError(), "stage 1 launch");
+        t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[0]));
+    }
+    {
+        LaunchTimer t("island_stage1_multi", s, timers_);
+        auto s1_multi = pass_stage1_kernel<1>;
+        hipLaunchKernelGGL(s1_multi, dim3(grid_s1_[1]), dim3(64), 0, s, a);
+        check(hipGetLastError(), "stage 1 launch");
+        t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[1]));
+    }
+    if (stage1_only) return;
+    {
+        LaunchTimer t("island_stage2", s, timers_);
+        hipLaunchKernelGGL(pass_stage2_kernel, dim3(grid_s2_[bucket]), dim3(64),
+                           static_cast<size_t>(2 * kNdBucket[bucket]) * sizeof(double), s, a);
+        check(hipGetLastError(), "stage 2 launch");
+        t.stop(0);
+    }
+    {
+        LaunchTimer t("island_flood", s, timers_);
+        auto flood1 = pass_flood_kernel<1, false>;
+        auto flood2 = pass_flood_kernel<2, false>;
+        auto flood2g = pass_flood_kernel<2, true>;
+        hipLaunchKernelGGL(flood1, dim3(grid_flood_[0]), dim3(256), flood_lds_bytes(0), s, a,
+                           static_cast<int>(kFloodClassSmall));
+        hipLaunchKernelGGL(flood1, dim3(grid_flood_[1]), dim3(256), flood_lds_bytes(1), s, a,
+                           static_cast<int>(kFloodClassWideX));
+        hipLaunchKernelGGL(flood2, dim3(grid_flood_[2]), dim3(256), flood_lds_bytes(2), s, a,
+                           static_cast<int>(kFloodClassWideZ));
+        hipLaunchKernelGGL(flood2g, dim3(grid_flood_[3]), dim3(256), 0, s, a, static_cast<int>(kFloodClassLarge));
+        check(hipGetLastError(), "flood launch");
+        t.stop(0);
+    }
+}
+
+void IslandEngine::drop_graphs() {
+    for (auto& kv : graphs_) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
+    graphs_.clear();
+}
+
+void IslandEngine::launch(bool stage1_only) {
+    if (!configured()) throw ArgError("island engine not configured");
+    if (inflight_n_ >= 2) throw ArgError("two passes already in flight");
+    const int ab = next_ab_;
+    hipStream_t s = stream_;
+    Flight& f = flight_[ab];
+    f = Flight{};
+    f.seq = ++seq_;
+    f.stage1_only = stage1_only;
+    // upload admitted jobs, then plan the pass exactly as pass_build will
+    int nd_max = 0;
+    for (int i = 0; i < cfg_jobs_; ++i) {
+        HostJob& h = host_jobs_[i];
+        if (h.used && h.upload) {
+            check(hipMemcpyAsync(static_cast<IslandJob*>(jobs_.ptr) + i, h_jobs_ + i, sizeof(IslandJob),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D job entry");
+            h.upload = false;
+        }
+        if (h.used && h.cursor < h.count) nd_max = std::max(nd_max, h.nd);
+    }
+    f.seeds = plan_pass(f.cursors, f.finished, f.seeds_kind);
+    const int bucket = bucket_of(nd_max);
+    if (timers_ && timing_enabled()) {
+        enqueue_kernels(ab, stage1_only, bucket);
+    } else if (!use_graphs_) {
+        enqueue_kernels(ab, stage1_only, bucket);
+    } else {
+        const int key = ab | (stage1_only ? 2 : 0) | (bucket << 2);
+        auto it = graphs_.find(key);
+        if (it == graphs_.end()) {
+            const bool saved = timers_;
+            timers_ = false;
+            hipGraph_t g = nullptr;
+            check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_kernels(ab, stage1_only, bucket);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                timers_ = saved;
+                throw;
+            }
+            check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+            timers_ = saved;
+            hipGraphExec_t ge = nullptr;
+            check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+            it = graphs_.emplace(key, std::make_pair(g, ge)).first;
+        }
+        check(hipGraphLaunch(it->second.second, s), "hipGraphLaunch");
+    }
+    check(hipMemcpyAsync(h_ctr_[ab], ctr_[ab].ptr, kCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+          "D2H counters");
+    check(hipMemcpyAsync(h_snap_[ab], jobs_.ptr, static_cast<size_t>(cfg_jobs_) * sizeof(IslandJob),
+                         hipMemcpyDeviceToHost, s),
+          "D2H job table");
+    const int64_t prefix = std::min(kHitPrefix, cfg_slots_);
+    check(hipMemcpyAsync(h_hits_[ab], hits_[ab].ptr, static_cast<size_t>(prefix) * sizeof(bsf_island_hit),
+                         hipMemcpyDeviceToHost, s),
+          "D2H hits");
+    if (stage1_only)
+        check(hipMemcpyAsync(h_stage1_[ab], surv_seed_.ptr, static_cast<size_t>(cfg_slots_) * sizeof(int64_t),
+                             hipMemcpyDeviceToHost, s),
+              "D2H stage 1 survivors");
+    check(hipEventRecord(done_[ab], s), "hipEventRecord");
+    next_ab_ ^= 1;
+    ++inflight_n_;
+}
+
+void IslandEngine::complete_oldest(PassResult& out) {
+    if (inflight_n_ == 0) throw ArgError("no pass in flight");
+    const int ab = oldest_ab_;
+    const Flight& f = flight_[ab];
+    check(hipEventSynchronize(done_[ab]), "pass completion");
+    const unsigned long long* c = h_ctr_[ab];
+    const int64_t nh = std::min<int64_t>(static_cast<int64_t>(c[kCtrHits]), cfg_slots_);
+    const int64_t ns = std::min<int64_t>(static_cast<int64_t>(c[kCtrSurv]), cfg_slots_);
+    const int64_t prefix = std::min(kHitPrefix, cfg_slots_);
+    if (nh > prefix) {
+        // the device buffer of this pass is not reused before the pass after next
+        check(hipMemcpyAsync(h_hits_[ab] + prefix, static_cast<bsf_island_hit*>(hits_[ab].ptr) + prefix,
+                             static_cast<size_t>(nh - prefix) * sizeof(bsf_island_hit), hipMemcpyDeviceToHost,
+                             copy_stream_),
+              "D2H hits");
+        check(hipStreamSynchronize(copy_stream_), "D2H hits");
+    }
+    out = PassResult{};
+    out.seq = f.seq;
+    out.seeds = f.seeds;
+    out.survivors = ns;
+    out.hits.assign(h_hits_[ab], h_hits_[ab] + nh);
+    if (f.stage1_only) out.stage1.assign(h_stage1_[ab], h_stage1_[ab] + ns);
+    // the device advanced the same cursors as the host's replica of pass_build
+    for (const auto& sc : f.cursors) {
+        const IslandJob& dj = h_snap_[ab][sc.first];
+        if (static_cast<int64_t>(dj.cursor) != sc.second)
+            throw HipError("pass " + std::to_string(f.seq) + ": device job cursor " + std::to_string(dj.cursor) +
+                           " differs from the planned " + std::to_string(sc.second));
+        HostJob& h = host_jobs_[sc.first];
+        h.done = sc.second;
+        h.hits = dj.hits;
+    }
+    out.finished = f.finished;
+    stats.passes += 1;
+    stats.seeds += static_cast<uint64_t>(f.seeds);
+    stats.survivors += static_cast<uint64_t>(ns);
+    stats.hits += static_cast<uint64_t>(nh);
+    oldest_ab_ ^= 1;
+    --inflight_n_;
+}
+
+void IslandEngine::drive(bool stage1_only, const std::function<void(PassResult&)>& on_pass) {
+    PassResult r;
+    for (;;) {
+        while (inflight_n_ < 2 && has_work()) launch(stage1_only);
+        if (inflight_n_ == 0) break;
+        complete_oldest(r);
+        on_pass(r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// library engine and tuning table
+// ---------------------------------------------------------------------------
+static IslandEngine g_engine;
+static std::map<std::pair<int, int>, Tuning> g_tuning;
+
+IslandEngine& engine() { return g_engine; }
+
+void ensure_engine() {
+    if (!g_engine.configured()) g_engine.configure(kDefaultPassSlots, kDefaultMaxJobs, kDefaultMinQuantum, true);
+}
+
+Tuning tuning_for(const bsf_island_params& p) {
+    auto it = g_tuning.find({p.chunks_x, p.chunks_z});
+    if (it != g_tuning.end()) return it->second;
+    Tuning t;
+    t.worlds_per_wave = default_worlds_per_wave(p);
+    t.gate = 1;
+    return t;
+}
+
+void set_tuning(const bsf_island_params& p, const Tuning& t) { g_tuning[{p.chunks_x, p.chunks_z}] = t; }
+
+void validate_island_params(const bsf_island_params* p) {
+    if (!p) throw ArgError("null island params");
+    if (p->chunks_x < 1 || p->chunks_z < 1 || p->chunks_x > 8 || p->chunks_z > 8)
+        throw ArgError("island window must be 1..8 chunks on each axis");
+    const long long lim = 1LL << 24;
+    if (std::llabs(static_cast<long long>(p->chunk_x0)) > lim || std::llabs(static_cast<long long>(p->chunk_z0)) > lim)
+        throw ArgError("island window origin out of range");
+}
+
+}  // namespace bsf

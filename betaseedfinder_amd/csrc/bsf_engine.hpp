@@ -1,0 +1,147 @@
+// bsf_engine.hpp -- host side of the island search engine: device job table, pass buffers,
+// captured pass graphs, the pass driver shared by the plain, batched and served entry points,
+// and the autotuner's table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "../../include/bsf.h"
+#include "bsf_host.hpp"
+#include "bsf_island.hpp"
+
+namespace bsf {
+
+// What the host keeps about a job-table slot.  The host replays pass_build's allocation
+// rule exactly (cursor), so it knows every pass's seeds at launch; the device cursor is
+// checked against it when the pass completes.
+struct HostJob {
+    bool used = false;
+    int64_t id = -1;                 // caller's job id
+    int32_t tag = 0;
+    int64_t count = 0;
+    int worlds_per_wave = 1;
+    int64_t cursor = 0;              // seeds handed to launched passes
+    int64_t done = 0;                // seeds of completed passes
+    unsigned long long hits = 0;     // device hit counter at the last completed pass
+    bool upload = false;             // entry still to be copied to the device
+    int nd = 0;                      // density columns of the window
+};
+
+struct EngineStats {
+    uint64_t passes = 0, seeds = 0, survivors = 0, hits = 0;
+};
+
+// Tuned per-window settings (keyed by chunks_x, chunks_z).
+struct Tuning {
+    int worlds_per_wave = 1;
+    int gate = 1;
+};
+
+class IslandEngine {
+   public:
+    // (Re)allocates for `pass_slots` seeds per pass and `max_jobs` resident jobs.
+    void configure(int64_t pass_slots, int max_jobs, int min_quantum, bool timers);
+    bool configured() const { return cfg_slots_ > 0; }
+    void release();
+
+    int64_t pass_slots() const { return cfg_slots_; }
+    int max_jobs() const { return cfg_jobs_; }
+    int free_slots() const;
+
+    // Puts a job into a free job-table slot (returns the slot, -1 when the table is full).
+    // d_seeds: device seed list that outlives the job (or nullptr for the range).
+    int admit(int64_t id, int32_t tag, int64_t first_seed, const int64_t* d_seeds, int64_t count,
+              const bsf_island_params& p, const Tuning& tuning);
+    // Returns a finished job's slot to the free pool.
+    void retire(int slot);
+    const HostJob& job(int slot) const { return host_jobs_[slot]; }
+    // True while some resident job has seeds not yet handed to a launched pass.
+    bool has_work() const;
+
+    // Result of one completed pass.
+    struct PassResult {
+        uint64_t seq = 0;
+        std::vector<bsf_island_hit> hits;        // hits of the pass (window_index = job tag)
+        std::vector<int64_t> stage1;             // Stage-1 survivor seeds (stage1_only passes)
+        std::vector<int> finished;               // slots whose job completed with this pass
+        int64_t seeds = 0, survivors = 0;
+    };
+
+    // Runs passes until every resident job is done; `on_pass` sees each result in order.
+    void drive(bool stage1_only, const std::function<void(PassResult&)>& on_pass);
+
+    // Pipelined primitives used by drive() and by the server loop: at most two passes in
+    // flight, each with its own result buffers.
+    void launch(bool stage1_only);
+    int inflight() const { return inflight_n_; }
+    void complete_oldest(PassResult& out);
+
+    hipStream_t stream() const { return stream_; }
+    void set_graphs(bool on) { use_graphs_ = on; }
+
+    EngineStats stats;
+
+   private:
+    struct Flight {
+        uint64_t seq = 0;
+        bool stage1_only = false;
+        int64_t seeds = 0;
+        int64_t seeds_kind[2] = {0, 0};                // seeds by Stage-1 kind
+        std::vector<std::pair<int, int64_t>> cursors;  // (slot, cursor after this pass)
+        std::vector<int> finished;
+    };
+    void enqueue_kernels(int ab, bool stage1_only, int bucket);
+    void drop_graphs();
+    EngineArgs args_for(int ab, bool stage1_only) const;
+    int64_t plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
+                      int64_t seeds_kind[2]);
+
+    int64_t cfg_slots_ = 0;
+    int cfg_jobs_ = 0;
+    int cfg_quantum_ = 1024;
+    bool timers_ = false, use_graphs_ = true;
+    hipStream_t stream_ = nullptr, copy_stream_ = nullptr;
+    DeviceBuffer jobs_, segs_, desc_, perms_, offs_, surv_slot_, surv_seed_, surv_job_, lattice_, flood_list_,
+            flood_scratch_;
+    DeviceBuffer ctr_[2], hits_[2];
+    // pinned host memory
+    IslandJob* h_jobs_ = nullptr;
+    IslandJob* h_snap_[2] = {nullptr, nullptr};
+    unsigned long long* h_ctr_[2] = {nullptr, nullptr};
+    bsf_island_hit* h_hits_[2] = {nullptr, nullptr};
+    int64_t* h_stage1_[2] = {nullptr, nullptr};
+    hipEvent_t done_[2] = {nullptr, nullptr};
+    Flight flight_[2];
+    std::vector<HostJob> host_jobs_;
+    int host_rr_ = 0;
+    uint64_t seq_ = 0;
+    int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[2] = {0, 0}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
+    int64_t flood_scratch_words_ = 0;
+    int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | bucket << 2
+    static constexpr int64_t kHitPrefix = 4096;  // hits copied back with every pass
+};
+
+// Density columns per survivor of the two lattice layouts (4x4 and 8x8 windows).
+constexpr int kNdBucket[2] = {400, 1600};
+
+// The library's own engine (plain and batched entry points, under the library lock).
+IslandEngine& engine();
+void ensure_engine();
+
+constexpr int64_t kDefaultPassSlots = int64_t(1) << 17;
+constexpr int kDefaultMaxJobs = 1024;
+constexpr int kDefaultMinQuantum = 1024;
+
+void validate_island_params(const bsf_island_params* p);
+// Settings for a window: the autotuner's choice when it has one, else the default.
+Tuning tuning_for(const bsf_island_params& p);
+void set_tuning(const bsf_island_params& p, const Tuning& t);
+
+}  // namespace bsf

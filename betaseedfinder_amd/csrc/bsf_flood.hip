@@ -1,0 +1,474 @@
+// bsf_flood.hip -- island flood of the search engine (gfx950): one 256-thread workgroup per
+// Stage-2 survivor, block bitsets of the window in LDS (or, for the largest windows, in a
+// per-workgroup global scratch that stays in L2).
+//
+// Bitsets hold one z-mask per (x, y) row for y in [68, 127]: W 64-bit words per row (W = 2
+// when the window is wider than 4 chunks in z).  The flood reproduces
+// IslandSearchEngine.findFloatingIslandInSeed's exact block scan (:2861-2977): candidate
+// starts in Java order, 6-neighbour components above y = 67 (floodFillFlat, :4096-4202),
+// isMatch (:3972-4003) and compareComponents (:4079-4094).
+#include <hip/hip_runtime.h>
+
+#include "bsf_island.hpp"
+
+namespace bsf {
+
+template <int W>
+struct Z {
+    unsigned long long v[W];
+};
+
+template <int W>
+__device__ __forceinline__ Z<W> zload(const unsigned long long* p) {
+    Z<W> z;
+#pragma unroll
+    for (int i = 0; i < W; ++i) z.v[i] = p[i];
+    return z;
+}
+template <int W>
+__device__ __forceinline__ void zstore(unsigned long long* p, const Z<W>& z) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) p[i] = z.v[i];
+}
+template <int W>
+__device__ __forceinline__ Z<W> zor(const Z<W>& a, const Z<W>& b) {
+    Z<W> r;
+#pragma unroll
+    for (int i = 0; i < W; ++i) r.v[i] = a.v[i] | b.v[i];
+    return r;
+}
+template <int W>
+__device__ __forceinline__ Z<W> zand(const Z<W>& a, const Z<W>& b) {
+    Z<W> r;
+#pragma unroll
+    for (int i = 0; i < W; ++i) r.v[i] = a.v[i] & b.v[i];
+    return r;
+}
+template <int W>
+__device__ __forceinline__ bool zeq(const Z<W>& a, const Z<W>& b) {
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < W; ++i) e = e && a.v[i] == b.v[i];
+    return e;
+}
+template <int W>
+__device__ __forceinline__ bool zany(const Z<W>& a) {
+    unsigned long long o = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) o |= a.v[i];
+    return o != 0;
+}
+template <int W>
+__device__ __forceinline__ int zpop(const Z<W>& a) {
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) n += __popcll(a.v[i]);
+    return n;
+}
+template <int W>
+__device__ __forceinline__ Z<W> zzero() {
+    Z<W> r;
+#pragma unroll
+    for (int i = 0; i < W; ++i) r.v[i] = 0;
+    return r;
+}
+
+// Bits of m lying in a run of consecutive ones of m that contains a bit of s (s within m):
+// the carry of m + s sweeps each run upward from its seed bits; the bit-reversed sum sweeps
+// downward.  With two words the carry crosses from word 0 into word 1.
+__device__ __forceinline__ Z<1> zfill(const Z<1>& s, const Z<1>& m) {
+    const unsigned long long up = ((m.v[0] + s.v[0]) ^ m.v[0]) & m.v[0];
+    const unsigned long long rm = __builtin_bitreverse64(m.v[0]);
+    const unsigned long long rs = __builtin_bitreverse64(s.v[0]);
+    const unsigned long long down = __builtin_bitreverse64(((rm + rs) ^ rm) & rm);
+    Z<1> r;
+    r.v[0] = up | down | s.v[0];
+    return r;
+}
+__device__ __forceinline__ void add128(unsigned long long a0, unsigned long long a1, unsigned long long b0,
+                                       unsigned long long b1, unsigned long long& r0, unsigned long long& r1) {
+    r0 = a0 + b0;
+    r1 = a1 + b1 + (r0 < a0 ? 1ULL : 0ULL);
+}
+__device__ __forceinline__ Z<2> zfill(const Z<2>& s, const Z<2>& m) {
+    unsigned long long u0, u1;
+    add128(m.v[0], m.v[1], s.v[0], s.v[1], u0, u1);
+    // reversed 128-bit value: word 0 <- reverse(word 1), word 1 <- reverse(word 0)
+    const unsigned long long rm0 = __builtin_bitreverse64(m.v[1]), rm1 = __builtin_bitreverse64(m.v[0]);
+    const unsigned long long rs0 = __builtin_bitreverse64(s.v[1]), rs1 = __builtin_bitreverse64(s.v[0]);
+    unsigned long long d0, d1;
+    add128(rm0, rm1, rs0, rs1, d0, d1);
+    d0 = (d0 ^ rm0) & rm0;
+    d1 = (d1 ^ rm1) & rm1;
+    Z<2> r;
+    r.v[0] = (((u0 ^ m.v[0]) & m.v[0]) | __builtin_bitreverse64(d1) | s.v[0]);
+    r.v[1] = (((u1 ^ m.v[1]) & m.v[1]) | __builtin_bitreverse64(d0) | s.v[1]);
+    return r;
+}
+
+// The eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks) from one lattice
+// y-cell, walking generateBareTerrain's incremental chain (BetaTerrain173.java:599-664) once:
+// d1..d4 step by d5..d8 per level, d10/d11 step i2 times by d12/d13 along x, d15 steps by
+// d16 along z.  The i2 steps are all formed and the right one selected, so lanes with
+// different x stay converged.
+template <int W>
+__device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, Z<W> rows[8]) {
+    const int cxi = x >> 4, lx = x & 15;
+    const int i1 = lx >> 2, i2 = lx & 3;
+#pragma unroll
+    for (int l1 = 0; l1 < 8; ++l1) rows[l1] = zzero<W>();
+    for (int czi = 0; czi < cz_n; ++czi) {
+        const double* cl = lat + static_cast<size_t>(cxi * cz_n + czi) * 25 * 8;
+        for (int j1 = 0; j1 < 4; ++j1) {
+            auto L = [&](int ii, int jj, int kk) -> double {
+                return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
+            };
+            double d1 = L(i1, j1, k1), d2 = L(i1, j1 + 1, k1), d3 = L(i1 + 1, j1, k1), d4 = L(i1 + 1, j1 + 1, k1);
+            const double d5 = (L(i1, j1, k1 + 1) - d1) * 0.125;
+            const double d6 = (L(i1, j1 + 1, k1 + 1) - d2) * 0.125;
+            const double d7 = (L(i1 + 1, j1, k1 + 1) - d3) * 0.125;
+            const double d8 = (L(i1 + 1, j1 + 1, k1 + 1) - d4) * 0.125;
+            const int shift = czi * 16 + j1 * 4;
+            const bool hi_word = W == 2 && shift >= 64;
+            const int sh = shift & 63;
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const double d12 = (d3 - d1) * 0.25;
+                const double d13 = (d4 - d2) * 0.25;
+                const double a1 = d1 + d12, a2 = a1 + d12, a3 = a2 + d12;
+                const double b1 = d2 + d13, b2 = b1 + d13, b3 = b2 + d13;
+                const double d10 = i2 == 0 ? d1 : (i2 == 1 ? a1 : (i2 == 2 ? a2 : a3));
+                const double d11 = i2 == 0 ? d2 : (i2 == 1 ? b1 : (i2 == 2 ? b2 : b3));
+                const double d16 = (d11 - d10) * 0.25;
+                double d15 = d10;
+                unsigned long long bits = 0;
+#pragma unroll
+                for (int k2 = 0; k2 < 4; ++k2) {
+                    if (d15 > 0.0) bits |= 1ULL << k2;
+                    d15 += d16;
+                }
+                if (hi_word) rows[l1].v[W - 1] |= bits << sh;
+                else rows[l1].v[0] |= bits << sh;
+                d1 += d5;
+                d2 += d6;
+                d3 += d7;
+                d4 += d8;
+            }
+        }
+    }
+}
+
+struct CompStats {
+    int blocks, columns, minx, maxx, minz, maxz, miny, maxy;
+    long long sumx, sumz;
+    int bottom, side;
+};
+
+__device__ __forceinline__ int sum_of_bit_positions(unsigned long long m) {
+    int s = 0;
+    s += 1 * __popcll(m & 0xAAAAAAAAAAAAAAAAULL);
+    s += 2 * __popcll(m & 0xCCCCCCCCCCCCCCCCULL);
+    s += 4 * __popcll(m & 0xF0F0F0F0F0F0F0F0ULL);
+    s += 8 * __popcll(m & 0xFF00FF00FF00FF00ULL);
+    s += 16 * __popcll(m & 0xFFFF0000FFFF0000ULL);
+    s += 32 * __popcll(m & 0xFFFFFFFF00000000ULL);
+    return s;
+}
+
+// Java Math.round of a double of small magnitude: floor(a + 1/2), ties towards +infinity.
+__device__ __forceinline__ int java_round(double a) {
+    const double f = floor(a);
+    int r = static_cast<int>(f);
+    if (a - f >= 0.5) ++r;
+    return r;
+}
+
+__device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats& b) {
+    // compareComponents(a, b) > 0, IslandSearchEngine.java:4079-4094
+    if (a.blocks != b.blocks) return a.blocks > b.blocks;
+    if (a.columns != b.columns) return a.columns > b.columns;
+    const int areaA = (a.maxx - a.minx + 1) * (a.maxz - a.minz + 1);
+    const int areaB = (b.maxx - b.minx + 1) * (b.maxz - b.minz + 1);
+    return areaA > areaB;
+}
+
+// Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
+// (x, y in [68, 127]) with z along the bits.  Rounds alternate two sweeps: thread x walks its
+// rows up and down in y, then thread y walks its rows up and down in x; z runs fill in one
+// step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is the
+// component and the order of the sweeps cannot change it.
+template <int W>
+__device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
+    const int t = threadIdx.x;
+    for (;;) {
+        int changed = 0;
+        if (t < nx) {
+            unsigned long long* row = comp + static_cast<size_t>(t) * kRowStride * W;
+            const unsigned long long* mrow = rem + static_cast<size_t>(t) * kRowStride * W;
+            const unsigned long long* left = t > 0 ? row - kRowStride * W : nullptr;
+            const unsigned long long* right = t < nx - 1 ? row + kRowStride * W : nullptr;
+            Z<W> prev = zzero<W>();
+            for (int y = 0; y < kLevels; ++y) {
+                const Z<W> m = zload<W>(mrow + y * W);
+                const Z<W> c = zload<W>(row + y * W);
+                Z<W> nb = zor(c, prev);
+                if (left) nb = zor(nb, zload<W>(left + y * W));
+                if (right) nb = zor(nb, zload<W>(right + y * W));
+                const Z<W> n = zfill(zand(nb, m), m);
+                if (!zeq(n, c)) {
+                    zstore(row + y * W, n);
+                    changed = 1;
+                }
+                prev = n;
+            }
+            prev = zzero<W>();
+            for (int y = kLevels - 1; y >= 0; --y) {
+                const Z<W> m = zload<W>(mrow + y * W);
+                const Z<W> c = zload<W>(row + y * W);
+                const Z<W> n = zfill(zand(zor(c, prev), m), m);
+                if (!zeq(n, c)) {
+                    zstore(row + y * W, n);
+                    changed = 1;
+                }
+                prev = n;
+            }
+        }
+        __syncthreads();
+        if (t < kLevels) {
+            Z<W> prev = zzero<W>();
+            for (int x = 0; x < nx; ++x) {
+                const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
+                const Z<W> m = zload<W>(rem + r);
+                const Z<W> c = zload<W>(comp + r);
+                Z<W> nb = zor(c, prev);
+                if (t > 0) nb = zor(nb, zload<W>(comp + r - W));
+                if (t < kLevels - 1) nb = zor(nb, zload<W>(comp + r + W));
+                const Z<W> n = zfill(zand(nb, m), m);
+                if (!zeq(n, c)) {
+                    zstore(comp + r, n);
+                    changed = 1;
+                }
+                prev = n;
+            }
+            prev = zzero<W>();
+            for (int x = nx - 1; x >= 0; --x) {
+                const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
+                const Z<W> m = zload<W>(rem + r);
+                const Z<W> c = zload<W>(comp + r);
+                const Z<W> n = zfill(zand(zor(c, prev), m), m);
+                if (!zeq(n, c)) {
+                    zstore(comp + r, n);
+                    changed = 1;
+                }
+                prev = n;
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
+template <int W, bool GLOBAL>
+__global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) {
+    extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
+    __shared__ CompStats s_cs;
+    __shared__ unsigned long long s_zor[W];
+    __shared__ int s_total;
+    __shared__ unsigned long long s_item;
+    const int tid = threadIdx.x;
+    if (a.stage1_only) return;
+    unsigned long long* const base =
+            GLOBAL ? a.flood_scratch + static_cast<size_t>(blockIdx.x) * a.flood_scratch_words : flood_lds;
+    const unsigned long long n_items = a.ctr[kCtrFloodCount + cls];
+    const int32_t* list = a.flood_list + static_cast<size_t>(cls) * a.pass_slots;
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.ctr[kCtrFloodNext + cls], 1ULL);
+        __syncthreads();
+        const unsigned long long item = s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const int64_t sv = list[item];
+        const int32_t jix = a.surv_job[sv];
+        const bsf_island_params p = a.jobs[jix].p;
+        const int nx = p.chunks_x * 16;
+        const int nz = p.chunks_z * 16;
+        const int rows = nx * kRowStride;
+        unsigned long long* const rem = base;
+        unsigned long long* const comp = base + static_cast<size_t>(rows) * W;
+        unsigned long long* const st67 = base + static_cast<size_t>(2 * rows) * W;
+        const int minWX = p.chunk_x0 * 16, minWZ = p.chunk_z0 * 16;
+        const double* lat = a.lattice + static_cast<size_t>(sv) * a.lat_stride;
+
+        // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
+        for (int task = tid; task < nx * 8; task += blockDim.x) {
+            const int x = task >> 3, k1 = 8 + (task & 7);
+            Z<W> r8[8];
+            stone_cell_rows<W>(lat, x, k1, p.chunks_z, r8);
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const int y = k1 * 8 + l1;
+                if (y >= kFloorY) zstore(rem + (static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W, r8[l1]);
+                else if (y == kFloorY - 1) zstore(st67 + static_cast<size_t>(x) * W, r8[l1]);
+            }
+            if (k1 == 8) zstore(rem + (static_cast<size_t>(x) * kRowStride + kLevels) * W, zzero<W>());  // row padding
+        }
+        __syncthreads();
+        // 2. every component that reaches a y = 68 block with stone below touches the bottom
+        //    and can never match (isMatch): grow them all at once from their ground contacts
+        //    and drop them, as Java's flood would mark them visited.
+        for (int r = tid; r < rows; r += blockDim.x) {
+            const int x = r / kRowStride, y = r % kRowStride;
+            const Z<W> v = y == 0 ? zand(zload<W>(rem + static_cast<size_t>(r) * W), zload<W>(st67 + static_cast<size_t>(x) * W))
+                                  : zzero<W>();
+            zstore(comp + static_cast<size_t>(r) * W, v);
+        }
+        __syncthreads();
+        grow_component<W>(comp, rem, nx);
+        if (tid == 0) s_total = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int r = tid; r < rows; r += blockDim.x) {
+            Z<W> left = zload<W>(rem + static_cast<size_t>(r) * W);
+            const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
+#pragma unroll
+            for (int i = 0; i < W; ++i) left.v[i] &= ~c.v[i];
+            zstore(rem + static_cast<size_t>(r) * W, left);
+            mine += zpop(left);
+        }
+        atomicAdd(&s_total, mine);
+        __syncthreads();
+        // 3. all remaining stone is floating; fewer blocks than minBlocks cannot hold a match
+        if (s_total < p.min_blocks) continue;
+
+        bool have_best = false;
+        CompStats best{};
+        // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
+        for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
+            const int cxi = ch / p.chunks_z, czi = ch % p.chunks_z;
+            for (int sc = 0; sc < 4; ++sc) {
+                const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
+                const int z = czi * 16 + ((sc & 1) ? 12 : 4);
+                const int zw = W == 2 ? (z >> 6) : 0;
+                const unsigned long long bitz = 1ULL << (z & 63);
+                for (int y = kFloorY; y < 128; ++y) {
+                    // stone not yet visited (visited or grounded blocks are no longer in rem)
+                    const unsigned long long here = rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw];
+                    if ((here & bitz) == 0) continue;
+                    // the block below is not stone: below y = 68 read the y = 67 row; above, a
+                    // stone block below would share this block's component, so it is still in rem
+                    const unsigned long long below =
+                            y == kFloorY ? st67[static_cast<size_t>(x) * W + zw]
+                                         : rem[(static_cast<size_t>(x) * kRowStride + (y - 1 - kFloorY)) * W + zw];
+                    if ((below & bitz) != 0) continue;
+                    for (int r = tid; r < rows * W; r += blockDim.x) comp[r] = 0;
+                    __syncthreads();
+                    if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw] = bitz;
+                    __syncthreads();
+                    grow_component<W>(comp, rem, nx);
+                    // statistics of the component
+                    if (tid == 0) {
+                        s_cs.blocks = 0;
+                        s_cs.columns = 0;
+                        s_cs.minx = 0x7fffffff;
+                        s_cs.maxx = -0x7fffffff;
+                        s_cs.miny = 0x7fffffff;
+                        s_cs.maxy = -0x7fffffff;
+                        s_cs.minz = 0;
+                        s_cs.maxz = 0;
+                        s_cs.sumx = 0;
+                        s_cs.sumz = 0;
+                        s_cs.bottom = 0;
+                        s_cs.side = 0;
+#pragma unroll
+                        for (int i = 0; i < W; ++i) s_zor[i] = 0;
+                    }
+                    __syncthreads();
+                    for (int rx = tid; rx < nx; rx += blockDim.x) {
+                        Z<W> col = zzero<W>();
+                        int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
+                        for (int ry = 0; ry < kLevels; ++ry) {
+                            const Z<W> c = zload<W>(comp + (static_cast<size_t>(rx) * kRowStride + ry) * W);
+                            if (zany(c)) {
+                                col = zor(col, c);
+                                blk += zpop(c);
+                                ymin = min(ymin, ry + kFloorY);
+                                ymax = max(ymax, ry + kFloorY);
+                            }
+                        }
+                        if (zany(col)) {
+                            const int ncol = zpop(col);
+                            long long zsum = static_cast<long long>(ncol) * minWZ + sum_of_bit_positions(col.v[0]);
+                            if (W == 2) zsum += sum_of_bit_positions(col.v[W - 1]) + 64LL * __popcll(col.v[W - 1]);
+                            atomicAdd(&s_cs.blocks, blk);
+                            atomicAdd(&s_cs.columns, ncol);
+                            atomicMin(&s_cs.minx, rx);
+                            atomicMax(&s_cs.maxx, rx);
+                            atomicMin(&s_cs.miny, ymin);
+                            atomicMax(&s_cs.maxy, ymax);
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumx),
+                                      static_cast<unsigned long long>(static_cast<long long>(ncol) * (minWX + rx)));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumz), static_cast<unsigned long long>(zsum));
+#pragma unroll
+                            for (int i = 0; i < W; ++i) atomicOr(&s_zor[i], col.v[i]);
+                            const Z<W> c0 = zload<W>(comp + static_cast<size_t>(rx) * kRowStride * W);
+                            if (zany(zand(c0, zload<W>(st67 + static_cast<size_t>(rx) * W)))) atomicOr(&s_cs.bottom, 1);
+                            if (rx == 0 || rx == nx - 1) atomicOr(&s_cs.side, 1);
+                        }
+                    }
+                    for (int r = tid; r < rows * W; r += blockDim.x) rem[r] &= ~comp[r];
+                    __syncthreads();
+                    CompStats cs = s_cs;
+                    const unsigned long long z0 = s_zor[0], z1 = s_zor[W - 1];
+                    if (W == 1 || z0 != 0) cs.minz = __ffsll(static_cast<long long>(z0)) - 1;
+                    else cs.minz = 64 + __ffsll(static_cast<long long>(z1)) - 1;
+                    if (W == 2 && z1 != 0) cs.maxz = 64 + 63 - __clzll(static_cast<long long>(z1));
+                    else cs.maxz = 63 - __clzll(static_cast<long long>(z0));
+                    if (cs.minz == 0 || cs.maxz == nz - 1) cs.side = 1;
+                    // isMatch, IslandSearchEngine.java:3972-4003
+                    const int width = cs.maxx - cs.minx + 1;
+                    const int depth = cs.maxz - cs.minz + 1;
+                    const bool match = cs.blocks >= p.min_blocks && cs.columns >= p.min_columns &&
+                                       cs.maxy >= p.min_y_for_match && !cs.bottom && width >= p.min_width &&
+                                       depth >= p.min_depth &&
+                                       cs.blocks / static_cast<double>(cs.columns > 1 ? cs.columns : 1) >=
+                                               p.min_avg_thickness;
+                    if (match && (!have_best || comp_better(cs, best))) {
+                        best = cs;
+                        have_best = true;
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (tid == 0 && have_best) {
+            const unsigned long long slot = atomicAdd(&a.ctr[kCtrHits], 1ULL);
+            atomicAdd(&a.jobs[jix].hits, 1ULL);
+            if (static_cast<int64_t>(slot) < a.pass_slots) {
+                bsf_island_hit h;
+                h.seed = a.surv_seed[sv];
+                h.min_x = minWX + best.minx;
+                h.max_x = minWX + best.maxx;
+                h.min_z = minWZ + best.minz;
+                h.max_z = minWZ + best.maxz;
+                h.min_y = best.miny;
+                h.max_y = best.maxy;
+                h.blocks = best.blocks;
+                h.columns = best.columns;
+                h.center_x = java_round(static_cast<double>(best.sumx) / best.columns);
+                h.center_z = java_round(static_cast<double>(best.sumz) / best.columns);
+                h.chunk_x = h.center_x >> 4;
+                h.chunk_z = h.center_z >> 4;
+                h.flags = best.side ? 1 : 0;
+                h.window_index = a.jobs[jix].tag;
+                a.hits[slot] = h;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void pass_flood_kernel<1, false>(EngineArgs, int);
+template __global__ void pass_flood_kernel<2, false>(EngineArgs, int);
+template __global__ void pass_flood_kernel<2, true>(EngineArgs, int);
+
+}  // namespace bsf

@@ -67,10 +67,11 @@ struct LaunchTimer {
     hipEvent_t start = nullptr;
     hipStream_t stream = nullptr;
     bool active = false;
-    LaunchTimer(const char* n, hipStream_t s);
+    LaunchTimer(const char* n, hipStream_t s, bool allow = true);
     void stop(uint64_t units);
 };
 void resolve_timers();
+bool timing_enabled();
 
 void require_init();
 

@@ -1,84 +1,356 @@
-// bsf_island.hip -- column-parallel island Stage 1 / Stage 2 over the generator cache (gfx950).
+// bsf_island.hip -- island search engine passes (gfx950): pass construction from the device
+// job table, per-world generator cache, Stage 1 and Stage 2.
 //
-// Stage 1 (one 64-lane wave per seed): the four sample columns of every chunk of the window
-// (blocks 4/12 x 4/12, IslandSearchEngine.java:4005-4007) are lattice columns 1 and 3 of the
-// chunk; a candidate start at y >= 68 over non-stone (:2909-2925) exists iff the y-chain
-// d1 += d5 of generateBareTerrain (BetaTerrain173.java:599-664) at such a column turns stone
-// after a non-stone block.  Every hit contains a candidate start, so this gate is exact.
+// Stage 1 (one 64-lane wave per work item): the four sample columns of every chunk of the
+// window (blocks 4/12 x 4/12, IslandSearchEngine.java:4005-4007) are lattice columns 1 and 3
+// of the chunk; a candidate start at y >= 68 over non-stone (:2909-2925) exists iff the
+// y-chain d1 += d5 of generateBareTerrain (BetaTerrain173.java:599-664) at such a column
+// turns stone after a non-stone block.  Every hit contains a candidate start, so this gate is
+// exact.  Windows with at most 32 sample columns put several worlds side by side in one wave.
 //
-// Stage 2 (one workgroup per survivor): the shared (4w+1) x (4h+1) noise lattice, levels
-// 8..15 (blocks 64..127; level 16 is -10 after the taper), one column per lane, then the
-// 25 per-chunk density columns with each chunk's own climate sample.
-#include "bsf_island.hpp"
-
+// Stage 2 (one wave per survivor): the shared (4w+1) x (4h+1) noise lattice, levels 8..15
+// (blocks 64..127; level 16 is -10 after the taper), one column per lane, then the 25
+// per-chunk density columns with each chunk's own climate sample.
 #include <hip/hip_runtime.h>
+
+#include "bsf_cache.hpp"
+#include "bsf_island.hpp"
 
 namespace bsf {
 
-__device__ __forceinline__ int64_t island_seed(const IslandArgs& a, int64_t slot) {
-    return a.seeds != nullptr ? a.seeds[slot] : a.first_seed + slot;
+__constant__ DivMagic isl_magic[257];
+
+hipError_t upload_island_tables() {
+    DivMagic t[257] = {};
+    for (uint32_t d = 1; d <= 256; ++d) {
+        uint32_t k = 0;
+        while ((2u << k) <= d) ++k;
+        if ((d & (d - 1)) == 0) {
+            t[d].shift = 31 - k;
+        } else {
+            t[d].m = static_cast<uint32_t>(((1ULL << (32 + k)) + d - 1) / d);
+            t[d].shift = k;
+        }
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(isl_magic), t, sizeof(t));
 }
 
-__global__ __launch_bounds__(64) void island_s1_kernel(IslandArgs a) {
-    __shared__ GenStage st;
-    const int tid = threadIdx.x;
-    const int cz_n = a.p.chunks_z;
-    const int ncols = 4 * a.p.chunks_x * cz_n;
-    for (int64_t slot = blockIdx.x; slot < a.count; slot += gridDim.x) {
-        const uint32_t* perms = a.perms + static_cast<size_t>(slot) * kGenCount * 64;
-        const double* offs = a.offsets + static_cast<size_t>(slot) * kGenCount * 3;
-        bool found = false;
-        for (int base = 0; base < ncols; base += 64) {
-            const int c = base + tid;
-            const bool active = c < ncols;
-            const int ch = active ? c >> 2 : 0, sc = c & 3;
-            const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
-            const int cx = a.p.chunk_x0 + ch / cz_n, cz = a.p.chunk_z0 + ch % cz_n;
-            const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
-            // the chunk's climate sample of lattice (i, l): temperatures[(3i+1)*16 + 3l+1]
-            const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
-            ColAcc<8, 16> acc;
-            colpar_noise<8, 16, true>(st, perms, offs, X, Z, active, acc);
-            double temp = 0.0, rain = 0.0;
-            colpar_climate_all<1>(st, perms, offs, &BX, &BZ, active ? 1 : 0, &temp, &rain);
-            const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
-            double lat[9];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) lat[k] = lattice_density(cs, 8 + k, acc.n1[k], acc.n2[k], acc.n3[k]);
-            lat[8] = -10.0;
-            if (active) {
-                bool below = false;
-#pragma unroll
-                for (int k1 = 0; k1 < 8; ++k1) {
-                    double d1 = lat[k1];
-                    const double d5 = (lat[k1 + 1] - d1) * 0.125;
-#pragma unroll
-                    for (int l1 = 0; l1 < 8; ++l1) {
-                        const bool stone = d1 > 0.0;
-                        if (64 + k1 * 8 + l1 >= 68 && stone && !below) found = true;
-                        below = stone;
-                        d1 += d5;
-                    }
+__global__ void pass_reset_kernel(EngineArgs a) {
+    const int t = threadIdx.x;
+    if (t < kCounters) a.ctr[t] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// pass construction
+// ---------------------------------------------------------------------------
+constexpr int kBuildThreads = 256;
+
+// Exclusive prefix sum over the block; *total receives the block sum.
+__device__ long long block_scan(long long v, long long* tmp, long long* total) {
+    const int t = threadIdx.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBuildThreads; off <<= 1) {
+        const long long add = t >= off ? tmp[t - off] : 0;
+        __syncthreads();
+        tmp[t] += add;
+        __syncthreads();
+    }
+    *total = tmp[kBuildThreads - 1];
+    const long long excl = tmp[t] - v;
+    __syncthreads();
+    return excl;
+}
+
+__device__ __forceinline__ int64_t items_of(int64_t take, int k) { return (take + k - 1) / k; }
+
+// Hands every job with seeds left a share of the pass, round robin from desc->rr_next:
+// share = max(min_quantum, ceil(pass_slots / active jobs)) seeds, cut off at pass_slots.
+// Thread t owns job-table positions [t*C, (t+1)*C) in round-robin order; segments go to the
+// list of their job's Stage-1 kind.
+__global__ __launch_bounds__(kBuildThreads) void pass_build_kernel(EngineArgs a) {
+    __shared__ long long tmp[kBuildThreads];
+    __shared__ int s_last;
+    const int t = threadIdx.x;
+    const int E = a.max_jobs;
+    const int C = (E + kBuildThreads - 1) / kBuildThreads;
+    const int rr = a.desc->rr_next;
+    if (t == 0) s_last = -1;
+    long long active = 0;
+    for (int c = 0; c < C; ++c) {
+        const int r = t * C + c;
+        if (r < E) {
+            const IslandJob& j = a.jobs[(rr + r) % E];
+            if (j.cursor < static_cast<unsigned long long>(j.count)) ++active;
+        }
+    }
+    long long n_active = 0;
+    (void)block_scan(active, tmp, &n_active);
+    long long q = n_active > 0 ? (a.pass_slots + n_active - 1) / n_active : 0;
+    if (q < a.min_quantum) q = a.min_quantum;
+    const long long P = a.pass_slots;
+    auto want_of = [&](const IslandJob& j) -> long long {
+        const long long rem = j.count - static_cast<long long>(j.cursor);
+        return rem > 0 ? (rem < q ? rem : q) : 0;
+    };
+    long long want_local = 0;
+    for (int c = 0; c < C; ++c) {
+        const int r = t * C + c;
+        if (r < E) want_local += want_of(a.jobs[(rr + r) % E]);
+    }
+    long long want_total = 0;
+    const long long want_off = block_scan(want_local, tmp, &want_total);
+    // per kind: segments and Stage-1 items of this thread's entries
+    long long nz_local[2] = {0, 0}, items_local[2] = {0, 0};
+    {
+        long long o = want_off;
+        for (int c = 0; c < C; ++c) {
+            const int r = t * C + c;
+            if (r < E) {
+                const IslandJob& j = a.jobs[(rr + r) % E];
+                const long long want = want_of(j);
+                long long take = P - o;
+                take = take < 0 ? 0 : (take > want ? want : take);
+                if (take > 0) {
+                    const int kind = stage1_kind(j.worlds_per_wave);
+                    nz_local[kind] += 1;
+                    items_local[kind] += items_of(take, j.worlds_per_wave);
                 }
+                o += want;
             }
         }
-        if (__syncthreads_or(found) && tid == 0) {
-            const unsigned long long k = atomicAdd(a.surv_count, 1ULL);
-            if (static_cast<int64_t>(k) < a.surv_cap) {
-                a.surv_slot[k] = slot;
-                a.surv_seed[k] = island_seed(a, slot);
+    }
+    long long tot[4];
+    const long long seg_off0 = block_scan(nz_local[0], tmp, &tot[0]);
+    const long long seg_off1 = block_scan(nz_local[1], tmp, &tot[1]);
+    const long long item_off0 = block_scan(items_local[0], tmp, &tot[2]);
+    const long long item_off1 = block_scan(items_local[1], tmp, &tot[3]);
+    {
+        long long o = want_off;
+        long long sg[2] = {seg_off0, seg_off1}, it[2] = {item_off0, item_off1};
+        int last = -1;
+        for (int c = 0; c < C; ++c) {
+            const int r = t * C + c;
+            if (r < E) {
+                const int ji = (rr + r) % E;
+                IslandJob& j = a.jobs[ji];
+                const long long want = want_of(j);
+                long long take = P - o;
+                take = take < 0 ? 0 : (take > want ? want : take);
+                if (take > 0) {
+                    const int kind = stage1_kind(j.worlds_per_wave);
+                    PassSeg g;
+                    g.job = ji;
+                    g.pad = 0;
+                    g.slot0 = o;
+                    g.item0 = it[kind];
+                    g.start = static_cast<int64_t>(j.cursor);
+                    g.count = take;
+                    a.segs[static_cast<size_t>(kind) * E + sg[kind]] = g;
+                    j.cursor += static_cast<unsigned long long>(take);
+                    ++sg[kind];
+                    it[kind] += items_of(take, j.worlds_per_wave);
+                    last = r;
+                }
+                o += want;
             }
+        }
+        if (last >= 0) atomicMax(&s_last, last);
+    }
+    __syncthreads();
+    if (t == 0) {
+        a.desc->nseg[0] = static_cast<int32_t>(tot[0]);
+        a.desc->nseg[1] = static_cast<int32_t>(tot[1]);
+        a.desc->nitems[0] = tot[2];
+        a.desc->nitems[1] = tot[3];
+        a.desc->nslots = want_total < P ? want_total : P;
+        if (s_last >= 0) a.desc->rr_next = (rr + s_last + 1) % E;
+    }
+}
+
+// Last segment of a list whose field (slot0 or item0) is <= key (the list is sorted by it).
+template <bool BY_ITEM>
+__device__ __forceinline__ int find_seg(const PassSeg* segs, int nseg, int64_t key) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t v = BY_ITEM ? segs[mid].item0 : segs[mid].slot0;
+        if (v <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The segment holding pass slot `slot` (one of the two kind lists).
+__device__ __forceinline__ const PassSeg& seg_of_slot(const EngineArgs& a, int64_t slot) {
+    const int n0 = a.desc->nseg[0], n1 = a.desc->nseg[1];
+    if (n0 > 0) {
+        const PassSeg& g = a.segs[find_seg<false>(a.segs, n0, slot)];
+        if (g.slot0 <= slot && slot < g.slot0 + g.count) return g;
+    }
+    const PassSeg* s1 = a.segs + a.max_jobs;
+    return s1[find_seg<false>(s1, n1, slot)];
+}
+
+__device__ __forceinline__ int64_t job_seed(const IslandJob& j, int64_t i) {
+    return j.seeds != nullptr ? j.seeds[i] : j.first_seed + i;
+}
+
+// ---------------------------------------------------------------------------
+// generator cache: one world per lane
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void pass_cache_kernel(EngineArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    const int lane = threadIdx.x;
+    LanePerm P;
+    P.base = perm_lds + lane * 4;
+    const int64_t n = a.desc->nslots;
+    const int64_t groups = (n + 63) >> 6;
+    for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const int64_t w = (grp << 6) + lane;
+        const bool active = w < n;
+        const int64_t slot = active ? w : n - 1;
+        const PassSeg& g = seg_of_slot(a, slot);
+        const int64_t seed = job_seed(a.jobs[g.job], g.start + (slot - g.slot0));
+        build_world_cache(seed, active, P, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                          a.offsets + static_cast<size_t>(slot) * kGenCount * 3, isl_magic);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1
+// ---------------------------------------------------------------------------
+// Wave-aggregated append of survivors (ballot + popcount prefix).
+__device__ __forceinline__ void append_survivors(const EngineArgs& a, bool pass, int64_t slot, int64_t seed, int job) {
+    const unsigned long long mask = __ballot(pass);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctr[kCtrSurv], static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (pass) {
+        const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+        if (static_cast<int64_t>(k) < a.pass_slots) {
+            a.surv_slot[k] = slot;
+            a.surv_seed[k] = seed;
+            a.surv_job[k] = job;
         }
     }
 }
 
+// Candidate start in one sample column (lattice column (i, l) of chunk (cx, cz)).
+template <class S>
+__device__ __forceinline__ bool stage1_column(const S& s, int cx, int cz, int i, int l, bool active) {
+    const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
+    // the chunk's climate sample of lattice (i, l): temperatures[(3i+1)*16 + 3l+1]
+    const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
+    ColAcc<8, 16> acc;
+    colpar_noise<8, 16, true>(s, X, Z, active, acc);
+    double temp = 0.0, rain = 0.0;
+    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain);
+    const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
+    double lat[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lat[k] = lattice_density(cs, 8 + k, acc.n1[k], acc.n2[k], acc.n3[k]);
+    lat[8] = -10.0;
+    bool found = false;
+    if (active) {
+        bool below = false;
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+            double d1 = lat[k1];
+            const double d5 = (lat[k1 + 1] - d1) * 0.125;
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const bool stone = d1 > 0.0;
+                if (64 + k1 * 8 + l1 >= 68 && stone && !below) found = true;
+                below = stone;
+                d1 += d5;
+            }
+        }
+    }
+    return found;
+}
+
+// KIND 0: one world per wave (column batches of 64 for large windows).  KIND 1: several
+// worlds side by side, or no gate.  Separate kernels keep the register budget of the common
+// one-world form (and so its occupancy) free of the other form.
+template <int KIND>
+__global__ __launch_bounds__(64) void pass_stage1_kernel(EngineArgs a) {
+    __shared__ GenStage st;
+    __shared__ GenStageMulti stm;
+    __shared__ unsigned long long s_item;
+    const int tid = threadIdx.x;
+    const int nseg = a.desc->nseg[KIND];
+    const PassSeg* segs = a.segs + static_cast<size_t>(KIND) * a.max_jobs;
+    const unsigned long long nitems = static_cast<unsigned long long>(a.desc->nitems[KIND]);
+    unsigned long long* queue = &a.ctr[KIND == 0 ? kCtrStage1 : kCtrStage1Multi];
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(queue, 1ULL);
+        __syncthreads();
+        const unsigned long long item = s_item;
+        __syncthreads();
+        if (item >= nitems) break;
+        const PassSeg g = segs[find_seg<true>(segs, nseg, static_cast<int64_t>(item))];
+        const IslandJob& job = a.jobs[g.job];
+        const bsf_island_params p = job.p;
+        const int k = job.worlds_per_wave;
+        const int64_t first_local = (static_cast<int64_t>(item) - g.item0) * k;
+        const int nk = static_cast<int>(g.count - first_local < k ? g.count - first_local : k);
+        const int64_t slot_first = g.slot0 + first_local;
+        const int cz_n = p.chunks_z;
+        const int ncols = 4 * p.chunks_x * cz_n;
+        const uint32_t* perms = a.perms + static_cast<size_t>(slot_first) * kGenCount * 64;
+        const double* offs = a.offsets + static_cast<size_t>(slot_first) * kGenCount * 3;
+        if (KIND == 0) {
+            const OneWorld s{&st, perms, offs};
+            bool found = false;
+            for (int base = 0; base < ncols; base += 64) {
+                const int c = base + tid;
+                const bool active = c < ncols;
+                const int ch = active ? c >> 2 : 0, sc = c & 3;
+                const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
+                found |= stage1_column(s, p.chunk_x0 + ch / cz_n, p.chunk_z0 + ch % cz_n, i, l, active);
+            }
+            const bool any = __ballot(found) != 0;
+            append_survivors(a, tid == 0 && any, slot_first, job_seed(job, g.start + (slot_first - g.slot0)), g.job);
+        } else if (!job.gate) {
+            // no gate for this job: every seed of the item goes to Stage 2
+            const bool mine = tid < nk;
+            const int64_t slot = slot_first + (mine ? tid : 0);
+            append_survivors(a, mine, slot, job_seed(job, g.start + (slot - g.slot0)), g.job);
+        } else {
+            // nk worlds side by side: lanes [w * ncols, (w + 1) * ncols) evaluate world w
+            const int sub_raw = tid / ncols;
+            const bool active = sub_raw < nk;
+            const int sub = active ? sub_raw : 0;
+            const int c = tid - sub_raw * ncols;
+            const MultiWorld s{&stm, perms, offs, nk, sub};
+            const int ch = active ? c >> 2 : 0, sc = c & 3;
+            const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
+            const bool found = stage1_column(s, p.chunk_x0 + ch / cz_n, p.chunk_z0 + ch % cz_n, i, l, active);
+            const unsigned long long m = __ballot(found);
+            bool surv = false;
+            if (tid < nk) surv = ((m >> (tid * ncols)) & ((1ULL << ncols) - 1ULL)) != 0;
+            const int64_t slot = slot_first + (tid < nk ? tid : 0);
+            append_survivors(a, surv, slot, job_seed(job, g.start + (slot - g.slot0)), g.job);
+        }
+    }
+}
+
+template __global__ void pass_stage1_kernel<0>(EngineArgs);
+template __global__ void pass_stage1_kernel<1>(EngineArgs);
+
+// ---------------------------------------------------------------------------
+// Stage 2
+// ---------------------------------------------------------------------------
 // Densities of levels [Y0, Y1) for every density column (chunk-local lattice column) that the
 // noise column (u, v) of the window lattice belongs to.
 template <int Y0, int Y1>
-__device__ __forceinline__ void island_s2_emit(const IslandArgs& a, int64_t it, int nd, int u, int v, const double* n1,
-                                               const double* n2, const double* n3, double n4, double n5,
-                                               const double* s_temp, const double* s_rain) {
-    const int cx_n = a.p.chunks_x, cz_n = a.p.chunks_z;
+__device__ __forceinline__ void stage2_emit(const bsf_island_params& p, double* lat, int u, int v, const double* n1,
+                                            const double* n2, const double* n3, double n4, double n5,
+                                            const double* s_temp, const double* s_rain) {
+    const int cx_n = p.chunks_x, cz_n = p.chunks_z;
     for (int ca = (u >> 2) - 1; ca <= (u >> 2); ++ca) {
         const int i = u - 4 * ca;
         if (ca < 0 || ca >= cx_n || i > 4) continue;
@@ -87,7 +359,7 @@ __device__ __forceinline__ void island_s2_emit(const IslandArgs& a, int64_t it, 
             if (cb < 0 || cb >= cz_n || l > 4) continue;
             const int d = (ca * cz_n + cb) * 25 + i * 5 + l;
             const ColumnShape cs = column_shape(s_temp[d], s_rain[d], n4, n5);
-            double* o = a.lattice + (static_cast<size_t>(it) * nd + d) * 8 + (Y0 - 8);
+            double* o = lat + static_cast<size_t>(d) * 8 + (Y0 - 8);
 #pragma unroll
             for (int k = 0; k < Y1 - Y0; ++k) o[k] = lattice_density(cs, Y0 + k, n1[k], n2[k], n3[k]);
         }
@@ -95,52 +367,68 @@ __device__ __forceinline__ void island_s2_emit(const IslandArgs& a, int64_t it, 
 }
 
 constexpr int kS2Threads = 64;
-constexpr int kS2MaxDens = 8 * 8 * 25;
 
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
-__global__ __launch_bounds__(kS2Threads) void island_s2_kernel(IslandArgs a) {
+__global__ __launch_bounds__(kS2Threads) void pass_stage2_kernel(EngineArgs a) {
     __shared__ GenStage st;
+    __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
-    const int cx_n = a.p.chunks_x, cz_n = a.p.chunks_z;
-    const int nd = 25 * cx_n * cz_n;
-    double* s_temp = s_clim;
-    double* s_rain = s_clim + nd;
-    const int nzl = 4 * cz_n + 1;
-    const int nc = (4 * cx_n + 1) * nzl;
-    for (int64_t it = blockIdx.x; it < a.list_n; it += gridDim.x) {
-        const int64_t slot = a.list_slot[it];
-        const uint32_t* perms = a.perms + static_cast<size_t>(slot) * kGenCount * 64;
-        const double* offs = a.offsets + static_cast<size_t>(slot) * kGenCount * 3;
+    if (a.stage1_only) return;
+    unsigned long long nsurv = a.ctr[kCtrSurv];
+    if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.ctr[kCtrStage2], 1ULL);
+        __syncthreads();
+        const unsigned long long it = s_item;
+        __syncthreads();
+        if (it >= nsurv) break;
+        const int64_t slot = a.surv_slot[it];
+        const int32_t jix = a.surv_job[it];
+        const bsf_island_params p = a.jobs[jix].p;
+        const int cx_n = p.chunks_x, cz_n = p.chunks_z;
+        const int nd = 25 * cx_n * cz_n;
+        double* s_temp = s_clim;
+        double* s_rain = s_clim + nd;
+        const int nzl = 4 * cz_n + 1;
+        const int nc = (4 * cx_n + 1) * nzl;
+        const OneWorld s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                         a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
+        double* lat = a.lattice + static_cast<size_t>(it) * a.lat_stride;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)
         for (int base = 0; base < nd; base += kS2Threads) {
             const int d = base + tid;
             const bool active = d < nd;
             const int ch = active ? d / 25 : 0, il = active ? d % 25 : 0;
-            const int cx = a.p.chunk_x0 + ch / cz_n, cz = a.p.chunk_z0 + ch % cz_n;
+            const int cx = p.chunk_x0 + ch / cz_n, cz = p.chunk_z0 + ch % cz_n;
             const double BX = static_cast<double>(cx * 16 + 3 * (il / 5) + 1);
             const double BZ = static_cast<double>(cz * 16 + 3 * (il % 5) + 1);
             double t = 0.0, r = 0.0;
-            colpar_climate_all<1>(st, perms, offs, &BX, &BZ, active ? 1 : 0, &t, &r);
+            colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &t, &r);
             if (active) {
                 s_temp[d] = t;
                 s_rain[d] = r;
             }
         }
-        // noise lattice, one column per lane; stage_generator's barriers publish s_temp/s_rain
+        // noise lattice, one column per lane; the stager's barriers publish s_temp / s_rain
         for (int base = 0; base < nc; base += kS2Threads) {
             const int c = base + tid;
             const bool active = c < nc;
             const int u = active ? c / nzl : 0, v = active ? c % nzl : 0;
-            const double X = static_cast<double>(a.p.chunk_x0 * 4 + u);
-            const double Z = static_cast<double>(a.p.chunk_z0 * 4 + v);
+            const double X = static_cast<double>(p.chunk_x0 * 4 + u);
+            const double Z = static_cast<double>(p.chunk_z0 * 4 + v);
             ColAcc<8, 16> acc;
-            colpar_noise<8, 16, true>(st, perms, offs, X, Z, active, acc);
+            colpar_noise<8, 16, true>(s, X, Z, active, acc);
             __syncthreads();
-            if (active) island_s2_emit<8, 16>(a, it, nd, u, v, acc.n1, acc.n2, acc.n3, acc.n4, acc.n5, s_temp, s_rain);
+            if (active) stage2_emit<8, 16>(p, lat, u, v, acc.n1, acc.n2, acc.n3, acc.n4, acc.n5, s_temp, s_rain);
         }
         __syncthreads();
+        if (tid == 0) {
+            const int cls = a.jobs[jix].flood_class;
+            const unsigned long long pos = atomicAdd(&a.ctr[kCtrFloodCount + cls], 1ULL);
+            a.flood_list[static_cast<size_t>(cls) * a.pass_slots + pos] = static_cast<int32_t>(it);
+        }
     }
 }
 

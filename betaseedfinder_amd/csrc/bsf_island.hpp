@@ -1,4 +1,14 @@
-// bsf_island.hpp -- arguments of the column-parallel island Stage 1 / Stage 2 kernels.
+// bsf_island.hpp -- the island search engine's device structures and kernels.
+//
+// A *pass* is a fixed sequence of persistent kernels over at most `pass_slots` seeds taken
+// from the jobs resident in a device job table:
+//
+//   pass_reset -> pass_build -> pass_cache -> pass_stage1 -> pass_stage2 -> pass_flood x 4
+//
+// pass_build hands every active job a share of the pass (its seed cursor lives in the job
+// table and is advanced on the device), so the same launch sequence -- captured once as a
+// hipGraph -- serves one job or hundreds of heterogeneous ones.  Every later kernel takes
+// its work items from a device queue counter.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,23 +19,117 @@
 
 namespace bsf {
 
-struct IslandArgs {
+// One search job in the device job table.
+struct IslandJob {
+    int64_t first_seed;        // seeds first_seed + i (when seeds == nullptr)
+    const int64_t* seeds;      // explicit device seed list, or nullptr
+    int64_t count;             // seeds in the job
+    unsigned long long cursor; // seeds handed to passes so far (pass_build)
+    unsigned long long hits;   // hits found so far (flood kernels)
     bsf_island_params p;
-    const uint32_t* perms;              // generator cache [slot][kGenCount][64 dwords]
-    const double* offsets;              // [slot][kGenCount][3]
-    const int64_t* seeds;               // seeds of the cache slots, or nullptr for first_seed + slot
-    int64_t first_seed;
-    int64_t count;                      // cache slots (Stage 1 items)
-    int64_t* surv_slot;                 // Stage 1 survivors: cache slot and seed
-    int64_t* surv_seed;
-    unsigned long long* surv_count;
-    int64_t surv_cap;
-    const int64_t* list_slot;           // Stage 2 items: cache slots
-    int64_t list_n;
-    double* lattice;                    // Stage 2 output [item][25 * chunks][8] levels 8..15
+    int32_t tag;               // written into bsf_island_hit::window_index
+    int32_t worlds_per_wave;   // Stage-1 worlds per 64-lane wave (1 .. kMaxWorldsPerWave)
+    int32_t gate;              // 1: Stage-1 candidate gate; 0: every seed goes to Stage 2
+    int32_t flood_class;       // kFloodClass* below
+    int32_t pad[2];
 };
 
-__global__ void island_s1_kernel(IslandArgs a);
-__global__ void island_s2_kernel(IslandArgs a);
+// A contiguous run of one job's seeds inside a pass.
+struct PassSeg {
+    int32_t job;               // job table index
+    int32_t pad;
+    int64_t slot0;             // first pass slot
+    int64_t item0;             // first Stage-1 work item
+    int64_t start;             // offset of slot0's seed inside the job
+    int64_t count;
+};
+
+// Segments come in two lists by Stage-1 form: kind 0 = one world per wave, kind 1 = several
+// worlds per wave (or no gate).  Each list is in slot order with its own item numbering.
+struct PassDesc {
+    int32_t nseg[2];
+    int32_t rr_next;           // job table index where the next pass starts handing out seeds
+    int32_t pad;
+    int64_t nslots;
+    int64_t nitems[2];
+};
+
+__host__ __device__ inline int stage1_kind(int worlds_per_wave) { return worlds_per_wave == 1 ? 0 : 1; }
+
+// Device queue counters of a pass.
+enum {
+    kCtrCache = 0, kCtrStage1 = 1, kCtrSurv = 2, kCtrStage2 = 3,
+    kCtrFloodNext = 4,   // 4 entries: flood work-queue heads per class
+    kCtrFloodCount = 8,  // 4 entries: survivors per flood class
+    kCtrHits = 12,
+    kCtrStage1Multi = 13,
+    kCounters = 16
+};
+
+// Flood classes by window size (block bitsets of one window: rows x 64-bit z words).
+enum {
+    kFloodClassSmall = 0,  // z <= 64 blocks, x <= 64 blocks: LDS, 2 workgroups per CU
+    kFloodClassWideX = 1,  // z <= 64, x <= 128: LDS
+    kFloodClassWideZ = 2,  // z <= 128, x <= 64: LDS, two words per row
+    kFloodClassLarge = 3,  // z <= 128, x <= 128: bitsets in a per-workgroup global scratch
+    kFloodClasses = 4
+};
+
+constexpr int kFloorY = 68;
+constexpr int kLevels = 128 - kFloorY;   // 60 block rows per x
+constexpr int kRowStride = kLevels + 1;  // 61: odd stride spreads the x rows over the LDS banks
+
+// Words of one window's bitsets (rem + comp + st67) for nx x-rows of w words.
+__host__ __device__ constexpr size_t flood_words(int nx, int w) {
+    return static_cast<size_t>(2 * nx * kRowStride + nx) * w;
+}
+
+struct EngineArgs {
+    IslandJob* jobs;
+    int32_t max_jobs;
+    int32_t min_quantum;           // pass_build: smallest share handed to one job
+    int64_t pass_slots;            // seeds per pass
+    PassDesc* desc;
+    PassSeg* segs;                 // [2][max_jobs] by Stage-1 kind
+    uint32_t* perms;               // generator cache [slot][kGenCount][64]
+    double* offsets;               // [slot][kGenCount][3]
+    unsigned long long* ctr;       // [kCounters]
+    int64_t* surv_slot;            // [pass_slots] survivors: pass slot, seed, job index
+    int64_t* surv_seed;
+    int32_t* surv_job;
+    double* lattice;               // [survivor][lat_stride] window lattice, levels 8..15
+    int64_t lat_stride;            // doubles per survivor
+    int32_t* flood_list;           // [kFloodClasses][pass_slots] survivor indices
+    bsf_island_hit* hits;          // [pass_slots]
+    unsigned long long* flood_scratch;  // kFloodClassLarge bitsets, flood_scratch_words per workgroup
+    int64_t flood_scratch_words;
+    int32_t stage1_only;           // 1: stop after Stage 1 (survivor audits)
+    int32_t pad;
+};
+
+__global__ void pass_reset_kernel(EngineArgs a);
+__global__ void pass_build_kernel(EngineArgs a);
+__global__ void pass_cache_kernel(EngineArgs a);
+template <int KIND>
+__global__ void pass_stage1_kernel(EngineArgs a);
+__global__ void pass_stage2_kernel(EngineArgs a);
+template <int W, bool GLOBAL>
+__global__ void pass_flood_kernel(EngineArgs a, int cls);
+
+hipError_t upload_island_tables();
+
+// Worlds per Stage-1 wave that fit one world's sample columns (4 per chunk) into 64 lanes.
+inline int default_worlds_per_wave(const bsf_island_params& p) {
+    const int ncols = 4 * p.chunks_x * p.chunks_z;
+    if (ncols > 32) return 1;
+    const int k = 64 / ncols;
+    return k > kMaxWorldsPerWave ? kMaxWorldsPerWave : k;
+}
+
+inline int flood_class_of(const bsf_island_params& p) {
+    const int nx = 16 * p.chunks_x, nz = 16 * p.chunks_z;
+    if (nz <= 64) return nx <= 64 ? kFloodClassSmall : kFloodClassWideX;
+    return nx <= 64 ? kFloodClassWideZ : kFloodClassLarge;
+}
 
 }  // namespace bsf

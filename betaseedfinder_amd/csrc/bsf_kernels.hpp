@@ -20,19 +20,15 @@ struct ColumnSet {
     const int4* dens;
 };
 
-enum { MODE_DENSITY = 0, MODE_CANDIDATE = 1, MODE_LATTICE = 2 };
+enum { MODE_DENSITY = 0 };
 
 struct LatticeArgs {
     const int64_t* seeds;                 // explicit seeds, or nullptr for first_seed + i
     int64_t first_seed;
-    int64_t count;                        // number of seeds (upper bound when d_count is set)
-    const unsigned long long* d_count;    // optional device-resident count (survivor lists)
+    int64_t count;
     ColumnSet cols;
     double* scratch;                      // gridDim.x slots of ScratchMap::total * 64 doubles
-    double* out;                          // MODE_DENSITY / MODE_LATTICE output
-    int64_t* out_seeds;                   // MODE_CANDIDATE survivors
-    unsigned long long* out_count;
-    int64_t out_cap;
+    double* out;                          // densities [seed][nd][levels]
 };
 
 struct ScoutArgs {
@@ -44,22 +40,10 @@ struct ScoutArgs {
     int64_t out_cap;
 };
 
-struct FloodArgs {
-    bsf_island_params p;
-    const int64_t* seeds;                 // survivor seeds
-    const unsigned long long* d_count;    // survivor count on the device, or nullptr to use cap
-    int64_t cap;                          // survivor capacity (the count when d_count is null)
-    const double* lattice;                // [survivor][chunk*25 + i*5 + l][8] levels 8..15
-    bsf_island_hit* hits;
-    unsigned long long* hit_count;
-    int64_t hit_cap;
-    int32_t window_index;
-};
 
 template <int Y0, int Y1, int MODE>
 __global__ void lattice_kernel(LatticeArgs a);
 __global__ void scout_kernel(ScoutArgs a);
-__global__ void island_flood_kernel(FloodArgs a);
 
 // Uploads the nextInt division table into the kernels' constant memory (current device).
 hipError_t upload_tables();

@@ -10,6 +10,7 @@
 //  * coarse_score_kernel: one workgroup per item, bit-parallel component search over the
 //    61x61x17 coarse sign grid (one uint64 z-mask per (x, y) row).
 #include "bsf_stage0.hpp"
+#include "bsf_cache.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -33,25 +34,8 @@ hipError_t upload_stage0_tables() {
 }
 
 // ---------------------------------------------------------------------------
-// generator cache
+// generator cache (bsf_cache.hpp)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void store_generator(const LanePerm& p, const GenOffsets& g, uint32_t* perm_dst,
-                                                double* off_dst) {
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(p.base);
-#pragma unroll 8
-    for (int k = 0; k < 64; k += 4) {
-        uint4 v;
-        v.x = w[(k + 0) * 64];
-        v.y = w[(k + 1) * 64];
-        v.z = w[(k + 2) * 64];
-        v.w = w[(k + 3) * 64];
-        *reinterpret_cast<uint4*>(perm_dst + k) = v;
-    }
-    off_dst[0] = g.a;
-    off_dst[1] = g.b;
-    off_dst[2] = g.c;
-}
-
 __global__ __launch_bounds__(64) void gen_cache_kernel(GenCacheArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
     const int lane = threadIdx.x;
@@ -63,34 +47,8 @@ __global__ __launch_bounds__(64) void gen_cache_kernel(GenCacheArgs a) {
         const bool active = w < a.count;
         const int64_t idx = active ? w : a.count - 1;
         const int64_t seed = a.seeds != nullptr ? a.seeds[idx] : a.first_seed + idx;
-        uint32_t* perms = a.perms + static_cast<size_t>(active ? w : 0) * kGenCount * 64;
-        double* offs = a.offsets + static_cast<size_t>(active ? w : 0) * kGenCount * 3;
-        Lcg r;
-        r.set_seed(seed);
-        // BetaTerrain173 constructor order (:84-92): noise2 16, noise3 16, noise1 8, two consumed
-        // 4-octave generators, noise4 10, noise5 16.
-        for (int g = 0; g < kGenN4; ++g) {
-            const GenOffsets o = build_generator(r, P, s0_magic);
-            if (active) store_generator(P, o, perms + g * 64, offs + g * 3);
-        }
-        for (int i = 0; i < 8; ++i) consume_generator(r, s0_magic);
-        for (int g = kGenN4; g < kGenTemp; ++g) {
-            const GenOffsets o = build_generator(r, P, s0_magic);
-            if (active) store_generator(P, o, perms + g * 64, offs + g * 3);
-        }
-        // climate: Random(seed * 9871 / 39811 / 543321), WorldChunkManager173.java:25-30
-        const uint64_t us = static_cast<uint64_t>(seed);
-        const uint64_t mul[3] = {9871ULL, 39811ULL, 543321ULL};
-        const int base[3] = {kGenTemp, kGenRain, kGenBlend};
-        const int n[3] = {4, 4, 2};
-        for (int t = 0; t < 3; ++t) {
-            Lcg rc;
-            rc.set_seed(static_cast<int64_t>(us * mul[t]));
-            for (int o = 0; o < n[t]; ++o) {
-                const GenOffsets go = build_generator(rc, P, s0_magic);
-                if (active) store_generator(P, go, perms + (base[t] + o) * 64, offs + (base[t] + o) * 3);
-            }
-        }
+        build_world_cache(seed, active, P, a.perms + static_cast<size_t>(active ? w : 0) * kGenCount * 64,
+                          a.offsets + static_cast<size_t>(active ? w : 0) * kGenCount * 3, s0_magic);
     }
 }
 
@@ -108,10 +66,11 @@ struct S0Column {
 template <int Y0, int Y1, bool WITH_SHAPE>
 __device__ __forceinline__ void evaluate_cached_column(GenStage& st, const uint32_t* perms, const double* offs,
                                                        double X, double Z, bool active, S0Column<Y0, Y1>& c) {
-    colpar_noise<Y0, Y1, WITH_SHAPE>(st, perms, offs, X, Z, active, c.acc);
+    const OneWorld s{&st, perms, offs};
+    colpar_noise<Y0, Y1, WITH_SHAPE>(s, X, Z, active, c.acc);
     if (WITH_SHAPE) {
         const double BX = X * 4.0 + 2.0, BZ = Z * 4.0 + 2.0;
-        colpar_climate_all<1>(st, perms, offs, &BX, &BZ, active ? 1 : 0, &c.temp, &c.rain);
+        colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &c.temp, &c.rain);
     }
 }
 

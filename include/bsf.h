@@ -36,9 +36,10 @@ const char* bsf_last_error(void);
 int         bsf_device_banner(char* buf, size_t cap);
 
 /* ---- launch timing (HIP events on the launching stream; for benchmarks) ---------------
- * Kernel names: "density", "scout", "island_stage1", "island_stage2", "island_flood",
- * "stage0_p20", "stage0_upper", "stage0_lower", "stage0_coarse".  units = seeds (or
- * survivor seeds / worlds) processed by the timed launches. */
+ * Kernel names: "density", "scout", "island_cache", "island_stage1" (one world per wave),
+ * "island_stage1_multi" (several worlds per wave or no gate), "island_stage2", "island_flood",
+ * "stage0_p20", "stage0_upper", "stage0_lower", "stage0_coarse".  units = seeds (or worlds)
+ * processed by the timed launches (0 where the count is only known on the device). */
 int bsf_timing_enable(int on);
 int bsf_timing_reset(void);
 int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uint64_t* units);
@@ -96,6 +97,97 @@ int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_param
  * sample column), sorted.  Exposed for gate audits. */
 int bsf_island_stage1_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
                             int64_t* out_seeds, int64_t cap, int64_t* n_found);
+
+/* ---- batched island jobs (no counterpart in the reference: the reference worker runs one
+ * search per process).  Many jobs with their own seed ranges, windows and gate thresholds share
+ * every pass of the engine: a device job table, a pass built on the device from it, and one
+ * captured hipGraph per pass (cache, Stage 1, Stage 2, floods). ---------------------------- */
+typedef struct bsf_island_job {
+    int64_t first_seed;               /* seeds first_seed .. first_seed + count - 1 */
+    int64_t count;
+    bsf_island_params p;
+} bsf_island_job;
+
+/* Runs all jobs to completion.  Hits carry window_index = job index and are sorted by
+ * (job, seed); job_hits (optional, njobs entries) receives each job's hit count. */
+int bsf_island_batch(const bsf_island_job* jobs, int32_t njobs, bsf_island_hit* out, int64_t cap,
+                     int64_t* n_found, int64_t* job_hits);
+
+/* Engine shape: seeds per pass (64 .. 2^22), job-table entries, smallest per-job share of a
+ * pass.  Defaults 2^17, 1024, 1024. */
+int bsf_island_engine_configure(int64_t pass_slots, int32_t max_jobs, int32_t min_quantum);
+
+/* Totals of the library's engine since bsf_init (or the last configure): passes, seeds,
+ * Stage-1 survivors, hits. */
+int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits);
+
+/* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave and
+ * whether the Stage-1 gate runs (gate = 0 sends every seed to Stage 2).  bsf_island_autotune
+ * times every candidate on trial_seeds seeds from first_seed, checks that all candidates give
+ * the same hits, and keeps the fastest for later jobs of that shape. */
+typedef struct bsf_island_tuning {
+    int32_t worlds_per_wave;
+    int32_t gate;
+    double seeds_per_sec;             /* measured rate of the chosen setting (autotune only) */
+    int32_t candidates;               /* settings timed (autotune only) */
+    int32_t reserved;
+} bsf_island_tuning;
+int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t trial_seeds,
+                        bsf_island_tuning* out);
+int bsf_island_get_tuning(const bsf_island_params* p, bsf_island_tuning* out);
+int bsf_island_set_tuning(const bsf_island_params* p, const bsf_island_tuning* t);
+
+/* ---- serving (no counterpart in the reference).  A request loop on its own thread, stream
+ * and engine: jobs are admitted as job-table entries free up (state memory) into passes whose
+ * hit buffers bound every pass's output (hit-buffer memory); each pass mixes all resident jobs,
+ * hits stream back per pass as events.  fifo = 1 admits one job at a time: the plain
+ * one-job-per-launch worker on the same engine. ------------------------------------------ */
+typedef struct bsf_server bsf_server;
+
+typedef struct bsf_server_config {
+    int64_t state_bytes;              /* device job-table budget: entries = state_bytes / entry size */
+    int64_t hit_buffer_bytes;         /* device hit buffers: seeds per pass = bytes / (2 * 64) */
+    int64_t queue_limit;              /* waiting jobs beyond this are rejected */
+    int32_t min_quantum;              /* smallest per-job share of a pass (0: 1024) */
+    int32_t fifo;
+} bsf_server_config;
+
+#define BSF_EVENT_ADMITTED 1
+#define BSF_EVENT_HIT 2
+#define BSF_EVENT_DONE 3
+#define BSF_EVENT_REJECTED 4
+
+typedef struct bsf_server_event {
+    int64_t job_id;
+    int32_t kind;                     /* BSF_EVENT_* */
+    int32_t reserved;
+    double t_ms;                      /* server clock: ms since bsf_server_start */
+    bsf_island_hit hit;               /* BSF_EVENT_HIT only */
+} bsf_server_event;
+
+typedef struct bsf_server_job_info {
+    int32_t state;                    /* 0 queued, 1 resident, 2 done, 3 rejected */
+    int32_t reserved;
+    double submit_ms, admit_ms, first_hit_ms, done_ms;  /* -1 until reached */
+    int64_t seeds_done, hits;
+} bsf_server_job_info;
+
+typedef struct bsf_server_stats {
+    int64_t submitted, admitted, rejected, completed, queued, resident;
+    int64_t passes, seeds, hits;
+    int64_t pass_slots, max_jobs;
+} bsf_server_stats;
+
+int bsf_server_start(const bsf_server_config* cfg, bsf_server** out);
+/* Non-blocking; *job_id identifies the job in events.  A job beyond queue_limit is rejected
+ * (a BSF_EVENT_REJECTED event; the call itself still returns BSF_OK). */
+int bsf_server_submit(bsf_server* s, const bsf_island_job* job, int64_t* job_id);
+/* Up to cap events in order; waits up to timeout_ms for the first one. */
+int bsf_server_poll(bsf_server* s, bsf_server_event* events, int64_t cap, int64_t* n, int32_t timeout_ms);
+int bsf_server_get_job(bsf_server* s, int64_t job_id, bsf_server_job_info* out);
+int bsf_server_get_stats(bsf_server* s, bsf_server_stats* out);
+/* drain = 1 finishes every submitted job first; 0 stops after the passes in flight. */
+int bsf_server_stop(bsf_server* s, int32_t drain);
 
 /* ---- worker stream protocol bodies --------------------------------------------------- */
 /* `stream` mode batch: positive P20 column count per seed (0..64) for the 8x8 P20 sample of
