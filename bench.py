@@ -83,6 +83,141 @@ def stage1_fp64_ops_per_seed(n_cols=64, levels=8):
     return int(round(o3d + o2d + clim + combine))
 
 
+# The four filter profiles of the batched and served workloads: gate thresholds drawn
+# uniformly in each profile's bands (minimum island volume, average thickness).
+PROFILES = [
+    dict(blocks=(200, 650), columns=60, width=8, thick=(1.5, 2.0)),
+    dict(blocks=(650, 1100), columns=100, width=10, thick=(2.0, 2.5)),
+    dict(blocks=(1100, 1550), columns=140, width=12, thick=(2.5, 3.0)),
+    dict(blocks=(1550, 2001), columns=180, width=14, thick=(3.0, 3.5)),
+]
+
+
+def job_params(bsf, w, h, profile, u_blocks, u_thick):
+    pr = PROFILES[profile]
+    return bsf.island_params(-(w // 2), -(h // 2), w, h,
+                             min_blocks=int(pr["blocks"][0] + (pr["blocks"][1] - pr["blocks"][0]) * u_blocks),
+                             min_columns=pr["columns"], min_y_for_match=60, min_width=pr["width"],
+                             min_depth=pr["width"],
+                             min_avg_thickness=pr["thick"][0] + (pr["thick"][1] - pr["thick"][0]) * u_thick)
+
+
+def batched_jobs(bsf, n=64, lo=10, span=10):
+    """Heterogeneous jobs k = 0..n-1 from splitmix64(0xB5F00006 + k): size 2^(lo + span*u),
+    window w x h with w, h uniform in 1..8, one of the four profiles."""
+    jobs = []
+    for k in range(n):
+        r = [unit53(splitmix64(0xB5F00006 + k) + 977 * i) for i in range(7)]
+        size = int(round(2 ** (lo + span * r[0])))
+        w, h = 1 + int(8 * r[1]), 1 + int(8 * r[2])
+        first = splitmix64(0xB5F00006 + k) & MASK48
+        jobs.append((first, size, job_params(bsf, w, h, int(4 * r[3]), r[4], r[5])))
+    return jobs
+
+
+def pct(v, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(v, dtype=np.float64), q)), 2) if len(v) else None
+
+
+def run_batched(bsf, args):
+    jobs = batched_jobs(bsf, lo=args.batch_job_log2_lo, span=args.batch_job_log2_span)
+    shapes = sorted({(p.chunks_x, p.chunks_z) for _, _, p in jobs})
+    t = time.perf_counter()
+    tuned = {}
+    if not args.no_tune:
+        for (w, h) in shapes:
+            tt = bsf.autotune(bsf.island_params(-(w // 2), -(h // 2), w, h), 0x5EED, args.tune_seeds)
+            tuned["%dx%d" % (w, h)] = [tt.worlds_per_wave, tt.gate]
+    tune_s = time.perf_counter() - t
+    bsf.island_range(jobs[0][0], 4096, jobs[0][2])  # warm the pass graphs
+    t = time.perf_counter()
+    plain = [bsf.island_range(first, count, p, cap=1 << 20) for first, count, p in jobs]
+    plain_s = time.perf_counter() - t
+    t = time.perf_counter()
+    hits, per_job = bsf.island_batch(jobs)
+    batch_s = time.perf_counter() - t
+    same = all([h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in plain[i]] for i in range(len(jobs)))
+    seeds = sum(c for _, c, _ in jobs)
+    return {
+        "workload": "64 jobs k from splitmix64(0xB5F00006+k): size 2^(%d+%d u) seeds, window w x h uniform in "
+                    "1..8 chunks, 4 threshold profiles" % (args.batch_job_log2_lo, args.batch_job_log2_span),
+        "seeds": seeds,
+        "hits": int(sum(per_job)),
+        "batched_seeds_per_s": round(seeds / batch_s, 1),
+        "plain_seeds_per_s": round(seeds / plain_s, 1),
+        "speedup": round(plain_s / batch_s, 3),
+        "per_job_hits_equal_plain": bool(same),
+        "autotune_s": round(tune_s, 2),
+        "tuned": tuned,
+    }
+
+
+def served_trace(bsf, seconds, rate, median_log2, seed=0xB5F00008):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        z = rng.standard_normal()
+        size = int(min(2 ** 36, max(1, round(2 ** median_log2 * np.exp(2.0 * z)))))
+        w, h = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+        prof = int(rng.integers(0, 4))
+        first = int(rng.integers(0, 1 << 48))
+        out.append((t, first, size, job_params(bsf, w, h, prof, float(rng.random()), float(rng.random()))))
+
+
+def run_trace(bsf, trace, fifo, horizon_s):
+    srv = bsf.Server(fifo=fifo)
+    ids, hits_seen = [], 0
+    t0 = time.perf_counter()
+    i = 0
+    try:
+        while True:
+            now = time.perf_counter() - t0
+            while i < len(trace) and trace[i][0] <= now:
+                _, first, size, p = trace[i]
+                ids.append(srv.submit(first, size, p))
+                i += 1
+            hits_seen += sum(1 for e in srv.poll(timeout_ms=2) if e.kind == bsf.api.EVENT_HIT)
+            if i == len(trace):
+                st = srv.stats()
+                if st.completed + st.rejected >= len(ids) or now > horizon_s:
+                    break
+        wall = time.perf_counter() - t0
+        infos = [srv.job(j) for j in ids]
+        st = srv.stats()
+    finally:
+        srv.stop(drain=False)
+    done = [x for x in infos if x.state == 2]
+    ttfh = [x.first_hit_ms - x.submit_ms for x in done if x.first_hit_ms >= 0]
+    comp = [x.done_ms - x.submit_ms for x in done]
+    seeds_done = sum(x.seeds_done for x in infos)
+    return {
+        "jobs": len(ids), "completed": len(done), "rejected": int(st.rejected),
+        "unfinished_at_horizon": len(ids) - len(done) - int(st.rejected),
+        "ttfh_ms_p50": pct(ttfh, 50), "ttfh_ms_p99": pct(ttfh, 99),
+        "completion_ms_p50": pct(comp, 50), "completion_ms_p99": pct(comp, 99),
+        "seeds_per_s": round(seeds_done / wall, 1), "wall_s": round(wall, 2), "passes": int(st.passes),
+        "hits_streamed": hits_seen,
+    }
+
+
+def run_serving(bsf, args):
+    trace = served_trace(bsf, args.serve_seconds, args.serve_rate, args.serve_median_log2)
+    served = run_trace(bsf, trace, False, args.serve_seconds + args.serve_drain)
+    plain = run_trace(bsf, trace, True, args.serve_seconds + args.serve_drain)
+    return {
+        "workload": "Poisson arrivals %.0f jobs/s for %.0f s; size min(2^36, round(2^%d e^(2Z))); windows w x h "
+                    "uniform in 1..8 chunks; 4 threshold profiles" % (args.serve_rate, args.serve_seconds,
+                                                                      args.serve_median_log2),
+        "served": served,
+        "plain_fifo": plain,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +226,16 @@ def main():
     ap.add_argument("--batch-log2", type=int, default=23)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-seeds", type=int, default=1 << 15)
+    ap.add_argument("--batch-job-log2-lo", type=int, default=10)
+    ap.add_argument("--batch-job-log2-span", type=int, default=10)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-drain", type=float, default=45.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-median-log2", type=int, default=13)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (traffic-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -184,7 +329,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "island_s1_kernel (island Stage 1: 64 sample columns per seed)",
+            "kernel": "pass_stage1_kernel<0> (island Stage 1, one world per wave: 64 sample columns per seed)",
             "achieved": round(achieved_tflops, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -222,6 +367,10 @@ def main():
             "sample": "first %d seeds of the same range through the CPU oracle (%d threads, %.1f s); "
                       "hit lists equal: %s" % (n, threads, dt, gpu_sample == cpu_hits),
         }
+    if not args.no_batched:
+        result["batched"] = run_batched(bsf, args)
+    if not args.no_serve:
+        result["serving"] = run_serving(bsf, args)
     print(json.dumps(result))
 
 
