@@ -29,6 +29,11 @@ MASK48 = (1 << 48) - 1
 METRIC = "seeds/sec full island pipeline (bit-exact) + p50/p99 job latency, served vs plain"
 FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64, FMA counted as two flops
 FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no FMA: -ffp-contract=off)
+# HBM bytes per seed of pass_stage1_kernel<0>: FETCH_SIZE (x 1.99975, the factor a 256 MiB copy
+# reads back) 21,608 B + WRITE_SIZE 45 B, from the rocprofv3 --pmc runs summarised in
+# profiles/island_pipeline_4x4_v1134.md.  The algorithmic figure is the world's 76 cached
+# generators, 76 x (256 + 24) = 21,280 B.
+STAGE1_HBM_BYTES_PER_SEED = 21608 + 45
 
 
 def splitmix64(x):
@@ -338,7 +343,8 @@ def main():
             "fp64_ops_per_seed": ops_seed,
             "seeds_per_launch": int(seeds_per_launch),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-            "traffic": None,
+            "traffic": int(round(STAGE1_HBM_BYTES_PER_SEED * seeds_per_launch)),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
         },
     }
 
