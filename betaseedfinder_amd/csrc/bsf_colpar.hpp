@@ -2,7 +2,7 @@
 //
 // A wave works on one world (or on a few worlds side by side): each lane owns one lattice
 // column (noise x/z) or one climate sample, the generators are staged one at a time into
-// LDS (256-byte permutation + 3 offsets) and shared by the lanes of their world, and all
+// LDS (512-entry permutation + 3 offsets) and shared by the lanes of their world, and all
 // octave accumulators live in registers.  When the whole workgroup evaluates one world, the
 // Y-cell run structure of an octave is the same on every lane and the legacy corner reuse
 // is a uniform branch.
@@ -20,9 +20,12 @@ namespace bsf {
 constexpr int kGenN2 = 0, kGenN3 = 16, kGenN1 = 32, kGenN4 = 40, kGenN5 = 50;
 constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
 
+// A staged generator: the 512-entry permutation (the cached 256 bytes twice, see SharedPerm)
+// and the three offsets.
 struct GenStage {
-    uint32_t perm[64];
+    uint32_t perm[128];
     double off[3];
+    YTable yt;  // Y axis of the staged 3D octave (stage_generator_y)
 };
 
 // Stage generator g of the current world into LDS (every thread of the block must call this).
@@ -30,7 +33,27 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
                                                 const double* __restrict__ offs, int g) {
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < 64) st.perm[t] = perms[g * 64 + t];
+    if (t < 64) {
+        const uint32_t v = perms[g * 64 + t];
+        st.perm[t] = v;
+        st.perm[t + 64] = v;
+    }
+    if (t < 3) st.off[t] = offs[g * 3 + t];
+    __syncthreads();
+}
+
+// Stage generator g of a 3D octave with y scale sy, and its Y-axis table over [Y0, Y1).
+template <int Y0, int Y1>
+__device__ __forceinline__ void stage_generator_y(GenStage& st, const uint32_t* __restrict__ perms,
+                                                  const double* __restrict__ offs, int g, double sy) {
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const uint32_t v = perms[g * 64 + t];
+        st.perm[t] = v;
+        st.perm[t + 64] = v;
+        build_ytable<Y0, Y1>(st.yt, sy, offs[g * 3 + 1]);
+    }
     if (t < 3) st.off[t] = offs[g * 3 + t];
     __syncthreads();
 }
@@ -42,6 +65,11 @@ struct OneWorld {
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
     __device__ __forceinline__ void stage(int g) const { stage_generator(*st, perms, offs, g); }
+    template <int Y0, int Y1>
+    __device__ __forceinline__ void stage_y(int g, double sy) const {
+        stage_generator_y<Y0, Y1>(*st, perms, offs, g, sy);
+    }
+    __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
     __device__ __forceinline__ double off(int i) const { return st->off[i]; }
 };
@@ -50,7 +78,7 @@ struct OneWorld {
 // group `sub` evaluates world first_slot + sub.  Used when a world needs few columns.
 constexpr int kMaxWorldsPerWave = 16;
 struct GenStageMulti {
-    uint32_t perm[kMaxWorldsPerWave][64];
+    uint32_t perm[kMaxWorldsPerWave][128];
     double off[kMaxWorldsPerWave][3];
 };
 struct MultiWorld {
@@ -64,7 +92,9 @@ struct MultiWorld {
         __syncthreads();
         for (int j = threadIdx.x; j < worlds * 64; j += blockDim.x) {
             const int w = j >> 6, t = j & 63;
-            st->perm[w][t] = perms[(static_cast<size_t>(w) * kGenCount + g) * 64 + t];
+            const uint32_t v = perms[(static_cast<size_t>(w) * kGenCount + g) * 64 + t];
+            st->perm[w][t] = v;
+            st->perm[w][t + 64] = v;
         }
         for (int j = threadIdx.x; j < worlds * 3; j += blockDim.x) {
             const int w = j / 3, t = j % 3;
@@ -91,20 +121,33 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
     constexpr int NY = Y1 - Y0;
     double amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
-        s.stage(g0 + o);
-        if (active) {
-            const SharedPerm P = s.perm();
-            const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
-            YAxis<Y0, Y1> ya;
-            ya.build(sy, s.off(1));
-            // one world per workgroup: the same value on every lane, so make the branch scalar
-            const uint32_t fresh = S::kUniform ? __builtin_amdgcn_readfirstlane(ya.fresh) : ya.fresh;
-            const Axis ax = axis(X * sx + s.off(0));
-            const Axis az = axis(Z * sx + s.off(2));
-            double v[NY];
-            perlin3_column_cells<Y0, Y1>(P, ax, az, ya, fresh, w, v);
+        const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
+        if constexpr (S::kUniform) {
+            // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
+            // are the same on every lane, so the corner-reuse branch is scalar
+            s.template stage_y<Y0, Y1>(g0 + o, sy);
+            if (active) {
+                const YTable& yt = s.ytable();
+                const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
+                const Axis ax = axis(X * sx + s.off(0));
+                const Axis az = axis(Z * sx + s.off(2));
+                double v[NY];
+                perlin3_column_table<Y0, Y1>(s.perm(), ax, az, yt, fresh, w, v);
 #pragma unroll
-            for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+                for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+            }
+        } else {
+            s.stage(g0 + o);
+            if (active) {
+                YAxis<Y0, Y1> ya;
+                ya.build(sy, s.off(1));
+                const Axis ax = axis(X * sx + s.off(0));
+                const Axis az = axis(Z * sx + s.off(2));
+                double v[NY];
+                perlin3_column_cells<Y0, Y1>(s.perm(), ax, az, ya, ya.fresh, w, v);
+#pragma unroll
+                for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+            }
         }
         amp /= 2.0;
     }

@@ -78,11 +78,14 @@ struct LanePerm {
     __device__ __forceinline__ void set(int k, uint8_t v) const { base[off(static_cast<uint32_t>(k) & 255u)] = v; }
 };
 
-// A permutation shared by a whole workgroup (all lanes evaluate different columns of one
-// generator): 256 bytes in LDS, plain byte index.
+// A permutation shared by the lanes of one world (they evaluate different columns of one
+// generator): the full 512-entry Java table in LDS (d[i + 256] == d[i],
+// NoiseGeneratorPerlin173.java:74).  Every index the noise functions form is a table entry
+// (< 256) plus a lattice index (< 256) plus at most 1, so it is < 512 and needs no mask, and
+// the "+ 1" neighbours become immediate offsets of the LDS byte loads.
 struct SharedPerm {
     const uint8_t* t;
-    __device__ __forceinline__ int get(int k) const { return t[k & 255]; }
+    __device__ __forceinline__ int get(int k) const { return t[k]; }
 };
 
 // A Perlin (or simplex) generator build: NoiseGeneratorPerlin173.reseed (:63-78) and
@@ -140,11 +143,16 @@ __device__ __forceinline__ double lerp(double t, double a, double b) { return a 
 
 // a(int, x, y, z), :145-165.  Branch-free and exact: each case is (+/-u) + (+/-v) with
 // u = x for h < 8 else y, v = y for h < 4, x for h in {12,14}, else z.  Negation is exact.
+// The sign flips are XORs of bit 0 / bit 1 of h into the sign bit (IEEE negation).
+__device__ __forceinline__ double flip_sign(double v, uint32_t sign_bit) {
+    return __hiloint2double(__double2hiint(v) ^ static_cast<int>(sign_bit), __double2loint(v));
+}
 __device__ __forceinline__ double grad3(int h, double x, double y, double z) {
     h &= 15;
     const double u = h < 8 ? x : y;
     const double v = h < 4 ? y : ((h == 12 || h == 14) ? x : z);
-    return ((h & 1) ? -u : u) + ((h & 2) ? -v : v);
+    const uint32_t uh = static_cast<uint32_t>(h);
+    return flip_sign(u, uh << 31) + flip_sign(v, (uh << 30) & 0x80000000u);
 }
 
 // a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0.
@@ -306,6 +314,86 @@ __device__ __forceinline__ void perlin3_column_cells(Perm p, const Axis& ax, con
         }
         const double r0 = lerp(ya.fd[k], q00, q01);
         const double r1 = lerp(ya.fd[k], q10, q11);
+        out[k] = lerp(az.fd, r0, r1) * weight;
+    }
+}
+
+// The Y axis of one 3D octave for a workgroup whose lanes share the generator: lane k of
+// wave 0 evaluates level Y0 + k, the cell runs come from a ballot, and the table is
+// published in LDS.  Same values as YAxis (which every lane would build for itself).
+constexpr int kMaxLevels = 17;
+struct YTable {
+    double cell_f[kMaxLevels], cell_fm1[kMaxLevels], fd[kMaxLevels];
+    int yi[kMaxLevels];
+    uint32_t fresh;
+};
+
+// Called by every lane of wave 0.
+template <int Y0, int Y1>
+__device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
+    constexpr int NY = Y1 - Y0;
+    static_assert(NY <= kMaxLevels, "too many levels");
+    const int k = static_cast<int>(threadIdx.x);
+    const bool mine = k < NY;
+    const double v = static_cast<double>(Y0 + (mine ? k : 0)) * sy + b;
+    const int fl = java_floor(v);
+    const int idx = fl & 255;
+    const double f = v - static_cast<double>(fl);
+    const int idx_prev = __shfl_up(idx, 1);
+    const bool fresh = mine && (k == 0 || idx != idx_prev);
+    const uint32_t mask = static_cast<uint32_t>(__ballot(fresh));
+    // the cell of level Y0 may begin below Y0: walk back while the masked index stays the same
+    double v0 = static_cast<double>(Y0) * sy + b;
+    const int fl0 = java_floor(v0);
+    double start0 = v0 - static_cast<double>(fl0);
+    if (Y0 > 0) {
+        for (int y = Y0 - 1; y >= 0; --y) {
+            const double vy = static_cast<double>(y) * sy + b;
+            const int fy = java_floor(vy);
+            if ((fy & 255) != (fl0 & 255)) break;
+            start0 = vy - static_cast<double>(fy);
+        }
+    }
+    const int kk = mine ? k : 0;
+    const int start = 31 - __clz(static_cast<int>(mask & ((2u << kk) - 1u)));
+    double cf = __shfl(f, start);
+    if (start == 0) cf = start0;
+    if (mine) {
+        t.cell_f[k] = cf;
+        t.cell_fm1[k] = cf - 1.0;
+        t.fd[k] = fade(f);
+        t.yi[k] = idx;
+    }
+    if (k == 0) t.fresh = mask;
+}
+
+// perlin3_column_cells with the Y axis read from a shared table; `fresh` must be uniform.
+template <int Y0, int Y1, class Perm>
+__device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YTable& yt,
+                                                     uint32_t fresh, double weight, double* out) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+    double q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
+#pragma unroll
+    for (int k = 0; k < Y1 - Y0; ++k) {
+        if ((fresh >> k) & 1u) {
+            const int yi = yt.yi[k];
+            const double yf = yt.cell_f[k];
+            const double yfm1 = yt.cell_fm1[k];
+            const int p0 = xp0 + yi;
+            const int p00 = p.get(p0) + az.i;
+            const int p01 = p.get(p0 + 1) + az.i;
+            const int p1 = xp1 + yi;
+            const int p10 = p.get(p1) + az.i;
+            const int p11 = p.get(p1 + 1) + az.i;
+            q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        }
+        const double fd = yt.fd[k];
+        const double r0 = lerp(fd, q00, q01);
+        const double r1 = lerp(fd, q10, q11);
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
 }
