@@ -115,12 +115,15 @@ struct ColAcc {
     double n4, n5;
 };
 
+// 3D octave group g0 over octaves [o0, octaves): with o0 > 0 the sum of the
+// low-frequency part only (octave o carries weight 2^o; amp halves exactly per octave).
 template <int Y0, int Y1, class S>
 __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
-                                          double Z, bool active, double* acc) {
+                                          double Z, bool active, double* acc, int o0 = 0) {
     constexpr int NY = Y1 - Y0;
     double amp = 1.0;
-    for (int o = 0; o < octaves; ++o) {
+    for (int o = 0; o < o0; ++o) amp /= 2.0;
+    for (int o = o0; o < octaves; ++o) {
         const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
         if constexpr (S::kUniform) {
             // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
@@ -134,7 +137,7 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
                 double v[NY];
                 perlin3_column_table<Y0, Y1>(s.perm(), ax, az, yt, fresh, w, v);
 #pragma unroll
-                for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+                for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
         } else {
             s.stage(g0 + o);
@@ -146,7 +149,7 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
                 double v[NY];
                 perlin3_column_cells<Y0, Y1>(s.perm(), ax, az, ya, ya.fresh, w, v);
 #pragma unroll
-                for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+                for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
         }
         amp /= 2.0;

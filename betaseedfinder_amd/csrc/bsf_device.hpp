@@ -511,23 +511,57 @@ __device__ __forceinline__ ColumnShape column_shape(double temp, double rain, do
     return s;
 }
 
-// One lattice density (BetaTerrain173.java:792-829) at level y of a 17-level column.
-__device__ __forceinline__ double lattice_density(const ColumnShape& cs, int y, double n1, double n2, double n3) {
+// One lattice density (BetaTerrain173.java:792-829) at level y of a 17-level column, in
+// three parts: the height bias d9, the noise1 blend factor d12, and the density from them
+// and the noise2 / noise3 sums.
+__device__ __forceinline__ double level_bias(const ColumnShape& cs, int y) {
     double d9 = (static_cast<double>(y) - cs.d7) * 12.0 / cs.d5;
     if (d9 < 0.0) d9 *= 4.0;
-    const double d10 = n2 / 512.0;
-    const double d11 = n3 / 512.0;
-    const double d12 = (n1 / 10.0 + 1.0) / 2.0;
-    double d8;
-    if (d12 < 0.0) d8 = d10;
-    else if (d12 > 1.0) d8 = d11;
-    else d8 = d10 + (d11 - d10) * d12;
-    d8 -= d9;
+    return d9;
+}
+__device__ __forceinline__ double n1_blend(double n1) { return (n1 / 10.0 + 1.0) / 2.0; }
+// The top taper (y > 13), increasing in d8.
+__device__ __forceinline__ double top_taper(double d8, int y) {
     if (y > 13) {
         const double d13 = static_cast<double>(static_cast<float>(y - 13) / 3.0F);
         d8 = d8 * (1.0 - d13) + -10.0 * d13;
     }
     return d8;
+}
+__device__ __forceinline__ double density_from(double d9, double d12, double n2, double n3, int y) {
+    const double d10 = n2 / 512.0;
+    const double d11 = n3 / 512.0;
+    double d8;
+    if (d12 < 0.0) d8 = d10;
+    else if (d12 > 1.0) d8 = d11;
+    else d8 = d10 + (d11 - d10) * d12;
+    d8 -= d9;
+    return top_taper(d8, y);
+}
+__device__ __forceinline__ double lattice_density(const ColumnShape& cs, int y, double n1, double n2, double n3) {
+    return density_from(level_bias(cs, y), n1_blend(n1), n2, n3, y);
+}
+
+// Bounds of density_from when noise2 and noise3 are only known to lie in [n - u, n + u]
+// (widened by `slack` for rounding): the blend is a convex combination and the taper is
+// increasing, so the ends of the interval map to the ends.
+__device__ __forceinline__ void density_bounds(double d9, double d12, double n2, double n3, double u, int y,
+                                               double slack, double& lo, double& hi) {
+    const double l2 = (n2 - u) / 512.0, h2 = (n2 + u) / 512.0;
+    const double l3 = (n3 - u) / 512.0, h3 = (n3 + u) / 512.0;
+    double l, h;
+    if (d12 < 0.0) {
+        l = l2;
+        h = h2;
+    } else if (d12 > 1.0) {
+        l = l3;
+        h = h3;
+    } else {
+        l = l2 < l3 ? l2 : l3;
+        h = h2 > h3 ? h2 : h3;
+    }
+    lo = top_taper(l - d9, y) - slack;
+    hi = top_taper(h - d9, y) + slack;
 }
 
 }  // namespace bsf

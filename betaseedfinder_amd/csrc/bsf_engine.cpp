@@ -50,7 +50,8 @@ void IslandEngine::release() {
     drop_graphs();
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &perms_, &offs_, &surv_slot_, &surv_seed_, &surv_job_, &lattice_,
-                            &flood_list_, &flood_scratch_, &ctr_[0], &ctr_[1], &hits_[0], &hits_[1]})
+                            &flood_list_, &flood_scratch_, &open_, &surv_flag_, &ctr_[0], &ctr_[1], &hits_[0],
+                            &hits_[1]})
         b->release();
     auto free_pinned = [](void* p) {
         if (p) (void)hipHostFree(p);
@@ -102,6 +103,10 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     surv_job_.ensure(P * sizeof(int32_t));
     lattice_.ensure(P * static_cast<size_t>(kNdBucket[1]) * 8 * sizeof(double));
     flood_list_.ensure(static_cast<size_t>(kFloodClasses) * P * sizeof(int32_t));
+    // every sample column of every one-world seed can be open: 4 columns per chunk, 64 chunks
+    open_cap_ = static_cast<int64_t>(P) * 256;
+    open_.ensure(static_cast<size_t>(open_cap_) * sizeof(OpenColumn));
+    surv_flag_.ensure(P * sizeof(int32_t));
     for (int i = 0; i < 2; ++i) {
         ctr_[i].ensure(kCounters * sizeof(unsigned long long));
         hits_[i].ensure(P * sizeof(bsf_island_hit));
@@ -121,6 +126,7 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_cache_ = occupancy_grid(pass_cache_kernel, 64, 0, 0);
     grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
+    grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
     for (int b = 0; b < 2; ++b)
         grid_s2_[b] = occupancy_grid(pass_stage2_kernel, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
     grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, 256, flood_lds_bytes(0), 0);
@@ -239,6 +245,9 @@ EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
     a.hits = static_cast<bsf_island_hit*>(hits_[ab].ptr);
     a.flood_scratch = static_cast<unsigned long long*>(flood_scratch_.ptr);
     a.flood_scratch_words = flood_scratch_words_;
+    a.open = static_cast<OpenColumn*>(open_.ptr);
+    a.open_cap = open_cap_;
+    a.surv_flag = static_cast<int32_t*>(surv_flag_.ptr);
     a.stage1_only = stage1_only ? 1 : 0;
     return a;
 }
@@ -247,7 +256,7 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
     const EngineArgs a = args_for(ab, stage1_only);
     hipStream_t s = stream_;
     const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
-    hipLaunchKernelGGL(pass_reset_kernel, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(pass_reset_kernel, dim3(256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(pass_build_kernel, dim3(1), dim3(256), 0, s, a);
     check(hipGetLastError(), "pass build launch");
     {
@@ -270,6 +279,12 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
         hipLaunchKernelGGL(s1_multi, dim3(grid_s1_[1]), dim3(64), 0, s, a);
         check(hipGetLastError(), "stage 1 launch");
         t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[1]));
+    }
+    {
+        LaunchTimer t("island_stage1b", s, timers_);
+        hipLaunchKernelGGL(pass_stage1b_kernel, dim3(grid_s1b_), dim3(64), 0, s, a);
+        check(hipGetLastError(), "stage 1b launch");
+        t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[0]));
     }
     if (stage1_only) return;
     {

@@ -108,7 +108,9 @@ class IslandEngine {
     bool timers_ = false, use_graphs_ = true;
     hipStream_t stream_ = nullptr, copy_stream_ = nullptr;
     DeviceBuffer jobs_, segs_, desc_, perms_, offs_, surv_slot_, surv_seed_, surv_job_, lattice_, flood_list_,
-            flood_scratch_;
+            flood_scratch_, open_, surv_flag_;
+    int64_t open_cap_ = 0;
+    int grid_s1b_ = 0;
     DeviceBuffer ctr_[2], hits_[2];
     // pinned host memory
     IslandJob* h_jobs_ = nullptr;

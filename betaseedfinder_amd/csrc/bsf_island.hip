@@ -35,9 +35,12 @@ hipError_t upload_island_tables() {
     return hipMemcpyToSymbol(HIP_SYMBOL(isl_magic), t, sizeof(t));
 }
 
+// Clears the pass counters and the survivor flags (a fill kernel, so that the pass can be
+// captured as a graph).
 __global__ void pass_reset_kernel(EngineArgs a) {
-    const int t = threadIdx.x;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < kCounters) a.ctr[t] = 0;
+    for (int64_t i = t; i < a.pass_slots; i += static_cast<int64_t>(gridDim.x) * blockDim.x) a.surv_flag[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -272,6 +275,67 @@ __device__ __forceinline__ bool stage1_column(const S& s, int cx, int cz, int i,
     return found;
 }
 
+// Stage 1a of one sample column: exact noise1 / noise4 / noise5 / climate, noise2 and noise3
+// over octaves kSplitOctave..15; returns whether an upward air-to-stone crossing at
+// y >= 64 is still possible, with the column's exact height biases and blend factors.
+template <class S>
+__device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i, int l, bool active, double* d9,
+                                               double* d12) {
+    const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
+    const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
+    ColAcc<8, 16> acc;
+    colpar_3d<8, 16>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, acc.n2, kSplitOctave);
+    colpar_3d<8, 16>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, acc.n3, kSplitOctave);
+    colpar_3d<8, 16>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, acc.n1);
+    acc.n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
+    acc.n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
+    double temp = 0.0, rain = 0.0;
+    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain);
+    const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
+    double lo[9], hi[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        d9[k] = level_bias(cs, 8 + k);
+        d12[k] = n1_blend(acc.n1[k]);
+        density_bounds(d9[k], d12[k], acc.n2[k], acc.n3[k], kSplitBound, 8 + k, kSplitSlack, lo[k], hi[k]);
+    }
+    lo[8] = hi[8] = -10.0;
+    // a candidate start needs density <= 0 at some block and > 0 at the block above; the
+    // blocks of y-cell k run linearly from level k to level k + 1, so that needs
+    // lat[k] <= 0 < lat[k + 1] for some k in 8..15 (y >= 64 covers y >= 68)
+    bool open = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) open |= lo[k] <= 0.0 && hi[k + 1] > 0.0;
+    return active && open;
+}
+
+// Wave-aggregated append of open columns.
+__device__ __forceinline__ void append_open(const EngineArgs& a, bool open, int64_t seed, int slot, int job, int X,
+                                            int Z, const double* d9, const double* d12) {
+    const unsigned long long mask = __ballot(open);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctr[kCtrOpen], static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (open) {
+        const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+        if (static_cast<int64_t>(k) < a.open_cap) {
+            OpenColumn& c = a.open[k];
+            c.seed = seed;
+            c.slot = slot;
+            c.job = job;
+            c.X = X;
+            c.Z = Z;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                c.d9[i] = d9[i];
+                c.d12[i] = d12[i];
+            }
+        }
+    }
+}
+
 // KIND 0: one world per wave (column batches of 64 for large windows).  KIND 1: several
 // worlds side by side, or no gate.  Separate kernels keep the register budget of the common
 // one-world form (and so its occupancy) free of the other form.
@@ -303,17 +367,19 @@ __global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
         const uint32_t* perms = a.perms + static_cast<size_t>(slot_first) * kGenCount * 64;
         const double* offs = a.offsets + static_cast<size_t>(slot_first) * kGenCount * 3;
         if (KIND == 0) {
+            // Stage 1a: open columns go to Stage 1b, which appends the survivors
             const OneWorld s{&st, perms, offs};
-            bool found = false;
+            const int64_t seed = job_seed(job, g.start + (slot_first - g.slot0));
             for (int base = 0; base < ncols; base += 64) {
                 const int c = base + tid;
                 const bool active = c < ncols;
                 const int ch = active ? c >> 2 : 0, sc = c & 3;
                 const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
-                found |= stage1_column(s, p.chunk_x0 + ch / cz_n, p.chunk_z0 + ch % cz_n, i, l, active);
+                const int cx = p.chunk_x0 + ch / cz_n, cz = p.chunk_z0 + ch % cz_n;
+                double d9[8], d12[8];
+                const bool open = stage1a_column(s, cx, cz, i, l, active, d9, d12);
+                append_open(a, open, seed, static_cast<int>(slot_first), g.job, cx * 4 + i, cz * 4 + l, d9, d12);
             }
-            const bool any = __ballot(found) != 0;
-            append_survivors(a, tid == 0 && any, slot_first, job_seed(job, g.start + (slot_first - g.slot0)), g.job);
         } else if (!job.gate) {
             // no gate for this job: every seed of the item goes to Stage 2
             const bool mine = tid < nk;
@@ -340,6 +406,91 @@ __global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
 
 template __global__ void pass_stage1_kernel<0>(EngineArgs);
 template __global__ void pass_stage1_kernel<1>(EngineArgs);
+
+// Stage 1b: one open column per lane, noise2 and noise3 over all 16 octaves in Java order
+// with each lane's own world (lane-private permutations staged from the cache), then the
+// exact lattice densities of levels 8..15 and the candidate-start chain.
+template <int Y0, int Y1>
+__device__ __forceinline__ void lane_octaves3d(LanePerm P, const uint32_t* __restrict__ perms,
+                                               const double* __restrict__ offs, int g0, int octaves, double s_xz,
+                                               double s_y, double X, double Z, double* acc) {
+    constexpr int NY = Y1 - Y0;
+    uint32_t* w = reinterpret_cast<uint32_t*>(P.base);
+    double amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        const uint4* src = reinterpret_cast<const uint4*>(perms + (g0 + o) * 64);
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const uint4 v = src[q];
+            w[(4 * q + 0) * 64] = v.x;
+            w[(4 * q + 1) * 64] = v.y;
+            w[(4 * q + 2) * 64] = v.z;
+            w[(4 * q + 3) * 64] = v.w;
+        }
+        const double oa = offs[(g0 + o) * 3 + 0], ob = offs[(g0 + o) * 3 + 1], oc = offs[(g0 + o) * 3 + 2];
+        const double sx = s_xz * amp, sy = s_y * amp, wgt = 1.0 / amp;
+        YAxis<Y0, Y1> ya;
+        ya.build(sy, ob);
+        const Axis ax = axis(X * sx + oa);
+        const Axis az = axis(Z * sx + oc);
+        double v[NY];
+        perlin3_column_cells<Y0, Y1>(P, ax, az, ya, ya.fresh, wgt, v);
+#pragma unroll
+        for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+        amp /= 2.0;
+    }
+}
+
+__global__ __launch_bounds__(64) void pass_stage1b_kernel(EngineArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    __shared__ unsigned long long s_grp;
+    const int lane = threadIdx.x;
+    LanePerm P;
+    P.base = perm_lds + lane * 4;
+    unsigned long long n = a.ctr[kCtrOpen];
+    if (n > static_cast<unsigned long long>(a.open_cap)) n = static_cast<unsigned long long>(a.open_cap);
+    for (;;) {
+        if (lane == 0) s_grp = atomicAdd(&a.ctr[kCtrStage1b], 1ULL);
+        __syncthreads();
+        const unsigned long long grp = s_grp;
+        __syncthreads();
+        if (grp * 64 >= n) break;
+        const unsigned long long idx = grp * 64 + lane;
+        const bool active = idx < n;
+        const OpenColumn c = a.open[active ? idx : n - 1];
+        const uint32_t* perms = a.perms + static_cast<size_t>(c.slot) * kGenCount * 64;
+        const double* offs = a.offsets + static_cast<size_t>(c.slot) * kGenCount * 3;
+        const double X = static_cast<double>(c.X), Z = static_cast<double>(c.Z);
+        double n2[8], n3[8];
+        lane_octaves3d<8, 16>(P, perms, offs, kGenN2, 16, 684.412, 684.412, X, Z, n2);
+        lane_octaves3d<8, 16>(P, perms, offs, kGenN3, 16, 684.412, 684.412, X, Z, n3);
+        double lat[9];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) lat[k] = density_from(c.d9[k], c.d12[k], n2[k], n3[k], 8 + k);
+        lat[8] = -10.0;
+        bool found = false, below = false;
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+            double d1 = lat[k1];
+            const double d5 = (lat[k1 + 1] - d1) * 0.125;
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const bool stone = d1 > 0.0;
+                if (64 + k1 * 8 + l1 >= 68 && stone && !below) found = true;
+                below = stone;
+                d1 += d5;
+            }
+        }
+        if (active && found && atomicOr(&a.surv_flag[c.slot], 1) == 0) {
+            const unsigned long long k = atomicAdd(&a.ctr[kCtrSurv], 1ULL);
+            if (static_cast<int64_t>(k) < a.pass_slots) {
+                a.surv_slot[k] = c.slot;
+                a.surv_seed[k] = c.seed;
+                a.surv_job[k] = c.job;
+            }
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Stage 2

@@ -3,7 +3,8 @@
 // A *pass* is a fixed sequence of persistent kernels over at most `pass_slots` seeds taken
 // from the jobs resident in a device job table:
 //
-//   pass_reset -> pass_build -> pass_cache -> pass_stage1 -> pass_stage2 -> pass_flood x 4
+//   pass_reset -> pass_build -> pass_cache -> pass_stage1 (1a, multi) -> pass_stage1b
+//     -> pass_stage2 -> pass_flood x 4
 //
 // pass_build hands every active job a share of the pass (its seed cursor lives in the job
 // table and is advanced on the device), so the same launch sequence -- captured once as a
@@ -63,6 +64,8 @@ enum {
     kCtrFloodCount = 8,  // 4 entries: survivors per flood class
     kCtrHits = 12,
     kCtrStage1Multi = 13,
+    kCtrOpen = 14,       // open columns left by Stage 1a
+    kCtrStage1b = 15,
     kCounters = 16
 };
 
@@ -73,6 +76,27 @@ enum {
     kFloodClassWideZ = 2,  // z <= 128, x <= 64: LDS, two words per row
     kFloodClassLarge = 3,  // z <= 128, x <= 128: bitsets in a per-workgroup global scratch
     kFloodClasses = 4
+};
+
+// Stage 1 in two steps for one-world waves.  Stage 1a sums noise2 / noise3 over octaves
+// kSplitOctave..15 only (few Y cells each) plus noise1, noise4, noise5 and the climate in
+// full; octaves 0..kSplitOctave-1 of noise2 and noise3 are bounded.  Each octave is a 3D
+// Perlin value (|value| <= 2: a trilinear blend of gradients +/-u +/-v with |u|, |v| <= 1)
+// times its weight 2^o, so the two sums are within 2 * (2^kSplitOctave - 1) of their
+// low-frequency parts.  A column whose density bounds rule out an upward air-to-stone
+// crossing is closed; the others ("open" columns) are evaluated exactly by Stage 1b.
+constexpr int kSplitOctave = 10;
+constexpr double kSplitBound = 2.0 * ((1 << kSplitOctave) - 1);
+constexpr double kSplitSlack = 1e-6;   // rounding of the bounds and of the y-chain, far above either
+
+// One open column: where it is and the exact per-level parts that Stage 1a computed.
+struct OpenColumn {
+    int64_t seed;
+    int32_t slot;               // pass slot (generator cache entry)
+    int32_t job;
+    int32_t X, Z;               // noise lattice column
+    double d9[8];               // height bias of levels 8..15
+    double d12[8];              // noise1 blend factor of levels 8..15
 };
 
 constexpr int kFloorY = 68;
@@ -103,6 +127,9 @@ struct EngineArgs {
     bsf_island_hit* hits;          // [pass_slots]
     unsigned long long* flood_scratch;  // kFloodClassLarge bitsets, flood_scratch_words per workgroup
     int64_t flood_scratch_words;
+    OpenColumn* open;              // Stage 1a -> 1b
+    int64_t open_cap;
+    int32_t* surv_flag;            // [pass_slots] a Stage-1b survivor is appended once
     int32_t stage1_only;           // 1: stop after Stage 1 (survivor audits)
     int32_t pad;
 };
@@ -112,6 +139,7 @@ __global__ void pass_build_kernel(EngineArgs a);
 __global__ void pass_cache_kernel(EngineArgs a);
 template <int KIND>
 __global__ void pass_stage1_kernel(EngineArgs a);
+__global__ void pass_stage1b_kernel(EngineArgs a);
 __global__ void pass_stage2_kernel(EngineArgs a);
 template <int W, bool GLOBAL>
 __global__ void pass_flood_kernel(EngineArgs a, int cls);
