@@ -29,11 +29,10 @@ MASK48 = (1 << 48) - 1
 METRIC = "seeds/sec full island pipeline (bit-exact) + p50/p99 job latency, served vs plain"
 FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64, FMA counted as two flops
 FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no FMA: -ffp-contract=off)
-# HBM bytes per seed of pass_stage1_kernel<0>: FETCH_SIZE (x 1.99975, the factor a 256 MiB copy
-# reads back) 21,608 B + WRITE_SIZE 45 B, from the rocprofv3 --pmc runs summarised in
-# profiles/island_pipeline_4x4_v1134.md.  The algorithmic figure is the world's 76 cached
-# generators, 76 x (256 + 24) = 21,280 B.
-STAGE1_HBM_BYTES_PER_SEED = 21608 + 45
+# HBM bytes per unit of work of the FP64 kernels: FETCH_SIZE x 1.99975 (the factor by which
+# a 256 MiB device copy under-reads) + WRITE_SIZE, from the rocprofv3 --pmc runs summarised in
+# profiles/island_pipeline_4x4_v1134.md.
+HBM_BYTES_PER_UNIT = {"island_stage1": 16711, "island_stage2": 109515}
 
 
 def splitmix64(x):
@@ -69,23 +68,52 @@ OPS = {
     "shape": 20,            # column shape d5 / d7
     "density_level": 15,    # one lattice density
     "chain_level": 18,      # candidate y-chain per lattice level (sub, mul, 8 x (cmp, add))
+    "bounds_level": 14,     # Stage-1a density bounds of one level (interval ends, blend hull, taper)
 }
-NOISE_OCTAVES = ([("n2", 684.412 / 2 ** o) for o in range(16)] + [("n3", 684.412 / 2 ** o) for o in range(16)] +
-                 [("n1", 684.412 / 160.0 / 2 ** o) for o in range(8)])
+def cells(levels, sy):
+    """Expected Y cells (corner sets) of one octave over `levels` lattice levels: a cell of
+    scale sy covers 1/sy levels."""
+    return 1 + (levels - 1) * min(1.0, sy)
 
 
-def stage1_fp64_ops_per_seed(n_cols=64, levels=8):
-    """Expected algorithmic FP64 operations of one Stage-1 seed (64 sample columns, 8 levels).
-    A y cell of scale s covers 1/s levels, so 1 + (levels - 1) * min(1, s) corner sets are
-    computed per column and octave."""
-    o3d = 0.0
-    for _, sy in NOISE_OCTAVES:
-        cells = 1 + (levels - 1) * min(1.0, sy)
-        o3d += levels * OPS["yaxis_level"] + n_cols * (2 * OPS["axis"] + cells * OPS["corners"] + levels * OPS["point3"])
-    o2d = 26 * n_cols * (2 * OPS["axis"] + OPS["point2"])
-    clim = 10 * n_cols * OPS["simplex"]
-    combine = n_cols * (OPS["finish"] + OPS["shape"] + levels * (OPS["density_level"] + OPS["chain_level"]))
-    return int(round(o3d + o2d + clim + combine))
+def column3d_ops(levels, octaves, y_table=True):
+    """Algorithmic FP64 ops of 3D octaves [(sy), ...] at one column over `levels` levels; the
+    y axis is counted per octave only when it is not shared (y_table=False)."""
+    ops = 0.0
+    for sy in octaves:
+        ops += 2 * OPS["axis"] + cells(levels, sy) * OPS["corners"] + levels * OPS["point3"]
+        if not y_table:
+            ops += levels * OPS["yaxis_level"]
+    return ops
+
+
+N2 = [684.412 / 2 ** o for o in range(16)]
+N1 = [684.412 / 160.0 / 2 ** o for o in range(8)]
+SPLIT = 10   # Stage 1a sums noise2 / noise3 over octaves SPLIT..15
+
+
+def stage1a_ops_per_seed(n_cols=64, levels=8):
+    """Stage 1a: noise2 / noise3 octaves SPLIT..15, noise1, noise4/5, climate, bounds."""
+    o3 = N2[SPLIT:] * 2 + N1
+    per_col = (column3d_ops(levels, o3) + 26 * (2 * OPS["axis"] + OPS["point2"]) + 10 * OPS["simplex"] +
+               OPS["finish"] + OPS["shape"] + levels * (OPS["density_level"] + OPS["bounds_level"]))
+    return n_cols * per_col + len(o3) * levels * OPS["yaxis_level"]
+
+
+def stage1b_ops_per_column(levels=8):
+    """Stage 1b: one open column, noise2 + noise3 over all 16 octaves, densities, y-chain."""
+    return column3d_ops(levels, N2 * 2, y_table=False) + levels * (OPS["density_level"] + OPS["chain_level"])
+
+
+def stage2_ops_per_survivor(w=4, h=4, levels=8):
+    """Stage 2: the (4w+1)(4h+1) noise lattice (all 40 3D + 26 2D octaves) and the 25 w h
+    density columns (climate, shape, densities)."""
+    nc, nd = (4 * w + 1) * (4 * h + 1), 25 * w * h
+    batches = (nc + 63) // 64
+    noise = nc * (column3d_ops(levels, N2 * 2 + N1) + 26 * (2 * OPS["axis"] + OPS["point2"]))
+    ytab = batches * 40 * levels * OPS["yaxis_level"]
+    dens = nd * (10 * OPS["simplex"] + OPS["finish"] + OPS["shape"] + levels * OPS["density_level"])
+    return noise + ytab + dens
 
 
 # The four filter profiles of the batched and served workloads: gate thresholds drawn
@@ -298,14 +326,25 @@ def main():
     stats1 = bsf.engine_stats()
     seeds = args.steps * batch
     value = seeds / elapsed
-    cache_ms, _, _ = kstats("island_cache")
-    s1_ms, s1_n, s1_units = kstats("island_stage1")
-    s2_ms, _, _ = kstats("island_stage2")
-    fl_ms, fl_n, _ = kstats("island_flood")
-    ops_seed = stage1_fp64_ops_per_seed()
-    avg_launch_s = s1_ms / 1e3 / max(1, s1_n)
-    seeds_per_launch = s1_units / max(1, s1_n)
-    achieved_tflops = ops_seed * seeds_per_launch / avg_launch_s / 1e12
+    names = ["island_cache", "island_stage1", "island_stage1_multi", "island_stage1b", "island_stage2", "island_flood"]
+    ks = {n: kstats(n) for n in names}
+    passes = stats1[0] - stats0[0]
+    survivors = stats1[2] - stats0[2]
+    # the FP64 kernels and their algorithmic work per launch
+    fp64 = {
+        "island_stage1": dict(kernel="pass_stage1_kernel<0> (Stage 1a: 64 sample columns per seed, low-frequency "
+                                     "noise2/noise3 octaves + bounds)",
+                              unit="seed", ops=stage1a_ops_per_seed(), units=ks["island_stage1"][2]),
+        "island_stage2": dict(kernel="pass_stage2_kernel (Stage 2: 17x17 noise lattice + 400 density columns per "
+                                     "survivor)",
+                              unit="survivor", ops=stage2_ops_per_survivor(), units=survivors),
+    }
+    dom = max(fp64, key=lambda n: ks[n][0])
+    d = fp64[dom]
+    launches = max(1, ks[dom][1])
+    avg_launch_s = ks[dom][0] / 1e3 / launches
+    units_per_launch = d["units"] / launches
+    achieved_tflops = d["ops"] * units_per_launch / avg_launch_s / 1e12
 
     result = {
         "metric": METRIC,
@@ -328,23 +367,23 @@ def main():
             "min_island_blocks": min_blocks,
             "hits_in_timed_region": hits,
             "stage1_survivor_fraction": round((stats1[2] - stats0[2]) / max(1, stats1[1] - stats0[1]), 5),
-            "passes": stats1[0] - stats0[0],
-            "kernel_ms": {"island_cache": round(cache_ms, 3), "island_stage1": round(s1_ms, 3), "island_stage2": round(s2_ms, 3),
-                          "island_flood": round(fl_ms, 3)},
+            "passes": passes,
+            "kernel_ms": {n: round(ks[n][0], 3) for n in names},
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "pass_stage1_kernel<0> (island Stage 1, one world per wave: 64 sample columns per seed)",
+            "kernel": d["kernel"],
             "achieved": round(achieved_tflops, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
             "frac_of_nonfused_issue_peak": round(achieved_tflops / FP64_NONFUSED_PEAK, 4),
-            "fp64_ops_per_seed": ops_seed,
-            "seeds_per_launch": int(seeds_per_launch),
+            "fp64_ops_per_unit": int(round(d["ops"])),
+            "unit_of_work": d["unit"],
+            "units_per_launch": round(units_per_launch, 1),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-            "traffic": int(round(STAGE1_HBM_BYTES_PER_SEED * seeds_per_launch)),
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "traffic": int(round(HBM_BYTES_PER_UNIT[dom] * units_per_launch)),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x calibration + WRITE_SIZE)",
         },
     }
 
