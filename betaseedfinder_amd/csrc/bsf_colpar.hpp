@@ -42,32 +42,48 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
     __syncthreads();
 }
 
-// Stage generator g of a 3D octave with y scale sy, and its Y-axis table over [Y0, Y1).
-template <int Y0, int Y1>
-__device__ __forceinline__ void stage_generator_y(GenStage& st, const uint32_t* __restrict__ perms,
-                                                  const double* __restrict__ offs, int g, double sy) {
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < 64) {
-        const uint32_t v = perms[g * 64 + t];
-        st.perm[t] = v;
-        st.perm[t + 64] = v;
-        build_ytable<Y0, Y1>(st.yt, sy, offs[g * 3 + 1]);
-    }
-    if (t < 3) st.off[t] = offs[g * 3 + t];
-    __syncthreads();
-}
-
-// Stager for one world per workgroup: every lane reads the same staged generator.
+// Stager for one world per workgroup: every lane reads the same staged generator.  The
+// cache entry of the next generator (g + 1, the usual successor) is loaded into registers
+// while generator g is evaluated, so a stage only waits for its LDS stores.
 struct OneWorld {
     static constexpr bool kUniform = true;
     GenStage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
-    __device__ __forceinline__ void stage(int g) const { stage_generator(*st, perms, offs, g); }
+    mutable int pf_g = -1;  // generator held in the prefetch registers
+    mutable uint32_t pf_perm = 0;
+    mutable double pf_off = 0.0;  // offset[threadIdx.x] for threads 0..2
+    mutable double pf_b = 0.0;    // offset[1] in every thread (the y offset)
+
+    __device__ __forceinline__ void fetch(int g) const {
+        const int t = threadIdx.x;
+        if (t < 64) pf_perm = perms[g * 64 + t];
+        pf_off = offs[g * 3 + (t < 3 ? t : 0)];
+        pf_b = offs[g * 3 + 1];
+        pf_g = g;
+    }
+    __device__ __forceinline__ void publish(int g) const {
+        if (pf_g != g) fetch(g);
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < 64) {
+            st->perm[t] = pf_perm;
+            st->perm[t + 64] = pf_perm;
+        }
+        if (t < 3) st->off[t] = pf_off;
+    }
+    __device__ __forceinline__ void stage(int g) const {
+        publish(g);
+        __syncthreads();
+        if (g + 1 < kGenCount) fetch(g + 1);
+    }
+    // Generator g of a 3D octave with y scale sy, and its Y-axis table over [Y0, Y1).
     template <int Y0, int Y1>
     __device__ __forceinline__ void stage_y(int g, double sy) const {
-        stage_generator_y<Y0, Y1>(*st, perms, offs, g, sy);
+        publish(g);
+        if (threadIdx.x < 64) build_ytable<Y0, Y1>(st->yt, sy, pf_b);
+        __syncthreads();
+        if (g + 1 < kGenCount) fetch(g + 1);
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }

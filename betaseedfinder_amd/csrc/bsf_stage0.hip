@@ -118,7 +118,7 @@ __device__ __forceinline__ int highest_reentry(uint32_t positive) {
 
 // P20 / upper: Y 11..15 of the item's Stage0 columns (Y 16 is -10 after the taper).
 template <int MODE>
-__global__ __launch_bounds__(256) void s0_upper_kernel(S0Args a) {
+__global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
     __shared__ GenStage st;
     __shared__ int s_count;
     const int tid = threadIdx.x;
