@@ -238,6 +238,36 @@ def run_trace(bsf, trace, fifo, horizon_s):
     }
 
 
+def run_density_scout(bsf, args):
+    """The other two fixed workloads, GPU only: the 65,536-seed density grid of chunk (0,0)
+    (host arrays in and out) and the Stage-0 scout over 2^scout_log2 seeds of its range."""
+    import numpy as np
+    seeds = np.arange(65536, dtype=np.int64)
+    bsf.density_grid(seeds[:1024])
+    t = time.perf_counter()
+    grid = bsf.density_grid(seeds)
+    dens_s = time.perf_counter() - t
+    start = splitmix64(0xB5F00002) & MASK48
+    thr = -50.0 + 100.0 * unit53(0xB5F00003)
+    n = 1 << args.scout_log2
+    bsf.scout_range(start, 1 << 20, thr)
+    t = time.perf_counter()
+    surv = bsf.scout_range(start, n, thr)
+    scout_s = time.perf_counter() - t
+    digest = 0
+    for v in surv.tolist():
+        digest = (digest + splitmix64(v & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+    return {
+        "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
+                         "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
+                         "checksum": float(np.sum(grid))},
+        "scout": {"workload": "Stage-0 scout, 2^%d seeds from splitmix64(0xB5F00002) & (2^48-1), threshold %.6f"
+                              % (args.scout_log2, thr),
+                  "seeds_per_s": round(n / scout_s, 1), "survivors": int(len(surv)),
+                  "digest": "%016x" % digest},
+    }
+
+
 def run_serving(bsf, args):
     trace = served_trace(bsf, args.serve_seconds, args.serve_rate, args.serve_median_log2)
     served = run_trace(bsf, trace, False, args.serve_seconds + args.serve_drain)
@@ -260,6 +290,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--scout-log2", type=int, default=30)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-seeds", type=int, default=1 << 15)
@@ -412,6 +444,8 @@ def main():
             "sample": "first %d seeds of the same range through the CPU oracle (%d threads, %.1f s); "
                       "hit lists equal: %s" % (n, threads, dt, gpu_sample == cpu_hits),
         }
+    if not args.no_configs:
+        result["other_workloads"] = run_density_scout(bsf, args)
     if not args.no_batched:
         result["batched"] = run_batched(bsf, args)
     if not args.no_serve:
