@@ -451,3 +451,28 @@ def test_scout_statistic_is_octave_zero_sum(oracle):
                 for v in col:
                     s += v
         assert d2h(s) == d2h(oracle.scout_statistic(seed))
+
+
+def test_perlin_value_bound_used_by_the_stage1_split(oracle):
+    """Stage 1a bounds every 3D Perlin octave by |value| <= 2 (gradients +/-u +/-v with |u|, |v| <= 1,
+    blended with weights in [0, 1]).  Points sampled near cell corners, edges and centres of many
+    generators, with the Y-cell quirk (column form), stay inside it."""
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    perm = (ctypes.c_uint8 * 256)()
+    abc = (ctypes.c_double * 3)()
+    worst = 0.0
+    fracs = np.array([0.0, 1e-9, 0.25, 0.5, 0.75, 1.0 - 1e-9])
+    for seed in range(40):
+        L.bsfo_perlin_state(ctypes.c_int64(seed), seed % 40, perm, abc)
+        for _ in range(60):
+            base = rng.integers(-1000, 1000, size=3)
+            fx, fy, fz = rng.choice(fracs, size=3)
+            x, y, z = base[0] + fx - abc[0], base[1] + fy - abc[1], base[2] + fz - abc[2]
+            v = L.bsfo_perlin3_point(perm, abc, ctypes.c_double(x), ctypes.c_double(y), ctypes.c_double(z))
+            worst = max(worst, abs(v))
+            col = (ctypes.c_double * 17)()
+            L.bsfo_perlin3_column(perm, abc, ctypes.c_double(x), ctypes.c_double(rng.uniform(0.01, 3.0)),
+                                  ctypes.c_double(z), 17, col)
+            worst = max(worst, max(abs(c) for c in col))
+    assert 0.5 < worst <= 2.0
