@@ -106,6 +106,9 @@ __device__ __forceinline__ Z<2> zfill(const Z<2>& s, const Z<2>& m) {
     return r;
 }
 
+// Lattice values are below 2^11 in magnitude, so the upsampler's rounding stays below 1e-12.
+constexpr double kSignMargin = 1e-9;
+
 // The eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks) from one lattice
 // y-cell, walking generateBareTerrain's incremental chain (BetaTerrain173.java:599-664) once:
 // d1..d4 step by d5..d8 per level, d10/d11 step i2 times by d12/d13 along x, d15 steps by
@@ -124,13 +127,29 @@ __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, i
                 return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
             };
             double d1 = L(i1, j1, k1), d2 = L(i1, j1 + 1, k1), d3 = L(i1 + 1, j1, k1), d4 = L(i1 + 1, j1 + 1, k1);
-            const double d5 = (L(i1, j1, k1 + 1) - d1) * 0.125;
-            const double d6 = (L(i1, j1 + 1, k1 + 1) - d2) * 0.125;
-            const double d7 = (L(i1 + 1, j1, k1 + 1) - d3) * 0.125;
-            const double d8 = (L(i1 + 1, j1 + 1, k1 + 1) - d4) * 0.125;
+            const double e1 = L(i1, j1, k1 + 1), e2 = L(i1, j1 + 1, k1 + 1), e3 = L(i1 + 1, j1, k1 + 1),
+                         e4 = L(i1 + 1, j1 + 1, k1 + 1);
             const int shift = czi * 16 + j1 * 4;
             const bool hi_word = W == 2 && shift >= 64;
             const int sh = shift & 63;
+            // every block of the cell is a convex combination of its 8 corners (up to a few ulps),
+            // so a cell whose corners are all clearly negative is air and all clearly positive
+            // is stone
+            const double mx = fmax(fmax(fmax(d1, d2), fmax(d3, d4)), fmax(fmax(e1, e2), fmax(e3, e4)));
+            const double mn = fmin(fmin(fmin(d1, d2), fmin(d3, d4)), fmin(fmin(e1, e2), fmin(e3, e4)));
+            if (mx <= -kSignMargin) continue;
+            if (mn > kSignMargin) {
+#pragma unroll
+                for (int l1 = 0; l1 < 8; ++l1) {
+                    if (hi_word) rows[l1].v[W - 1] |= 0xFULL << sh;
+                    else rows[l1].v[0] |= 0xFULL << sh;
+                }
+                continue;
+            }
+            const double d5 = (e1 - d1) * 0.125;
+            const double d6 = (e2 - d2) * 0.125;
+            const double d7 = (e3 - d3) * 0.125;
+            const double d8 = (e4 - d4) * 0.125;
 #pragma unroll
             for (int l1 = 0; l1 < 8; ++l1) {
                 const double d12 = (d3 - d1) * 0.25;
@@ -197,74 +216,112 @@ __device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats&
 // rows up and down in y, then thread y walks its rows up and down in x; z runs fill in one
 // step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is the
 // component and the order of the sweeps cannot change it.
+// One sweep of thread x along y over its x slice (up, then down): adds the blocks of rem that
+// touch comp within the slice or in the neighbouring slices' rows.  Returns whether it grew.
+template <int W>
+__device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, int nx) {
+    const int t = threadIdx.x;
+    int changed = 0;
+    if (t < nx) {
+        unsigned long long* row = comp + static_cast<size_t>(t) * kRowStride * W;
+        const unsigned long long* mrow = rem + static_cast<size_t>(t) * kRowStride * W;
+        const unsigned long long* left = t > 0 ? row - kRowStride * W : nullptr;
+        const unsigned long long* right = t < nx - 1 ? row + kRowStride * W : nullptr;
+        Z<W> prev = zzero<W>();
+        for (int y = 0; y < kLevels; ++y) {
+            const Z<W> m = zload<W>(mrow + y * W);
+            const Z<W> c = zload<W>(row + y * W);
+            Z<W> nb = zor(c, prev);
+            if (left) nb = zor(nb, zload<W>(left + y * W));
+            if (right) nb = zor(nb, zload<W>(right + y * W));
+            const Z<W> n = zfill(zand(nb, m), m);
+            if (!zeq(n, c)) {
+                zstore(row + y * W, n);
+                changed = 1;
+            }
+            prev = n;
+        }
+        prev = zzero<W>();
+        for (int y = kLevels - 1; y >= 0; --y) {
+            const Z<W> m = zload<W>(mrow + y * W);
+            const Z<W> c = zload<W>(row + y * W);
+            const Z<W> n = zfill(zand(zor(c, prev), m), m);
+            if (!zeq(n, c)) {
+                zstore(row + y * W, n);
+                changed = 1;
+            }
+            prev = n;
+        }
+    }
+    return changed;
+}
+
+// One sweep of thread y along x over its y level (forward, then back).
+template <int W>
+__device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, int nx) {
+    const int t = threadIdx.x;
+    int changed = 0;
+    if (t < kLevels) {
+        Z<W> prev = zzero<W>();
+        for (int x = 0; x < nx; ++x) {
+            const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
+            const Z<W> m = zload<W>(rem + r);
+            const Z<W> c = zload<W>(comp + r);
+            Z<W> nb = zor(c, prev);
+            if (t > 0) nb = zor(nb, zload<W>(comp + r - W));
+            if (t < kLevels - 1) nb = zor(nb, zload<W>(comp + r + W));
+            const Z<W> n = zfill(zand(nb, m), m);
+            if (!zeq(n, c)) {
+                zstore(comp + r, n);
+                changed = 1;
+            }
+            prev = n;
+        }
+        prev = zzero<W>();
+        for (int x = nx - 1; x >= 0; --x) {
+            const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
+            const Z<W> m = zload<W>(rem + r);
+            const Z<W> c = zload<W>(comp + r);
+            const Z<W> n = zfill(zand(zor(c, prev), m), m);
+            if (!zeq(n, c)) {
+                zstore(comp + r, n);
+                changed = 1;
+            }
+            prev = n;
+        }
+    }
+    return changed;
+}
+
+// Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
+// (x, y in [68, 127]) with z along the bits.  Rounds alternate the two sweeps; z runs fill in
+// one step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is
+// the component and the order of the sweeps cannot change it.
 template <int W>
 __device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
-    const int t = threadIdx.x;
     for (;;) {
-        int changed = 0;
-        if (t < nx) {
-            unsigned long long* row = comp + static_cast<size_t>(t) * kRowStride * W;
-            const unsigned long long* mrow = rem + static_cast<size_t>(t) * kRowStride * W;
-            const unsigned long long* left = t > 0 ? row - kRowStride * W : nullptr;
-            const unsigned long long* right = t < nx - 1 ? row + kRowStride * W : nullptr;
-            Z<W> prev = zzero<W>();
-            for (int y = 0; y < kLevels; ++y) {
-                const Z<W> m = zload<W>(mrow + y * W);
-                const Z<W> c = zload<W>(row + y * W);
-                Z<W> nb = zor(c, prev);
-                if (left) nb = zor(nb, zload<W>(left + y * W));
-                if (right) nb = zor(nb, zload<W>(right + y * W));
-                const Z<W> n = zfill(zand(nb, m), m);
-                if (!zeq(n, c)) {
-                    zstore(row + y * W, n);
-                    changed = 1;
-                }
-                prev = n;
-            }
-            prev = zzero<W>();
-            for (int y = kLevels - 1; y >= 0; --y) {
-                const Z<W> m = zload<W>(mrow + y * W);
-                const Z<W> c = zload<W>(row + y * W);
-                const Z<W> n = zfill(zand(zor(c, prev), m), m);
-                if (!zeq(n, c)) {
-                    zstore(row + y * W, n);
-                    changed = 1;
-                }
-                prev = n;
-            }
-        }
+        int changed = sweep_y<W>(comp, rem, nx);
         __syncthreads();
-        if (t < kLevels) {
-            Z<W> prev = zzero<W>();
-            for (int x = 0; x < nx; ++x) {
-                const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
-                const Z<W> m = zload<W>(rem + r);
-                const Z<W> c = zload<W>(comp + r);
-                Z<W> nb = zor(c, prev);
-                if (t > 0) nb = zor(nb, zload<W>(comp + r - W));
-                if (t < kLevels - 1) nb = zor(nb, zload<W>(comp + r + W));
-                const Z<W> n = zfill(zand(nb, m), m);
-                if (!zeq(n, c)) {
-                    zstore(comp + r, n);
-                    changed = 1;
-                }
-                prev = n;
-            }
-            prev = zzero<W>();
-            for (int x = nx - 1; x >= 0; --x) {
-                const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
-                const Z<W> m = zload<W>(rem + r);
-                const Z<W> c = zload<W>(comp + r);
-                const Z<W> n = zfill(zand(zor(c, prev), m), m);
-                if (!zeq(n, c)) {
-                    zstore(comp + r, n);
-                    changed = 1;
-                }
-                prev = n;
-            }
-        }
+        changed |= sweep_x<W>(comp, rem, nx);
         if (!__syncthreads_or(changed)) break;
     }
+}
+
+// Number of blocks in rem and not in comp, over the whole window (block-wide sum).
+template <int W>
+__device__ int count_outside(const unsigned long long* rem, const unsigned long long* comp, int rows, int* s_total) {
+    if (threadIdx.x == 0) *s_total = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        const Z<W> m = zload<W>(rem + static_cast<size_t>(r) * W);
+        const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
+#pragma unroll
+        for (int i = 0; i < W; ++i) mine += __popcll(m.v[i] & ~c.v[i]);
+    }
+    atomicAdd(s_total, mine);
+    __syncthreads();
+    return *s_total;
 }
 
 template <int W, bool GLOBAL>
@@ -301,7 +358,9 @@ __global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) 
 
         // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
         for (int task = tid; task < nx * 8; task += blockDim.x) {
-            const int x = task >> 3, k1 = 8 + (task & 7);
+            // level-major: the lanes of a wave share one y-cell level, so the all-air levels above
+            // the terrain take the short path together
+            const int x = task % nx, k1 = 8 + task / nx;
             Z<W> r8[8];
             stone_cell_rows<W>(lat, x, k1, p.chunks_z, r8);
 #pragma unroll
@@ -323,22 +382,25 @@ __global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) 
             zstore(comp + static_cast<size_t>(r) * W, v);
         }
         __syncthreads();
-        grow_component<W>(comp, rem, nx);
-        if (tid == 0) s_total = 0;
+        // 2a. one sweep along y from the contacts already reaches most grounded stone; what is
+        //     left bounds the floating stone from above, and fewer than minBlocks floating
+        //     blocks cannot hold a match
+        sweep_y<W>(comp, rem, nx);
         __syncthreads();
-        int mine = 0;
+        if (count_outside<W>(rem, comp, rows, &s_total) < p.min_blocks) continue;
+        // 2b. the ground components in full
+        grow_component<W>(comp, rem, nx);
+        // 3. all remaining stone is floating
+        const int floating = count_outside<W>(rem, comp, rows, &s_total);
+        if (floating < p.min_blocks) continue;
         for (int r = tid; r < rows; r += blockDim.x) {
             Z<W> left = zload<W>(rem + static_cast<size_t>(r) * W);
             const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
 #pragma unroll
             for (int i = 0; i < W; ++i) left.v[i] &= ~c.v[i];
             zstore(rem + static_cast<size_t>(r) * W, left);
-            mine += zpop(left);
         }
-        atomicAdd(&s_total, mine);
         __syncthreads();
-        // 3. all remaining stone is floating; fewer blocks than minBlocks cannot hold a match
-        if (s_total < p.min_blocks) continue;
 
         bool have_best = false;
         CompStats best{};
