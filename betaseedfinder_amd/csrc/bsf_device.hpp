@@ -95,6 +95,39 @@ struct GenOffsets {
     double a, b, c;
 };
 
+// Two consecutive nextInt draws, bounds b0 then b1.  The two LCG states are formed
+// independently (s1 = A s + C, s2 = A^2 s + (A + 1) C, mod 2^48); if either draw would be
+// rejected (probability < 2^-23) the pair is redone one draw at a time, as Java does.
+constexpr uint64_t kLcgMul2 = (kLcgMul * kLcgMul) & kLcgMask;
+constexpr uint64_t kLcgAdd2 = (kLcgMul * kLcgAdd + kLcgAdd) & kLcgMask;
+
+__device__ __forceinline__ bool int_from_bits(uint32_t bits, uint32_t bound, DivMagic mg, uint32_t& val) {
+    if ((bound & (bound - 1)) == 0) {
+        val = bits >> mg.shift;
+        return true;
+    }
+    const uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
+    val = bits - q * bound;
+    return static_cast<int32_t>(bits - val + (bound - 1)) >= 0;
+}
+
+__device__ __forceinline__ void next_two_ints(Lcg& r, uint32_t b0, DivMagic m0, uint32_t b1, DivMagic m1,
+                                              uint32_t& v0, uint32_t& v1) {
+    const uint64_t s1 = (r.s * kLcgMul + kLcgAdd) & kLcgMask;
+    const uint64_t s2 = (r.s * kLcgMul2 + kLcgAdd2) & kLcgMask;
+    const bool ok0 = int_from_bits(static_cast<uint32_t>(s1 >> 17), b0, m0, v0);
+    const bool ok1 = int_from_bits(static_cast<uint32_t>(s2 >> 17), b1, m1, v1);
+    if (ok0 && ok1) {
+        r.s = s2;
+        return;
+    }
+    v0 = next_int_uniform_bound(r, b0, m0);
+    v1 = next_int_uniform_bound(r, b1, m1);
+}
+
+// The shuffle walks two steps (i, i + 1) per LDS round trip: the four bytes are read first
+// and the effect of swap i on the reads of swap i + 1 is applied in registers (j1 >= i + 1,
+// so swap i can only have moved p[i + 1] (when j0 == i + 1) or p[j1] (when j1 == j0)).
 __device__ __forceinline__ GenOffsets build_generator(Lcg& r, LanePerm p, const DivMagic* __restrict__ magic) {
     GenOffsets o;
     o.a = r.next_double() * 256.0;
@@ -104,16 +137,24 @@ __device__ __forceinline__ GenOffsets build_generator(Lcg& r, LanePerm p, const 
     uint32_t* w = reinterpret_cast<uint32_t*>(p.base);
 #pragma unroll 8
     for (int k = 0; k < 64; ++k) w[k * 64] = 0x03020100u + 0x04040404u * static_cast<uint32_t>(k);
-#pragma unroll 4
-    for (int i = 0; i < 256; ++i) {
-        const uint32_t bound = 256u - static_cast<uint32_t>(i);
-        const uint32_t j = next_int_uniform_bound(r, bound, magic[bound]) + static_cast<uint32_t>(i);
-        uint8_t* pi = p.base + LanePerm::off(static_cast<uint32_t>(i));
-        uint8_t* pj = p.base + LanePerm::off(j);
-        const uint8_t vi = *pi;
-        const uint8_t vj = *pj;
-        *pi = vj;
-        *pj = vi;
+#pragma unroll 2
+    for (int i = 0; i < 256; i += 2) {
+        const uint32_t b0 = 256u - static_cast<uint32_t>(i), b1 = b0 - 1u;
+        uint32_t j0, j1;
+        next_two_ints(r, b0, magic[b0], b1, magic[b1], j0, j1);
+        j0 += static_cast<uint32_t>(i);
+        j1 += static_cast<uint32_t>(i) + 1u;
+        uint8_t* pi0 = p.base + LanePerm::off(static_cast<uint32_t>(i));
+        uint8_t* pj0 = p.base + LanePerm::off(j0);
+        uint8_t* pi1 = p.base + LanePerm::off(static_cast<uint32_t>(i) + 1u);
+        uint8_t* pj1 = p.base + LanePerm::off(j1);
+        const uint8_t vi0 = *pi0, vj0 = *pj0, vi1 = *pi1, vj1 = *pj1;
+        const uint8_t a = j0 == static_cast<uint32_t>(i) + 1u ? vi0 : vi1;  // p[i + 1] after swap i
+        const uint8_t c = j1 == j0 ? vi0 : vj1;                              // p[j1] after swap i
+        *pi0 = vj0;
+        *pj0 = vi0;
+        *pi1 = c;
+        *pj1 = a;
     }
     return o;
 }
