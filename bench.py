@@ -202,8 +202,8 @@ def served_trace(bsf, seconds, rate, median_log2, seed=0xB5F00008):
         out.append((t, first, size, job_params(bsf, w, h, prof, float(rng.random()), float(rng.random()))))
 
 
-def run_trace(bsf, trace, fifo, horizon_s):
-    srv = bsf.Server(fifo=fifo)
+def run_trace(bsf, trace, fifo, horizon_s, pass_log2=17):
+    srv = bsf.Server(fifo=fifo, hit_buffer_bytes=(1 << pass_log2) * 128)
     ids, hits_seen = [], 0
     t0 = time.perf_counter()
     i = 0
@@ -270,12 +270,13 @@ def run_density_scout(bsf, args):
 
 def run_serving(bsf, args):
     trace = served_trace(bsf, args.serve_seconds, args.serve_rate, args.serve_median_log2)
-    served = run_trace(bsf, trace, False, args.serve_seconds + args.serve_drain)
-    plain = run_trace(bsf, trace, True, args.serve_seconds + args.serve_drain)
+    served = run_trace(bsf, trace, False, args.serve_seconds + args.serve_drain, args.serve_pass_log2)
+    plain = run_trace(bsf, trace, True, args.serve_seconds + args.serve_drain, args.serve_pass_log2)
     return {
         "workload": "Poisson arrivals %.0f jobs/s for %.0f s; size min(2^36, round(2^%d e^(2Z))); windows w x h "
                     "uniform in 1..8 chunks; 4 threshold profiles" % (args.serve_rate, args.serve_seconds,
                                                                       args.serve_median_log2),
+        "seeds_per_pass": 1 << args.serve_pass_log2,
         "served": served,
         "plain_fifo": plain,
     }
@@ -301,6 +302,7 @@ def main():
     ap.add_argument("--serve-drain", type=float, default=45.0)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-median-log2", type=int, default=13)
+    ap.add_argument("--serve-pass-log2", type=int, default=17)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (traffic-counter calibration under rocprofv3)")
     args = ap.parse_args()
