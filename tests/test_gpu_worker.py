@@ -84,6 +84,21 @@ def test_density_mode(bsf, oracle):
 
 
 @pytest.mark.gpu
+def test_island_mode_csv_matches_library(bsf):
+    first, count = 0, 4096
+    p = subprocess.run([WORKER, "island", str(first), str(count), "-2", "-2", "4", "4", "200"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    lines = p.stdout.decode().strip().splitlines()
+    assert lines[0].startswith("seed,chunk_x,chunk_z,min_x")
+    got = [tuple(int(v) for v in line.split(",")) for line in lines[1:]]
+    ref = bsf.island_range(first, count, bsf.island_params(-2, -2, 4, 4, min_blocks=200))
+    assert got == [h[:13] + (h[13] & 1,) for h in ref]
+    assert len(got) > 0
+    assert "Island range closed | seeds=4096 | hits=%d" % len(got) in p.stderr.decode()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("payload,message", [
     (struct.pack("<I", 65), "exceeds configured world capacity"),
     (struct.pack("<I", 3) + struct.pack("<q", 1), "truncated Stage0 stream seed batch"),
