@@ -239,8 +239,9 @@ def run_trace(bsf, trace, fifo, horizon_s, pass_log2=17):
 
 
 def run_density_scout(bsf, args):
-    """The other two fixed workloads, GPU only: the 65,536-seed density grid of chunk (0,0)
-    (host arrays in and out) and the Stage-0 scout over 2^scout_log2 seeds of its range."""
+    """The other fixed workloads, GPU only: the 65,536-seed density grid of chunk (0,0) (host
+    arrays in and out), the Stage-0 scout over 2^scout_log2 seeds of its range, and the
+    worker's stage0stream batch body."""
     import numpy as np
     seeds = np.arange(65536, dtype=np.int64)
     bsf.density_grid(seeds[:1024])
@@ -257,6 +258,16 @@ def run_density_scout(bsf, args):
     digest = 0
     for v in surv.tolist():
         digest = (digest + splitmix64(v & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+    # stage0stream as the Java launcher runs it: 32768-world batches, 8 centres, research records
+    worlds = np.arange(start, start + 32768 * (args.stage0_batches + 1), dtype=np.int64)
+    bsf.stage0_records(worlds[:1024], "general", True, centers=8)
+    t = time.perf_counter()
+    totals = np.zeros(11, dtype=np.int64)
+    for b in range(1, args.stage0_batches + 1):
+        _, c = bsf.stage0_records(worlds[b * 32768:(b + 1) * 32768], "general", True, centers=8)
+        totals += np.array([getattr(c, n) for n, _ in c._fields_], dtype=np.int64)
+    s0_s = time.perf_counter() - t
+    n_worlds = 32768 * args.stage0_batches
     return {
         "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
                          "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
@@ -265,6 +276,12 @@ def run_density_scout(bsf, args):
                               % (args.scout_log2, thr),
                   "seeds_per_s": round(n / scout_s, 1), "survivors": int(len(surv)),
                   "digest": "%016x" % digest},
+        "stage0stream": {"workload": "stage0stream batches of 32768 worlds x 8 centres from splitmix64(0xB5F00002) "
+                                     "& (2^48-1), general profile, 34-byte research records (host in/out)",
+                         "batches": args.stage0_batches, "worlds_per_s": round(n_worlds / s0_s, 1),
+                         "regions_per_s": round(8 * n_worlds / s0_s, 1), "ms_per_batch":
+                         round(s0_s * 1e3 / args.stage0_batches, 2),
+                         "counters": dict(zip([n for n, _ in bsf.Stage0Counters._fields_], totals.tolist()))},
     }
 
 
@@ -293,6 +310,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--scout-log2", type=int, default=30)
+    ap.add_argument("--stage0-batches", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-seeds", type=int, default=1 << 15)

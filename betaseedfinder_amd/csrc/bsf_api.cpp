@@ -96,6 +96,10 @@ void LaunchTimer::stop(uint64_t units) {
     active = false;
 }
 
+void credit_timer_units(const char* name, uint64_t units) {
+    if (g_timing) g_timer_stats[name].units += units;
+}
+
 void resolve_timers() {
     for (auto& p : g_pending) {
         check(hipEventSynchronize(p.b), "hipEventSynchronize");

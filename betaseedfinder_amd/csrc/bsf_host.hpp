@@ -71,6 +71,9 @@ struct LaunchTimer {
     void stop(uint64_t units);
 };
 void resolve_timers();
+// Adds units to a named timer whose launches were timed before the units were known (a stage
+// that reads its item count from device memory).
+void credit_timer_units(const char* name, uint64_t units);
 bool timing_enabled();
 
 void require_init();

@@ -1,14 +1,17 @@
 // bsf_stage0.hip -- Stage0 worker kernels (stage0stream / stream modes) for gfx950.
 //
-// Two parallel forms:
+// Parallel forms:
 //  * gen_cache_kernel: one world per lane builds the world's 76 used generators (66 terrain,
 //    10 climate) in Random order and writes the permutations and offsets to a per-world cache
 //    (the role of terrain_perlin_cache.hpp / climate_perlin_cache.hpp in the reference).
-//  * s0_column_kernel: one workgroup per (world, centre) item, one lattice column per lane.
+//  * column evaluation: one workgroup per (world, centre) item, one lattice column per lane.
 //    Per generator the 256-byte permutation is staged once in LDS and shared by all lanes,
 //    so every octave accumulator stays in registers.
-//  * coarse_score_kernel: one workgroup per item, bit-parallel component search over the
-//    61x61x17 coarse sign grid (one uint64 z-mask per (x, y) row).
+//  * the staged gates (P20 -> upper -> lower re-entry + P19 / mega -> coarse): one workgroup per
+//    item; each stage appends the items that pass its gate to a device list that the next
+//    stage walks, so a batch needs no host round trip until its records are assembled.
+//  * s0_coarse_kernel: one workgroup per item, the 61x61x17 coarse sign grid built in LDS
+//    and scored by bit-parallel component search (one uint64 z-mask per (x, y) row).
 #include "bsf_stage0.hpp"
 #include "bsf_cache.hpp"
 
@@ -17,6 +20,31 @@
 namespace bsf {
 
 __constant__ DivMagic s0_magic[257];
+
+// P19 tree table (16 trees): feature -1 marks a leaf whose value is the tree output.
+struct P19DevNode {
+    double value;
+    int feature, left, right;
+};
+constexpr int kP19MaxNodes = 1024;
+__constant__ P19DevNode s0_p19_nodes[kP19MaxNodes];
+__constant__ int s0_p19_root[16];
+
+namespace host_tables {
+#include "p19_trees.inc"
+}
+
+hipError_t upload_p19_tables() {
+    using namespace host_tables;
+    constexpr int n = static_cast<int>(sizeof(kP19Nodes) / sizeof(kP19Nodes[0]));
+    static_assert(n <= kP19MaxNodes, "P19 table too large");
+    P19DevNode nodes[n];
+    for (int i = 0; i < n; ++i) nodes[i] = P19DevNode{kP19Nodes[i].value, kP19Nodes[i].feature, kP19Nodes[i].left,
+                                                      kP19Nodes[i].right};
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(s0_p19_nodes), nodes, sizeof(nodes));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(s0_p19_root), kP19TreeRoot, sizeof(kP19TreeRoot));
+}
 
 hipError_t upload_stage0_tables() {
     DivMagic t[257] = {};
@@ -30,7 +58,9 @@ hipError_t upload_stage0_tables() {
             t[d].shift = k;
         }
     }
-    return hipMemcpyToSymbol(HIP_SYMBOL(s0_magic), t, sizeof(t));
+    const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(s0_magic), t, sizeof(t));
+    if (e != hipSuccess) return e;
+    return upload_p19_tables();
 }
 
 // ---------------------------------------------------------------------------
@@ -116,14 +146,131 @@ __device__ __forceinline__ int highest_reentry(uint32_t positive) {
     return highest;
 }
 
-// P20 / upper: Y 11..15 of the item's Stage0 columns (Y 16 is -10 after the taper).
+// ---------------------------------------------------------------------------
+// P19 monster gate (p19_monster_gate.hpp:48-79, gpu_p20_benchmark.cpp:1022-1128)
+// ---------------------------------------------------------------------------
+// Feature expressions of the P19 trees as the reference writes them (int sums and products,
+// ratios against (count + 1e-6)), all compared as doubles.
+__device__ double p19_feature(int kind, int scout, const P19Features& p) {
+    const double EPS = 1.0E-6;
+    switch (kind) {
+        case 0: return static_cast<double>(p.full_y88) / (scout + EPS);
+        case 1: return p.full_y88;
+        case 2: return p.full_y96;
+        case 3: return p.y88_width + p.y88_depth + p.y96_width + p.y96_depth;
+        case 4: return p.y96_width * p.y96_depth;
+        case 5: return scout;
+        case 6: return p.y88_width * p.y88_depth;
+        case 7: return static_cast<double>(p.y88_largest) / (p.full_y88 + EPS);
+        case 8: return p.y96_width;
+        case 9: return p.full_y96 + p.full_y104 + p.full_y112;
+        case 10: return p.y88_depth;
+        case 11: return p.y88_width;
+        case 12: return p.y96_depth;
+        case 13: return static_cast<double>(p.y96_largest) / (p.full_y96 + EPS);
+        case 14: return static_cast<double>(p.full_y104) / (p.full_y96 + EPS);
+        case 15: return static_cast<double>(p.full_y96) / (p.full_y88 + EPS);
+        case 16: return p.y96_largest;
+        case 17: return p.full_y112;
+        case 18: return p.y88_largest + p.y96_largest;
+        case 19: return p.full_y104;
+        case 20: return p.y88_largest;
+        default: return 0.0;
+    }
+}
+
+// Output of one tree: go left while feature <= threshold, until a leaf.
+__device__ double p19_tree(int tree, int scout, const P19Features& p) {
+    int node = s0_p19_root[tree];
+    while (s0_p19_nodes[node].feature >= 0) {
+        const P19DevNode& n = s0_p19_nodes[node];
+        node = p19_feature(n.feature, scout, p) <= n.value ? n.left : n.right;
+    }
+    return s0_p19_nodes[node].value;
+}
+
+// fillMonsterShapeStats (gpu_p20_benchmark.cpp:1022-1088) on a 16x16 board held as four
+// 64-bit words: word w holds rows x = 4w..4w+3, row x at bits 16 (x & 3) .. + 15, bit z.
+// Width and depth span every set cell; largest is the biggest 4-neighbour cluster, grown by
+// bit-parallel dilation.
+__device__ void board_shape(const unsigned long long* b, int& largest, int& width, int& depth, int& touches) {
+    constexpr unsigned long long kZ0 = 0x0001000100010001ULL, kZ15 = 0x8000800080008000ULL;
+    unsigned long long rem[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rem[i] = b[i];
+    if ((rem[0] | rem[1] | rem[2] | rem[3]) == 0) {
+        largest = width = depth = touches = 0;
+        return;
+    }
+    int minx = 16, maxx = -1;
+    unsigned zm = 0;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        const unsigned row = static_cast<unsigned>(rem[x >> 2] >> (16 * (x & 3))) & 0xFFFFu;
+        if (row) {
+            minx = minx < x ? minx : x;
+            maxx = x;
+            zm |= row;
+        }
+    }
+    const int minz = __ffs(zm) - 1, maxz = 31 - __clz(zm);
+    touches = (minx == 0 || maxx == 15 || minz == 0 || maxz == 15) ? 1 : 0;
+    width = maxx - minx + 1;
+    depth = maxz - minz + 1;
+    int big = 0;
+    for (;;) {
+        const int k = rem[0] ? 0 : rem[1] ? 1 : rem[2] ? 2 : rem[3] ? 3 : 4;
+        if (k == 4) break;
+        unsigned long long c[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = i == k ? rem[i] & (~rem[i] + 1ULL) : 0ULL;
+        for (;;) {
+            unsigned long long nb[4];
+            bool same = true;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                unsigned long long h = c[i] | ((c[i] << 1) & ~kZ0) | ((c[i] >> 1) & ~kZ15) | (c[i] << 16) | (c[i] >> 16);
+                if (i > 0) h |= c[i - 1] >> 48;
+                if (i < 3) h |= c[i + 1] << 48;
+                nb[i] = h & rem[i];
+                same = same && nb[i] == c[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = nb[i];
+            if (same) break;
+        }
+        const int size = __popcll(c[0]) + __popcll(c[1]) + __popcll(c[2]) + __popcll(c[3]);
+        big = big > size ? big : size;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rem[i] &= ~c[i];
+    }
+    largest = big;
+}
+
+__device__ __forceinline__ void append_item(int64_t* list, int* count, int64_t item) {
+    const int at = atomicAdd(count, 1);
+    list[at] = item;
+}
+
+// ---------------------------------------------------------------------------
+// staged Stage0 gates
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t stage_items(const S0Args& a) {
+    return a.n_dev != nullptr ? static_cast<int64_t>(*a.n_dev) : a.n_items;
+}
+
+// P20 / upper: Y 11..15 of the item's Stage0 columns (Y 16 is -10 after the taper).  P20 runs
+// over every item and, with a list, resets the item's later-stage outputs and appends the
+// items whose count reaches p20_min; upper runs over that list, adds the P20 count (the
+// gated P20 of applyP20MinimumGateKernel) and appends the items reaching upper_min.
 template <int MODE>
 __global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
     __shared__ GenStage st;
     __shared__ int s_count;
     const int tid = threadIdx.x;
     const int ncols = MODE == S0_P20 ? 64 : 192;
-    for (int64_t it = blockIdx.x; it < a.n_items; it += gridDim.x) {
+    const int64_t n = stage_items(a);
+    for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
         const int64_t item = a.items != nullptr ? a.items[it] : it;
         const int64_t world = item / a.centers;
         const int center = static_cast<int>(item % a.centers);
@@ -151,32 +298,53 @@ __global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
         }
         __syncthreads();
         if (tid == 0) {
-            if (MODE == S0_P20) a.p20[item] = s_count;
-            else a.upper[item] = a.p20_gated[item] + s_count;
+            if (MODE == S0_P20) {
+                a.p20[item] = s_count;
+                if (a.flags != nullptr) {
+                    a.upper[item] = 0;
+                    a.high[item] = 0;
+                    a.coarse[item] = 0;
+                    a.score[item] = __builtin_nan("");
+                    a.flags[item] = 0;
+                    a.feats[item] = P19Features{};
+                }
+                if (a.next_items != nullptr && s_count >= a.p20_min) append_item(a.next_items, a.next_count, item);
+            } else {
+                const int upper = a.p20[item] + s_count;
+                a.upper[item] = upper;
+                if (upper >= a.upper_min) append_item(a.next_items, a.next_count, item);
+            }
         }
     }
 }
 
-// Lower re-entry: Y 0..10 for the columns whose upper mask is non-zero (compacted).
+// Lower re-entry (Y 0..10 of the columns whose upper mask is non-zero) and, for the items
+// whose high count reaches high_min, the P19 features, score, extreme signal and mega
+// topology reject (gpu_p20_benchmark.cpp:7473-7489).  Items that pass and are not rejected
+// are appended for the coarse stage.  Needs blockDim.x == 256 (one thread per column).
 __global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
     __shared__ GenStage st;
     __shared__ int s_active[256];
-    __shared__ int s_nactive;
-    __shared__ int s_high;
+    __shared__ uint8_t s_highest[256];
+    __shared__ int s_nactive, s_high;
+    __shared__ unsigned long long s_b88[4], s_b96[4];
+    __shared__ int s_c104[4], s_c112[4];
+    __shared__ P19Features s_f;
+    __shared__ double s_leaf[16];
     const int tid = threadIdx.x;
-    for (int64_t it = blockIdx.x; it < a.n_items; it += gridDim.x) {
+    const int64_t n = stage_items(a);
+    for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
         const int64_t item = a.items[it];
         const int64_t world = item / a.centers;
         const int center = static_cast<int>(item % a.centers);
         const uint32_t* perms = a.perms + static_cast<size_t>(world) * kGenCount * 64;
         const double* offs = a.offsets + static_cast<size_t>(world) * kGenCount * 3;
-        // highest re-entry rows are indexed by list position; 0xFF = no re-entry / not a candidate
-        for (int c = tid; c < 256; c += blockDim.x) a.highest[static_cast<size_t>(it) * 256 + c] = 0xFF;
+        s_highest[tid] = 0xFF;  // no re-entry / not a candidate column
         if (tid == 0) {
-            int n = 0;
+            int cnt = 0;
             for (int c = 0; c < 256; ++c)
-                if (a.upper_mask[static_cast<size_t>(item) * 256 + c]) s_active[n++] = c;
-            s_nactive = n;
+                if (a.upper_mask[static_cast<size_t>(item) * 256 + c]) s_active[cnt++] = c;
+            s_nactive = cnt;
             s_high = 0;
         }
         __syncthreads();
@@ -198,27 +366,80 @@ __global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
             for (int y = 0; y < 11; ++y)
                 if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0) positive |= 1u << y;
             const int h = highest_reentry(positive);
-            a.highest[static_cast<size_t>(it) * 256 + col] = h < 0 ? 0xFF : static_cast<uint8_t>(h);
+            s_highest[col] = h < 0 ? 0xFF : static_cast<uint8_t>(h);
             if (h >= 11) atomicAdd(&s_high, 1);
         }
         __syncthreads();
-        if (tid == 0) a.high[item] = s_high;
+        const int high = s_high;
+        if (tid == 0) a.high[item] = high;
+        if (high >= a.high_min) {  // buildMonsterFeatures (:1090-1114): one ballot per wave
+            const int h = s_highest[tid];
+            const bool valid = h != 0xFF;
+            const unsigned long long m88 = __ballot(valid && h >= 11);
+            const unsigned long long m96 = __ballot(valid && h >= 12);
+            const unsigned long long m104 = __ballot(valid && h >= 13);
+            const unsigned long long m112 = __ballot(valid && h >= 14);
+            if ((tid & 63) == 0) {
+                s_b88[tid >> 6] = m88;
+                s_b96[tid >> 6] = m96;
+                s_c104[tid >> 6] = __popcll(m104);
+                s_c112[tid >> 6] = __popcll(m112);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                P19Features f;
+                f.full_y88 = __popcll(s_b88[0]) + __popcll(s_b88[1]) + __popcll(s_b88[2]) + __popcll(s_b88[3]);
+                f.full_y96 = __popcll(s_b96[0]) + __popcll(s_b96[1]) + __popcll(s_b96[2]) + __popcll(s_b96[3]);
+                f.full_y104 = s_c104[0] + s_c104[1] + s_c104[2] + s_c104[3];
+                f.full_y112 = s_c112[0] + s_c112[1] + s_c112[2] + s_c112[3];
+                board_shape(s_b88, f.y88_largest, f.y88_width, f.y88_depth, f.y88_touches);
+                board_shape(s_b96, f.y96_largest, f.y96_width, f.y96_depth, f.y96_touches);
+                s_f = f;
+            }
+            __syncthreads();
+            if (tid < 16) s_leaf[tid] = p19_tree(tid, a.upper[item], s_f);
+            __syncthreads();
+            if (tid == 0) {
+                double sum = 0.0;
+                for (int t = 0; t < 16; ++t) sum = sum + s_leaf[t];  // tree0 + ... + tree15
+                const double sc = sum / 16;
+                const P19Features& f = s_f;
+                const bool extreme = f.y88_largest >= 14 || f.y96_largest >= 14;  // :77-79
+                const bool pass = extreme || sc >= a.p19_min;
+                // megaTopologyReject, gpu_p20_benchmark.cpp:1116-1128
+                const bool mega = pass && a.mega_topology && !extreme && sc < 5.0 &&
+                                  (f.full_y112 == 0 || f.y96_largest < 4);
+                a.score[item] = sc;
+                a.feats[item] = f;
+                a.flags[item] = static_cast<uint8_t>((pass ? S0F_PASS : 0) | (extreme ? S0F_EXTREME : 0) |
+                                                     (mega ? S0F_MEGA : 0));
+                if (pass && !mega) append_item(a.next_items, a.next_count, item);
+            }
+        }
         __syncthreads();
     }
 }
 
-// Coarse 61x61x17 signs (coarse_exact_core.hpp:218-386): columns in batches of the block.
-__global__ __launch_bounds__(256) void s0_coarse_signs_kernel(S0Args a) {
+// Exact radius-7 coarse score of one item: the 61x61x17 sign grid (coarse_exact_core.hpp:
+// 218-386) built in LDS as one 61-bit z mask per (x, y) row, then scoreSigns (:83-169): the
+// largest 6-connected positive component with maxY >= 8 that touches neither the bottom layer
+// nor a side and has a positive cell below its lowest cell in some column.  Components are
+// grown by bit-parallel dilation.
+__global__ __launch_bounds__(256) void s0_coarse_kernel(S0Args a) {
     __shared__ GenStage st;
+    __shared__ unsigned long long rem[kCoarseRows];
+    __shared__ unsigned long long sig[kCoarseRows];
+    __shared__ unsigned long long comp[kCoarseRows];
+    __shared__ int s_changed, s_seed, s_cells, s_maxy, s_bottom, s_side, s_reentry;
     const int tid = threadIdx.x;
-    for (int64_t it = blockIdx.x; it < a.n_items; it += gridDim.x) {
+    const int64_t n = stage_items(a);
+    for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
         const int64_t item = a.items[it];
         const int64_t world = item / a.centers;
         const int center = static_cast<int>(item % a.centers);
         const uint32_t* perms = a.perms + static_cast<size_t>(world) * kGenCount * 64;
         const double* offs = a.offsets + static_cast<size_t>(world) * kGenCount * 3;
-        unsigned long long* bits = a.coarse_bits + static_cast<size_t>(it) * kCoarseRows;
-        for (int r = tid; r < kCoarseRows; r += blockDim.x) bits[r] = 0;
+        for (int r = tid; r < kCoarseRows; r += blockDim.x) rem[r] = 0;
         __syncthreads();
         for (int base = 0; base < 61 * 61; base += blockDim.x) {
             const int col = base + tid;
@@ -234,28 +455,11 @@ __global__ __launch_bounds__(256) void s0_coarse_signs_kernel(S0Args a) {
 #pragma unroll
                 for (int y = 0; y < 16; ++y)
                     if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0)
-                        atomicOr(&bits[cx * 17 + y], 1ULL << cz);
+                        atomicOr(&rem[cx * 17 + y], 1ULL << cz);
             }
         }
-    }
-}
-
-// scoreSigns, coarse_exact_core.hpp:83-169: the largest 6-connected positive component with
-// maxY >= 8 that touches neither the bottom layer nor a side and has a positive cell below
-// its lowest cell in some column.  Components are grown by bit-parallel dilation.
-__global__ __launch_bounds__(256) void s0_coarse_score_kernel(S0Args a) {
-    __shared__ unsigned long long rem[kCoarseRows];
-    __shared__ unsigned long long sig[kCoarseRows];
-    __shared__ unsigned long long comp[kCoarseRows];
-    __shared__ int s_changed, s_seed, s_cells, s_maxy, s_bottom, s_side, s_reentry;
-    const int tid = threadIdx.x;
-    const unsigned long long full = (1ULL << 61) - 1;
-    for (int64_t it = blockIdx.x; it < a.n_items; it += gridDim.x) {
-        const unsigned long long* bits = a.coarse_bits + static_cast<size_t>(it) * kCoarseRows;
-        for (int r = tid; r < kCoarseRows; r += blockDim.x) {
-            rem[r] = bits[r] & full;
-            sig[r] = rem[r];
-        }
+        __syncthreads();
+        for (int r = tid; r < kCoarseRows; r += blockDim.x) sig[r] = rem[r];
         __syncthreads();
         int best = 0;
         for (;;) {
@@ -335,9 +539,80 @@ __global__ __launch_bounds__(256) void s0_coarse_score_kernel(S0Args a) {
             if (s_cells > best && s_maxy >= 8 && !s_bottom && !s_side && s_reentry) best = s_cells;
             __syncthreads();
         }
-        if (tid == 0) a.coarse_score[it] = best;
+        if (tid == 0) a.coarse[item] = best;
         __syncthreads();
     }
+}
+
+// Records and closing-summary counters of a batch (gpu_p20_benchmark.cpp:7504-7571): one
+// thread per item.  Counts clamp to u8, the coarse score is written as its low 16 bits.
+__global__ __launch_bounds__(256) void s0_records_kernel(S0RecordArgs a) {
+    __shared__ unsigned long long s_c[9];
+    const int tid = threadIdx.x;
+    if (tid < 9) s_c[tid] = 0;
+    __syncthreads();
+    unsigned long long c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t step = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + tid; i < a.n_items; i += step) {
+        const int pa = a.p20[i], pb = a.upper[i], pc = a.high[i], pd = a.coarse[i];
+        const int fl = a.flags[i];
+        uint8_t* r = a.out + static_cast<size_t>(i) * a.record_bytes;
+        auto u8 = [](int v) { return static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v)); };
+        auto u16 = [&](int at, int v) {
+            const unsigned u = static_cast<unsigned>(v < 0 ? 0 : (v > 65535 ? 65535 : v));
+            r[at] = static_cast<uint8_t>(u & 0xFFu);
+            r[at + 1] = static_cast<uint8_t>(u >> 8);
+        };
+        r[0] = u8(pa);
+        r[1] = u8(pb);
+        r[2] = u8(pc);
+        r[3] = (fl & S0F_PASS) ? 1 : 0;
+        r[4] = static_cast<uint8_t>(pd & 0xFF);
+        r[5] = static_cast<uint8_t>((pd >> 8) & 0xFF);
+        r[6] = (fl & S0F_EXTREME) ? 1 : 0;
+        r[7] = (fl & S0F_MEGA) ? 1 : 0;
+        const unsigned long long sb = static_cast<unsigned long long>(__double_as_longlong(a.score[i]));
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[8 + k] = static_cast<uint8_t>(sb >> (8 * k));
+        if (a.record_bytes == 34) {
+            const P19Features& f = a.feats[i];
+            u16(16, f.full_y88);
+            u16(18, f.full_y96);
+            u16(20, f.full_y104);
+            u16(22, f.full_y112);
+            u16(24, f.y88_largest);
+            r[26] = static_cast<uint8_t>(f.y88_width);
+            r[27] = static_cast<uint8_t>(f.y88_depth);
+            r[28] = f.y88_touches ? 1 : 0;
+            u16(29, f.y96_largest);
+            r[31] = static_cast<uint8_t>(f.y96_width);
+            r[32] = static_cast<uint8_t>(f.y96_depth);
+            r[33] = f.y96_touches ? 1 : 0;
+        }
+        const bool g1 = pa >= a.p20_min, g2 = g1 && pb >= a.upper_min, g3 = g2 && pc >= a.high_min;
+        c[0] += g1;
+        c[1] += g2;
+        if (g3) {
+            ++c[2];
+            ++c[3];
+            if (fl & S0F_PASS) {
+                ++c[4];
+                if (fl & S0F_MEGA) {
+                    ++c[6];
+                } else {
+                    ++c[7];
+                    if (pd >= a.coarse_min) ++c[8];
+                }
+            } else {
+                ++c[5];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        if (c[k]) atomicAdd(&s_c[k], c[k]);
+    __syncthreads();
+    if (tid < 9 && s_c[tid]) atomicAdd(&a.counters[tid], s_c[tid]);
 }
 
 template __global__ void s0_upper_kernel<S0_P20>(S0Args);

@@ -21,32 +21,65 @@ struct GenCacheArgs {
     int64_t first_seed;
 };
 
+// The twelve P19 topology features of one item (buildMonsterFeatures,
+// gpu_p20_benchmark.cpp:1090-1114), in record order.
+struct P19Features {
+    int full_y88, full_y96, full_y104, full_y112;
+    int y88_largest, y88_width, y88_depth, y88_touches;
+    int y96_largest, y96_width, y96_depth, y96_touches;
+};
+
+// Item flags of the P19 / mega stage.
+enum { S0F_PASS = 1, S0F_EXTREME = 2, S0F_MEGA = 4 };
+
 struct S0Args {
     const uint32_t* perms;
     const double* offsets;
     int centers;                 // 1 or 8
     int center_dx[8], center_dz[8];  // chunk offsets of the centres
-    const int64_t* items;        // item = world * centers + center (nullptr: 0..n_items-1)
-    int64_t n_items;
+    const int64_t* items;        // item = world * centers + center (nullptr: 0..n-1)
+    int64_t n_items;             // item count, or its bound when n_dev is set
+    const int* n_dev;            // item count written by the previous stage (nullptr: n_items)
+    int64_t* next_items;         // items that pass this stage's gate are appended here
+    int* next_count;             //   (nullptr: no list is built)
+    int p20_min, upper_min, high_min, mega_topology;  // profile gates
+    double p19_min;
     uint8_t* upper_mask;         // [item][256]
     double* d5;                  // [item][256]
     double* d7;                  // [item][256]
     int* p20;                    // [item]
-    const int* p20_gated;        // [item]
     int* upper;                  // [item]
     int* high;                   // [item]
-    uint8_t* highest;            // [list index][256]
-    unsigned long long* coarse_bits;  // [list index][kCoarseRows]
-    int* coarse_score;           // [list index]
+    int* coarse;                 // [item]
+    double* score;               // [item] P19 score (quiet NaN when not evaluated)
+    uint8_t* flags;              // [item] S0F_*
+    P19Features* feats;          // [item]
+};
+
+// Record assembly of a batch: item = world * centers + centre.
+struct S0RecordArgs {
+    int64_t n_items;
+    int record_bytes;            // 16 (lean) or 34 (research)
+    int p20_min, upper_min, high_min, coarse_min;
+    const int* p20;
+    const int* upper;
+    const int* high;
+    const int* coarse;
+    const double* score;
+    const uint8_t* flags;
+    const P19Features* feats;
+    uint8_t* out;
+    unsigned long long* counters;  // [9]: p20 .. coarse_passed of the closing summary
 };
 
 __global__ void gen_cache_kernel(GenCacheArgs a);
 template <int MODE>
 __global__ void s0_upper_kernel(S0Args a);
 __global__ void s0_lower_kernel(S0Args a);
-__global__ void s0_coarse_signs_kernel(S0Args a);
-__global__ void s0_coarse_score_kernel(S0Args a);
+__global__ void s0_coarse_kernel(S0Args a);
+__global__ void s0_records_kernel(S0RecordArgs a);
 
+// nextInt magic numbers and the P19 tree table.
 hipError_t upload_stage0_tables();
 
 }  // namespace bsf

@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -1302,6 +1303,103 @@ int bsfo_mega_topology_reject(double score, int extreme, const bsfo_p19_features
 }
 
 int bsfo_coarse_score_signs(const uint8_t* signs) { return coarseScoreSigns(signs); }
+
+// HuntProfileConfig (gpu_p20_benchmark.cpp:7270-7290) and the production centre chunk offsets
+// (:7316-7317).
+struct Stage0Profile {
+    int p20_min, upper_min, high_min;
+    double p19_min;
+    int coarse_min, mega;
+};
+static const Stage0Profile kStage0Profiles[4] = {
+    {1, 5, 5, 4.02, 85, 0}, {1, 8, 6, 4.02, 85, 1}, {3, 19, 12, 6.70, 95, 0}, {3, 19, 20, 6.70, 700, 0}};
+static const int kStage0CenterX[8] = {0, 15, 0, 15, -15, -15, 0, 15};
+static const int kStage0CenterZ[8] = {0, 0, 15, 15, 0, 15, -15, -15};
+
+void bsfo_stage0_records(const int64_t* seeds, int64_t count, int32_t profile, int32_t research, int32_t centers,
+                         int threads, uint8_t* out, uint64_t* counters) {
+    const Stage0Profile& pr = kStage0Profiles[profile];
+    const size_t rb = research ? 34 : 16;
+    std::atomic<uint64_t> sums[11];
+    for (auto& v : sums) v = 0;
+    parallelFor(count * centers, threads, [&](int64_t lo, int64_t hi) {
+        uint64_t c[11] = {0};
+        for (int64_t item = lo; item < hi; ++item) {
+            const int64_t seed = seeds[item / centers];
+            const int ci = static_cast<int>(item % centers);
+            const int ox = kStage0CenterX[ci] * 4, oz = kStage0CenterZ[ci] * 4;
+            bsfo_stage0_result r;
+            bsfo_stage0(seed, ox, oz, pr.p20_min, pr.upper_min, &r);
+            bsfo_p19_features f;
+            std::memset(&f, 0, sizeof(f));
+            double score = std::numeric_limits<double>::quiet_NaN();
+            int pass = 0, extreme = 0, mega = 0, coarse = 0;
+            const bool evaluated = r.p20 >= pr.p20_min && r.upper >= pr.upper_min && r.high >= pr.high_min;
+            if (evaluated) {
+                bsfo_p19_features_from_highest(r.highest, &f);
+                score = bsfo_p19_score(r.upper, &f);
+                extreme = bsfo_p19_extreme(&f);
+                pass = extreme || score >= pr.p19_min;
+                mega = pass && pr.mega && bsfo_mega_topology_reject(score, extreme, &f);
+                if (pass && !mega) coarse = bsfo_coarse_score(seed, ox, oz, nullptr);
+            }
+            uint8_t* o = out + static_cast<size_t>(item) * rb;
+            auto u8 = [](int v) { return static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v)); };
+            auto u16 = [&](size_t at, int v) {
+                const unsigned u = static_cast<unsigned>(v < 0 ? 0 : (v > 65535 ? 65535 : v));
+                o[at] = static_cast<uint8_t>(u & 0xFF);
+                o[at + 1] = static_cast<uint8_t>(u >> 8);
+            };
+            o[0] = u8(r.p20);
+            o[1] = u8(r.upper);
+            o[2] = u8(r.high);
+            o[3] = static_cast<uint8_t>(pass);
+            o[4] = static_cast<uint8_t>(coarse & 0xFF);
+            o[5] = static_cast<uint8_t>((coarse >> 8) & 0xFF);
+            o[6] = static_cast<uint8_t>(extreme);
+            o[7] = static_cast<uint8_t>(mega);
+            uint64_t bits;
+            std::memcpy(&bits, &score, 8);
+            for (int k = 0; k < 8; ++k) o[8 + k] = static_cast<uint8_t>(bits >> (8 * k));
+            if (research) {
+                u16(16, f.full_y88);
+                u16(18, f.full_y96);
+                u16(20, f.full_y104);
+                u16(22, f.full_y112);
+                u16(24, f.y88_largest);
+                o[26] = static_cast<uint8_t>(f.y88_width);
+                o[27] = static_cast<uint8_t>(f.y88_depth);
+                o[28] = static_cast<uint8_t>(f.y88_touches ? 1 : 0);
+                u16(29, f.y96_largest);
+                o[31] = static_cast<uint8_t>(f.y96_width);
+                o[32] = static_cast<uint8_t>(f.y96_depth);
+                o[33] = static_cast<uint8_t>(f.y96_touches ? 1 : 0);
+            }
+            // closing-summary counters, gpu_p20_benchmark.cpp:7555-7571
+            if (r.p20 >= pr.p20_min) ++c[2];
+            if (r.p20 >= pr.p20_min && r.upper >= pr.upper_min) ++c[3];
+            if (evaluated) {
+                ++c[4];
+                ++c[5];
+                if (pass) {
+                    ++c[6];
+                    if (mega) {
+                        ++c[8];
+                    } else {
+                        ++c[9];
+                        if (coarse >= pr.coarse_min) ++c[10];
+                    }
+                } else {
+                    ++c[7];
+                }
+            }
+        }
+        for (int k = 2; k < 11; ++k) sums[k] += c[k];
+    });
+    sums[0] += static_cast<uint64_t>(count);
+    sums[1] += static_cast<uint64_t>(count * centers);
+    for (int k = 0; k < 11; ++k) counters[k] += sums[k].load();
+}
 
 void bsfo_coarse_arrays(int64_t seed, int32_t ox, int32_t oz, double* n1, double* n2, double* n3, double* n4,
                         double* n5, double* temp, double* rain, uint8_t* signs) {

@@ -136,6 +136,13 @@ int      bsfo_mega_topology_reject(double score, int extreme, const bsfo_p19_fea
 /* coarse 61x61x17 sign grid at a coarse offset and its component score */
 int      bsfo_coarse_score(int64_t seed, int32_t coarse_offset_x, int32_t coarse_offset_z, uint8_t* signs_out);
 int      bsfo_coarse_score_signs(const uint8_t* signs);
+/* One `stage0stream` batch body (streamStage0Filter, gpu_p20_benchmark.cpp:7411-7571): for
+ * count worlds x centers (1 or 8) items, record item = world * centers + centre of 16 (lean)
+ * or 34 (research) bytes, and the eleven closing-summary counters added into counters[11]
+ * (worlds, regions, p20, stage1, stage05, stage075 evaluated/passed/rejected, mega rejected,
+ * coarse evaluated/passed).  profile: 0 general, 1 mega, 2 record60, 3 record80. */
+void     bsfo_stage0_records(const int64_t* seeds, int64_t count, int32_t profile, int32_t research,
+                             int32_t centers, int threads, uint8_t* out, uint64_t* counters);
 /* the coarse grid's intermediate arrays: n1/n2/n3/signs [61*61*17], n4/n5/temp/rain [61*61] */
 void     bsfo_coarse_arrays(int64_t seed, int32_t coarse_offset_x, int32_t coarse_offset_z,
                             double* n1, double* n2, double* n3, double* n4, double* n5,

@@ -135,3 +135,19 @@ def p19_score(scout, f):
 
 def coarse_score(seed, ox=0, oz=0):
     return lib().bsfo_coarse_score(ctypes.c_int64(seed), ox, oz, None)
+
+
+STAGE0_PROFILES = {"general": 0, "mega": 1, "record60": 2, "record80": 3}
+
+
+def run_stage0_records(seeds, profile="general", research=True, centers=8, threads=8):
+    """Records of one `stage0stream` batch: array [worlds, centers, 34 or 16] of uint8 and the
+    eleven closing-summary counters."""
+    s = np.ascontiguousarray(seeds, dtype=np.int64)
+    rb = 34 if research else 16
+    out = np.zeros((len(s), centers, rb), dtype=np.uint8)
+    counters = np.zeros(11, dtype=np.uint64)
+    lib().bsfo_stage0_records(s.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(s)), STAGE0_PROFILES[profile],
+                              1 if research else 0, centers, threads, out.ctypes.data_as(ctypes.c_void_p),
+                              counters.ctypes.data_as(ctypes.c_void_p))
+    return out, counters

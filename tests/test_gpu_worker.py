@@ -127,8 +127,13 @@ def test_worker_rejects_bad_arguments():
 
 def test_worker_fails_loudly_without_gpu():
     """On a host without a usable GPU the worker refuses to start (no CPU fallback)."""
-    import torch
-    if torch.cuda.is_available():
+    import betaseedfinder_amd as bsf
+    try:
+        bsf.init()
+        gpu = True
+    except Exception:
+        gpu = False
+    if gpu:
         pytest.skip("a GPU is present")
     if not os.path.exists(WORKER):
         pytest.skip("worker not built")

@@ -476,3 +476,40 @@ def test_perlin_value_bound_used_by_the_stage1_split(oracle):
                                   ctypes.c_double(z), 17, col)
             worst = max(worst, max(abs(c) for c in col))
     assert 0.5 < worst <= 2.0
+
+
+@pytest.mark.parametrize("profile,research,seeds,centers", [
+    ("general", True, list(range(24)), 8),
+    ("mega", False, list(range(100, 164)), 1),
+])
+def test_stage0_batch_driver_matches_stage_functions(oracle, profile, research, seeds, centers):
+    """The oracle's whole-batch driver (records + counters) against records assembled field by
+    field from its per-stage functions, which the reference fixtures pin."""
+    import struct
+    from stage0_expect import PROFILES, expected_record
+    recs, ctr = oracle.run_stage0_records(np.array(seeds, dtype=np.int64), profile, research, centers, threads=8)
+    p20_min, upper_min, high_min, _, coarse_min, _ = PROFILES[profile]
+    counts = [0] * 9
+    for w, seed in enumerate(seeds):
+        for c in range(centers):
+            exp = expected_record(oracle, seed, c, profile, research)
+            got = bytes(recs[w, c])
+            assert got == exp, (seed, c)
+            a, b, h = exp[0], exp[1], exp[2]  # the ranges here stay below the u8 clamp
+            coarse = exp[4] | (exp[5] << 8)
+            counts[0] += a >= p20_min
+            counts[1] += a >= p20_min and b >= upper_min
+            if a >= p20_min and b >= upper_min and h >= high_min:
+                counts[2] += 1
+                counts[3] += 1
+                if exp[3]:
+                    counts[4] += 1
+                    if exp[7]:
+                        counts[6] += 1
+                    else:
+                        counts[7] += 1
+                        counts[8] += coarse >= coarse_min
+                else:
+                    counts[5] += 1
+            assert not np.isnan(struct.unpack("<d", exp[8:16])[0]) or not exp[3]
+    assert ctr.tolist() == [len(seeds), len(seeds) * centers] + counts
