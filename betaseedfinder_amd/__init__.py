@@ -31,6 +31,10 @@ from .api import (  # noqa: F401
     lib,
     lib_path,
     p20_counts,
+    region_attempts,
+    RegionItem,
+    RegionResult,
     scout_range,
     stage0_records,
+    window_scan,
 )

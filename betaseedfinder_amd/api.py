@@ -165,6 +165,50 @@ def stage0_records(seeds, profile="general", research=True, centers=8):
 
 
 # ---------------------------------------------------------------------------
+# region scans
+# ---------------------------------------------------------------------------
+class RegionItem(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("center_chunk_x", ctypes.c_int32), ("center_chunk_z", ctypes.c_int32),
+                ("coarse_score", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class RegionResult(ctypes.Structure):
+    _fields_ = [("hit", IslandHit), ("found", ctypes.c_int32), ("radius_used", ctypes.c_int32),
+                ("normal_found", ctypes.c_int32), ("hot_scan_ran", ctypes.c_int32), ("hot_won", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+def window_scan(seeds, windows):
+    """Exact scan of windows[i] (IslandParams, 1..49 chunks per axis) for seeds[i]: a list with
+    the hit tuple of each window's best component, or None."""
+    init()
+    s = _seeds(seeds)
+    n = len(s)
+    if len(windows) != n:
+        raise ValueError("one window per seed")
+    ws = (IslandParams * max(1, n))(*windows)
+    hits = (IslandHit * max(1, n))()
+    found = np.zeros(max(1, n), dtype=np.int32)
+    _check(lib().bsf_window_scan(s.ctypes.data_as(ctypes.c_void_p), ws, ctypes.c_int64(n), hits,
+                                 found.ctypes.data_as(ctypes.c_void_p)))
+    return [hits[i].as_tuple() if found[i] else None for i in range(n)]
+
+
+def region_attempts(items, settings, radius=7):
+    """processGpuRegionAttempt for (seed, centre chunk x, centre chunk z, coarse score) items;
+    settings: IslandParams whose thresholds apply (window fields ignored).  Returns one dict per
+    item: hit (tuple or None), radius_used, normal_found, hot_scan_ran, hot_won."""
+    init()
+    n = len(items)
+    arr = (RegionItem * max(1, n))(*[RegionItem(int(sd), int(cx), int(cz), int(c), 0) for sd, cx, cz, c in items])
+    out = (RegionResult * max(1, n))()
+    _check(lib().bsf_region_attempts(arr, ctypes.c_int64(n), ctypes.byref(settings), ctypes.c_int32(radius), out))
+    return [{"hit": out[i].hit.as_tuple() if out[i].found else None, "radius_used": out[i].radius_used,
+             "normal_found": out[i].normal_found, "hot_scan_ran": out[i].hot_scan_ran, "hot_won": out[i].hot_won}
+            for i in range(n)]
+
+
+# ---------------------------------------------------------------------------
 # batched jobs, engine shape, autotuner
 # ---------------------------------------------------------------------------
 class IslandJob(ctypes.Structure):

@@ -1,15 +1,16 @@
-// bsf_flood.hip -- island flood of the search engine (gfx950): one 256-thread workgroup per
-// Stage-2 survivor, block bitsets of the window in LDS (or, for the largest windows, in a
-// per-workgroup global scratch that stays in L2).
+// bsf_flood.hip -- island flood (gfx950): one 256-thread workgroup per window, block bitsets
+// of the window in LDS (or, for the larger windows, in a per-workgroup global scratch that
+// stays in L2).  Used by the search engine (one window per Stage-2 survivor) and by the region
+// scanner (windows of up to 49 x 49 chunks).
 //
-// Bitsets hold one z-mask per (x, y) row for y in [68, 127]: W 64-bit words per row (W = 2
-// when the window is wider than 4 chunks in z).  The flood reproduces
-// IslandSearchEngine.findFloatingIslandInSeed's exact block scan (:2861-2977): candidate
-// starts in Java order, 6-neighbour components above y = 67 (floodFillFlat, :4096-4202),
-// isMatch (:3972-4003) and compareComponents (:4079-4094).
+// Bitsets hold one z-mask per (x, y) row for y in [68, 127]: W 64-bit words per row (z bit
+// 64 w + b in word w).  The flood reproduces IslandSearchEngine.findFloatingIslandInSeed's
+// exact block scan (:2861-2977): candidate starts in Java order, 6-neighbour components above
+// y = 67 (floodFillFlat, :4096-4202), isMatch (:3972-4003) and compareComponents (:4079-4094).
 #include <hip/hip_runtime.h>
 
 #include "bsf_island.hpp"
+#include "bsf_region.hpp"
 
 namespace bsf {
 
@@ -74,53 +75,48 @@ __device__ __forceinline__ Z<W> zzero() {
 }
 
 // Bits of m lying in a run of consecutive ones of m that contains a bit of s (s within m):
-// the carry of m + s sweeps each run upward from its seed bits; the bit-reversed sum sweeps
-// downward.  With two words the carry crosses from word 0 into word 1.
-__device__ __forceinline__ Z<1> zfill(const Z<1>& s, const Z<1>& m) {
-    const unsigned long long up = ((m.v[0] + s.v[0]) ^ m.v[0]) & m.v[0];
-    const unsigned long long rm = __builtin_bitreverse64(m.v[0]);
-    const unsigned long long rs = __builtin_bitreverse64(s.v[0]);
-    const unsigned long long down = __builtin_bitreverse64(((rm + rs) ^ rm) & rm);
-    Z<1> r;
-    r.v[0] = up | down | s.v[0];
-    return r;
-}
-__device__ __forceinline__ void add128(unsigned long long a0, unsigned long long a1, unsigned long long b0,
-                                       unsigned long long b1, unsigned long long& r0, unsigned long long& r1) {
-    r0 = a0 + b0;
-    r1 = a1 + b1 + (r0 < a0 ? 1ULL : 0ULL);
-}
-__device__ __forceinline__ Z<2> zfill(const Z<2>& s, const Z<2>& m) {
-    unsigned long long u0, u1;
-    add128(m.v[0], m.v[1], s.v[0], s.v[1], u0, u1);
-    // reversed 128-bit value: word 0 <- reverse(word 1), word 1 <- reverse(word 0)
-    const unsigned long long rm0 = __builtin_bitreverse64(m.v[1]), rm1 = __builtin_bitreverse64(m.v[0]);
-    const unsigned long long rs0 = __builtin_bitreverse64(s.v[1]), rs1 = __builtin_bitreverse64(s.v[0]);
-    unsigned long long d0, d1;
-    add128(rm0, rm1, rs0, rs1, d0, d1);
-    d0 = (d0 ^ rm0) & rm0;
-    d1 = (d1 ^ rm1) & rm1;
-    Z<2> r;
-    r.v[0] = (((u0 ^ m.v[0]) & m.v[0]) | __builtin_bitreverse64(d1) | s.v[0]);
-    r.v[1] = (((u1 ^ m.v[1]) & m.v[1]) | __builtin_bitreverse64(d0) | s.v[1]);
+// the carry of m + s sweeps each run upward from its seed bits; the same sum on the bit-
+// reversed words sweeps downward.  The W words form one (64 W)-bit value, so the carries
+// cross word boundaries.
+template <int W>
+__device__ __forceinline__ Z<W> zfill(const Z<W>& s, const Z<W>& m) {
+    Z<W> r;
+    unsigned long long carry = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const unsigned long long a = m.v[i], t = a + s.v[i];
+        const unsigned long long u = t + carry;
+        carry = (t < a ? 1ULL : 0ULL) | (u < t ? 1ULL : 0ULL);
+        r.v[i] = ((u ^ a) & a) | s.v[i];
+    }
+    carry = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {  // reversed word i is word W - 1 - i, bit-reversed
+        const unsigned long long a = __builtin_bitreverse64(m.v[W - 1 - i]);
+        const unsigned long long t = a + __builtin_bitreverse64(s.v[W - 1 - i]);
+        const unsigned long long u = t + carry;
+        carry = (t < a ? 1ULL : 0ULL) | (u < t ? 1ULL : 0ULL);
+        r.v[W - 1 - i] |= __builtin_bitreverse64((u ^ a) & a);
+    }
     return r;
 }
 
 // Lattice values are below 2^11 in magnitude, so the upsampler's rounding stays below 1e-12.
 constexpr double kSignMargin = 1e-9;
 
-// The eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks) from one lattice
-// y-cell, walking generateBareTerrain's incremental chain (BetaTerrain173.java:599-664) once:
-// d1..d4 step by d5..d8 per level, d10/d11 step i2 times by d12/d13 along x, d15 steps by
-// d16 along z.  The i2 steps are all formed and the right one selected, so lanes with
-// different x stay converged.
-template <int W>
-__device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, Z<W> rows[8]) {
+// Word wi of the eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks of chunks
+// 4 wi .. 4 wi + 3 in z) from one lattice y-cell, walking generateBareTerrain's incremental
+// chain (BetaTerrain173.java:599-664) once: d1..d4 step by d5..d8 per level, d10/d11 step i2
+// times by d12/d13 along x, d15 steps by d16 along z.  The i2 steps are all formed and the
+// right one selected, so lanes with different x stay converged.
+__device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, int wi,
+                                unsigned long long rows[8]) {
     const int cxi = x >> 4, lx = x & 15;
     const int i1 = lx >> 2, i2 = lx & 3;
 #pragma unroll
-    for (int l1 = 0; l1 < 8; ++l1) rows[l1] = zzero<W>();
-    for (int czi = 0; czi < cz_n; ++czi) {
+    for (int l1 = 0; l1 < 8; ++l1) rows[l1] = 0;
+    const int cz_end = 4 * wi + 4 < cz_n ? 4 * wi + 4 : cz_n;
+    for (int czi = 4 * wi; czi < cz_end; ++czi) {
         const double* cl = lat + static_cast<size_t>(cxi * cz_n + czi) * 25 * 8;
         for (int j1 = 0; j1 < 4; ++j1) {
             auto L = [&](int ii, int jj, int kk) -> double {
@@ -129,9 +125,7 @@ __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, i
             double d1 = L(i1, j1, k1), d2 = L(i1, j1 + 1, k1), d3 = L(i1 + 1, j1, k1), d4 = L(i1 + 1, j1 + 1, k1);
             const double e1 = L(i1, j1, k1 + 1), e2 = L(i1, j1 + 1, k1 + 1), e3 = L(i1 + 1, j1, k1 + 1),
                          e4 = L(i1 + 1, j1 + 1, k1 + 1);
-            const int shift = czi * 16 + j1 * 4;
-            const bool hi_word = W == 2 && shift >= 64;
-            const int sh = shift & 63;
+            const int sh = (czi - 4 * wi) * 16 + j1 * 4;
             // every block of the cell is a convex combination of its 8 corners (up to a few ulps),
             // so a cell whose corners are all clearly negative is air and all clearly positive
             // is stone
@@ -140,10 +134,7 @@ __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, i
             if (mx <= -kSignMargin) continue;
             if (mn > kSignMargin) {
 #pragma unroll
-                for (int l1 = 0; l1 < 8; ++l1) {
-                    if (hi_word) rows[l1].v[W - 1] |= 0xFULL << sh;
-                    else rows[l1].v[0] |= 0xFULL << sh;
-                }
+                for (int l1 = 0; l1 < 8; ++l1) rows[l1] |= 0xFULL << sh;
                 continue;
             }
             const double d5 = (e1 - d1) * 0.125;
@@ -166,8 +157,7 @@ __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, i
                     if (d15 > 0.0) bits |= 1ULL << k2;
                     d15 += d16;
                 }
-                if (hi_word) rows[l1].v[W - 1] |= bits << sh;
-                else rows[l1].v[0] |= bits << sh;
+                rows[l1] |= bits << sh;
                 d1 += d5;
                 d2 += d6;
                 d3 += d7;
@@ -211,18 +201,13 @@ __device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats&
     return areaA > areaB;
 }
 
-// Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
-// (x, y in [68, 127]) with z along the bits.  Rounds alternate two sweeps: thread x walks its
-// rows up and down in y, then thread y walks its rows up and down in x; z runs fill in one
-// step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is the
-// component and the order of the sweeps cannot change it.
-// One sweep of thread x along y over its x slice (up, then down): adds the blocks of rem that
-// touch comp within the slice or in the neighbouring slices' rows.  Returns whether it grew.
+// One sweep along y of each x slice (up, then down; thread t takes slices t, t + blockDim.x,
+// ...): adds the blocks of rem that touch comp within the slice or in the neighbouring
+// slices' rows.  Returns whether it grew.
 template <int W>
 __device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, int nx) {
-    const int t = threadIdx.x;
     int changed = 0;
-    if (t < nx) {
+    for (int t = threadIdx.x; t < nx; t += blockDim.x) {
         unsigned long long* row = comp + static_cast<size_t>(t) * kRowStride * W;
         const unsigned long long* mrow = rem + static_cast<size_t>(t) * kRowStride * W;
         const unsigned long long* left = t > 0 ? row - kRowStride * W : nullptr;
@@ -324,12 +309,216 @@ __device__ int count_outside(const unsigned long long* rem, const unsigned long 
     return *s_total;
 }
 
+// Shared state of one workgroup's flood.
+template <int W>
+struct FloodShared {
+    CompStats cs;
+    unsigned long long zor[W];
+    int total;
+};
+
+// The flood of one window: upsampling of its lattice (levels 8..15 of every density column,
+// [chunk x][chunk z][25][8] doubles) into the bitsets at base, ground removal, candidate
+// starts in Java order.  Returns (in every thread) whether a component matched, and the best
+// one in *best.  Every thread of the workgroup calls it.
+template <int W>
+__device__ bool flood_window(unsigned long long* const base, const double* __restrict__ lat,
+                             const bsf_island_params& p, FloodShared<W>& sh, CompStats* best_out) {
+    const int tid = threadIdx.x;
+    const int nx = p.chunks_x * 16;
+    const int nz = p.chunks_z * 16;
+    const int rows = nx * kRowStride;
+    unsigned long long* const rem = base;
+    unsigned long long* const comp = base + static_cast<size_t>(rows) * W;
+    unsigned long long* const st67 = base + static_cast<size_t>(2 * rows) * W;
+    const int minWZ = p.chunk_z0 * 16;
+
+    // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
+    for (int task = tid; task < nx * 8 * W; task += blockDim.x) {
+        // level-major: the lanes of a wave share one y-cell level, so the all-air levels above
+        // the terrain take the short path together
+        const int x = task % nx, rest = task / nx;
+        const int k1 = 8 + rest % 8, wi = rest / 8;
+        unsigned long long r8[8];
+        stone_cell_rows(lat, x, k1, p.chunks_z, wi, r8);
+#pragma unroll
+        for (int l1 = 0; l1 < 8; ++l1) {
+            const int y = k1 * 8 + l1;
+            if (y >= kFloorY) rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + wi] = r8[l1];
+            else if (y == kFloorY - 1) st67[static_cast<size_t>(x) * W + wi] = r8[l1];
+        }
+        if (k1 == 8) rem[(static_cast<size_t>(x) * kRowStride + kLevels) * W + wi] = 0;  // row padding
+    }
+    __syncthreads();
+    // 2. every component that reaches a y = 68 block with stone below touches the bottom
+    //    and can never match (isMatch): grow them all at once from their ground contacts
+    //    and drop them, as Java's flood would mark them visited.
+    for (int r = tid; r < rows; r += blockDim.x) {
+        const int x = r / kRowStride, y = r % kRowStride;
+        const Z<W> v = y == 0 ? zand(zload<W>(rem + static_cast<size_t>(r) * W), zload<W>(st67 + static_cast<size_t>(x) * W))
+                              : zzero<W>();
+        zstore(comp + static_cast<size_t>(r) * W, v);
+    }
+    __syncthreads();
+    // 2a. one sweep along y from the contacts already reaches most grounded stone; what is
+    //     left bounds the floating stone from above, and fewer than minBlocks floating
+    //     blocks cannot hold a match
+    sweep_y<W>(comp, rem, nx);
+    __syncthreads();
+    if (count_outside<W>(rem, comp, rows, &sh.total) < p.min_blocks) return false;
+    // 2b. the ground components in full
+    grow_component<W>(comp, rem, nx);
+    // 3. all remaining stone is floating
+    const int floating = count_outside<W>(rem, comp, rows, &sh.total);
+    if (floating < p.min_blocks) return false;
+    for (int r = tid; r < rows; r += blockDim.x) {
+        Z<W> left = zload<W>(rem + static_cast<size_t>(r) * W);
+        const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
+#pragma unroll
+        for (int i = 0; i < W; ++i) left.v[i] &= ~c.v[i];
+        zstore(rem + static_cast<size_t>(r) * W, left);
+    }
+    __syncthreads();
+
+    bool have_best = false;
+    CompStats best{};
+    // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
+    for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
+        const int cxi = ch / p.chunks_z, czi = ch % p.chunks_z;
+        for (int sc = 0; sc < 4; ++sc) {
+            const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
+            const int z = czi * 16 + ((sc & 1) ? 12 : 4);
+            const int zw = z >> 6;
+            const unsigned long long bitz = 1ULL << (z & 63);
+            for (int y = kFloorY; y < 128; ++y) {
+                // stone not yet visited (visited or grounded blocks are no longer in rem)
+                const unsigned long long here = rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw];
+                if ((here & bitz) == 0) continue;
+                // the block below is not stone: below y = 68 read the y = 67 row; above, a
+                // stone block below would share this block's component, so it is still in rem
+                const unsigned long long below =
+                        y == kFloorY ? st67[static_cast<size_t>(x) * W + zw]
+                                     : rem[(static_cast<size_t>(x) * kRowStride + (y - 1 - kFloorY)) * W + zw];
+                if ((below & bitz) != 0) continue;
+                for (int r = tid; r < rows * W; r += blockDim.x) comp[r] = 0;
+                __syncthreads();
+                if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw] = bitz;
+                __syncthreads();
+                grow_component<W>(comp, rem, nx);
+                // statistics of the component
+                if (tid == 0) {
+                    sh.cs.blocks = 0;
+                    sh.cs.columns = 0;
+                    sh.cs.minx = 0x7fffffff;
+                    sh.cs.maxx = -0x7fffffff;
+                    sh.cs.miny = 0x7fffffff;
+                    sh.cs.maxy = -0x7fffffff;
+                    sh.cs.minz = 0;
+                    sh.cs.maxz = 0;
+                    sh.cs.sumx = 0;
+                    sh.cs.sumz = 0;
+                    sh.cs.bottom = 0;
+                    sh.cs.side = 0;
+#pragma unroll
+                    for (int i = 0; i < W; ++i) sh.zor[i] = 0;
+                }
+                __syncthreads();
+                for (int rx = tid; rx < nx; rx += blockDim.x) {
+                    Z<W> col = zzero<W>();
+                    int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
+                    for (int ry = 0; ry < kLevels; ++ry) {
+                        const Z<W> c = zload<W>(comp + (static_cast<size_t>(rx) * kRowStride + ry) * W);
+                        if (zany(c)) {
+                            col = zor(col, c);
+                            blk += zpop(c);
+                            ymin = min(ymin, ry + kFloorY);
+                            ymax = max(ymax, ry + kFloorY);
+                        }
+                    }
+                    if (zany(col)) {
+                        const int ncol = zpop(col);
+                        long long zsum = static_cast<long long>(ncol) * minWZ;
+#pragma unroll
+                        for (int i = 0; i < W; ++i)
+                            zsum += sum_of_bit_positions(col.v[i]) + 64LL * i * __popcll(col.v[i]);
+                        atomicAdd(&sh.cs.blocks, blk);
+                        atomicAdd(&sh.cs.columns, ncol);
+                        atomicMin(&sh.cs.minx, rx);
+                        atomicMax(&sh.cs.maxx, rx);
+                        atomicMin(&sh.cs.miny, ymin);
+                        atomicMax(&sh.cs.maxy, ymax);
+                        atomicAdd(reinterpret_cast<unsigned long long*>(&sh.cs.sumx),
+                                  static_cast<unsigned long long>(static_cast<long long>(ncol) * (p.chunk_x0 * 16 + rx)));
+                        atomicAdd(reinterpret_cast<unsigned long long*>(&sh.cs.sumz), static_cast<unsigned long long>(zsum));
+#pragma unroll
+                        for (int i = 0; i < W; ++i)
+                            if (col.v[i]) atomicOr(&sh.zor[i], col.v[i]);
+                        const Z<W> c0 = zload<W>(comp + static_cast<size_t>(rx) * kRowStride * W);
+                        if (zany(zand(c0, zload<W>(st67 + static_cast<size_t>(rx) * W)))) atomicOr(&sh.cs.bottom, 1);
+                        if (rx == 0 || rx == nx - 1) atomicOr(&sh.cs.side, 1);
+                    }
+                }
+                for (int r = tid; r < rows * W; r += blockDim.x) rem[r] &= ~comp[r];
+                __syncthreads();
+                CompStats cs = sh.cs;
+                int lo = -1, hi = -1;  // lowest and highest z of the component
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const unsigned long long zo = sh.zor[i];
+                    if (zo != 0) {
+                        if (lo < 0) lo = 64 * i + __ffsll(static_cast<long long>(zo)) - 1;
+                        hi = 64 * i + 63 - __clzll(static_cast<long long>(zo));
+                    }
+                }
+                cs.minz = lo;
+                cs.maxz = hi;
+                if (cs.minz == 0 || cs.maxz == nz - 1) cs.side = 1;
+                // isMatch, IslandSearchEngine.java:3972-4003
+                const int width = cs.maxx - cs.minx + 1;
+                const int depth = cs.maxz - cs.minz + 1;
+                const bool match = cs.blocks >= p.min_blocks && cs.columns >= p.min_columns &&
+                                   cs.maxy >= p.min_y_for_match && !cs.bottom && width >= p.min_width &&
+                                   depth >= p.min_depth &&
+                                   cs.blocks / static_cast<double>(cs.columns > 1 ? cs.columns : 1) >=
+                                           p.min_avg_thickness;
+                if (match && (!have_best || comp_better(cs, best))) {
+                    best = cs;
+                    have_best = true;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    *best_out = best;
+    return have_best;
+}
+
+// The hit record of a window's best component.
+__device__ bsf_island_hit make_hit(int64_t seed, const bsf_island_params& p, const CompStats& best, int32_t tag) {
+    const int minWX = p.chunk_x0 * 16, minWZ = p.chunk_z0 * 16;
+    bsf_island_hit h;
+    h.seed = seed;
+    h.min_x = minWX + best.minx;
+    h.max_x = minWX + best.maxx;
+    h.min_z = minWZ + best.minz;
+    h.max_z = minWZ + best.maxz;
+    h.min_y = best.miny;
+    h.max_y = best.maxy;
+    h.blocks = best.blocks;
+    h.columns = best.columns;
+    h.center_x = java_round(static_cast<double>(best.sumx) / best.columns);
+    h.center_z = java_round(static_cast<double>(best.sumz) / best.columns);
+    h.chunk_x = h.center_x >> 4;
+    h.chunk_z = h.center_z >> 4;
+    h.flags = best.side ? 1 : 0;
+    h.window_index = tag;
+    return h;
+}
+
 template <int W, bool GLOBAL>
 __global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) {
     extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
-    __shared__ CompStats s_cs;
-    __shared__ unsigned long long s_zor[W];
-    __shared__ int s_total;
+    __shared__ FloodShared<W> sh;
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
     if (a.stage1_only) return;
@@ -347,187 +536,51 @@ __global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) 
         const int64_t sv = list[item];
         const int32_t jix = a.surv_job[sv];
         const bsf_island_params p = a.jobs[jix].p;
-        const int nx = p.chunks_x * 16;
-        const int nz = p.chunks_z * 16;
-        const int rows = nx * kRowStride;
-        unsigned long long* const rem = base;
-        unsigned long long* const comp = base + static_cast<size_t>(rows) * W;
-        unsigned long long* const st67 = base + static_cast<size_t>(2 * rows) * W;
-        const int minWX = p.chunk_x0 * 16, minWZ = p.chunk_z0 * 16;
-        const double* lat = a.lattice + static_cast<size_t>(sv) * a.lat_stride;
-
-        // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
-        for (int task = tid; task < nx * 8; task += blockDim.x) {
-            // level-major: the lanes of a wave share one y-cell level, so the all-air levels above
-            // the terrain take the short path together
-            const int x = task % nx, k1 = 8 + task / nx;
-            Z<W> r8[8];
-            stone_cell_rows<W>(lat, x, k1, p.chunks_z, r8);
-#pragma unroll
-            for (int l1 = 0; l1 < 8; ++l1) {
-                const int y = k1 * 8 + l1;
-                if (y >= kFloorY) zstore(rem + (static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W, r8[l1]);
-                else if (y == kFloorY - 1) zstore(st67 + static_cast<size_t>(x) * W, r8[l1]);
-            }
-            if (k1 == 8) zstore(rem + (static_cast<size_t>(x) * kRowStride + kLevels) * W, zzero<W>());  // row padding
-        }
-        __syncthreads();
-        // 2. every component that reaches a y = 68 block with stone below touches the bottom
-        //    and can never match (isMatch): grow them all at once from their ground contacts
-        //    and drop them, as Java's flood would mark them visited.
-        for (int r = tid; r < rows; r += blockDim.x) {
-            const int x = r / kRowStride, y = r % kRowStride;
-            const Z<W> v = y == 0 ? zand(zload<W>(rem + static_cast<size_t>(r) * W), zload<W>(st67 + static_cast<size_t>(x) * W))
-                                  : zzero<W>();
-            zstore(comp + static_cast<size_t>(r) * W, v);
-        }
-        __syncthreads();
-        // 2a. one sweep along y from the contacts already reaches most grounded stone; what is
-        //     left bounds the floating stone from above, and fewer than minBlocks floating
-        //     blocks cannot hold a match
-        sweep_y<W>(comp, rem, nx);
-        __syncthreads();
-        if (count_outside<W>(rem, comp, rows, &s_total) < p.min_blocks) continue;
-        // 2b. the ground components in full
-        grow_component<W>(comp, rem, nx);
-        // 3. all remaining stone is floating
-        const int floating = count_outside<W>(rem, comp, rows, &s_total);
-        if (floating < p.min_blocks) continue;
-        for (int r = tid; r < rows; r += blockDim.x) {
-            Z<W> left = zload<W>(rem + static_cast<size_t>(r) * W);
-            const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
-#pragma unroll
-            for (int i = 0; i < W; ++i) left.v[i] &= ~c.v[i];
-            zstore(rem + static_cast<size_t>(r) * W, left);
-        }
-        __syncthreads();
-
-        bool have_best = false;
-        CompStats best{};
-        // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
-        for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
-            const int cxi = ch / p.chunks_z, czi = ch % p.chunks_z;
-            for (int sc = 0; sc < 4; ++sc) {
-                const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
-                const int z = czi * 16 + ((sc & 1) ? 12 : 4);
-                const int zw = W == 2 ? (z >> 6) : 0;
-                const unsigned long long bitz = 1ULL << (z & 63);
-                for (int y = kFloorY; y < 128; ++y) {
-                    // stone not yet visited (visited or grounded blocks are no longer in rem)
-                    const unsigned long long here = rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw];
-                    if ((here & bitz) == 0) continue;
-                    // the block below is not stone: below y = 68 read the y = 67 row; above, a
-                    // stone block below would share this block's component, so it is still in rem
-                    const unsigned long long below =
-                            y == kFloorY ? st67[static_cast<size_t>(x) * W + zw]
-                                         : rem[(static_cast<size_t>(x) * kRowStride + (y - 1 - kFloorY)) * W + zw];
-                    if ((below & bitz) != 0) continue;
-                    for (int r = tid; r < rows * W; r += blockDim.x) comp[r] = 0;
-                    __syncthreads();
-                    if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw] = bitz;
-                    __syncthreads();
-                    grow_component<W>(comp, rem, nx);
-                    // statistics of the component
-                    if (tid == 0) {
-                        s_cs.blocks = 0;
-                        s_cs.columns = 0;
-                        s_cs.minx = 0x7fffffff;
-                        s_cs.maxx = -0x7fffffff;
-                        s_cs.miny = 0x7fffffff;
-                        s_cs.maxy = -0x7fffffff;
-                        s_cs.minz = 0;
-                        s_cs.maxz = 0;
-                        s_cs.sumx = 0;
-                        s_cs.sumz = 0;
-                        s_cs.bottom = 0;
-                        s_cs.side = 0;
-#pragma unroll
-                        for (int i = 0; i < W; ++i) s_zor[i] = 0;
-                    }
-                    __syncthreads();
-                    for (int rx = tid; rx < nx; rx += blockDim.x) {
-                        Z<W> col = zzero<W>();
-                        int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
-                        for (int ry = 0; ry < kLevels; ++ry) {
-                            const Z<W> c = zload<W>(comp + (static_cast<size_t>(rx) * kRowStride + ry) * W);
-                            if (zany(c)) {
-                                col = zor(col, c);
-                                blk += zpop(c);
-                                ymin = min(ymin, ry + kFloorY);
-                                ymax = max(ymax, ry + kFloorY);
-                            }
-                        }
-                        if (zany(col)) {
-                            const int ncol = zpop(col);
-                            long long zsum = static_cast<long long>(ncol) * minWZ + sum_of_bit_positions(col.v[0]);
-                            if (W == 2) zsum += sum_of_bit_positions(col.v[W - 1]) + 64LL * __popcll(col.v[W - 1]);
-                            atomicAdd(&s_cs.blocks, blk);
-                            atomicAdd(&s_cs.columns, ncol);
-                            atomicMin(&s_cs.minx, rx);
-                            atomicMax(&s_cs.maxx, rx);
-                            atomicMin(&s_cs.miny, ymin);
-                            atomicMax(&s_cs.maxy, ymax);
-                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumx),
-                                      static_cast<unsigned long long>(static_cast<long long>(ncol) * (minWX + rx)));
-                            atomicAdd(reinterpret_cast<unsigned long long*>(&s_cs.sumz), static_cast<unsigned long long>(zsum));
-#pragma unroll
-                            for (int i = 0; i < W; ++i) atomicOr(&s_zor[i], col.v[i]);
-                            const Z<W> c0 = zload<W>(comp + static_cast<size_t>(rx) * kRowStride * W);
-                            if (zany(zand(c0, zload<W>(st67 + static_cast<size_t>(rx) * W)))) atomicOr(&s_cs.bottom, 1);
-                            if (rx == 0 || rx == nx - 1) atomicOr(&s_cs.side, 1);
-                        }
-                    }
-                    for (int r = tid; r < rows * W; r += blockDim.x) rem[r] &= ~comp[r];
-                    __syncthreads();
-                    CompStats cs = s_cs;
-                    const unsigned long long z0 = s_zor[0], z1 = s_zor[W - 1];
-                    if (W == 1 || z0 != 0) cs.minz = __ffsll(static_cast<long long>(z0)) - 1;
-                    else cs.minz = 64 + __ffsll(static_cast<long long>(z1)) - 1;
-                    if (W == 2 && z1 != 0) cs.maxz = 64 + 63 - __clzll(static_cast<long long>(z1));
-                    else cs.maxz = 63 - __clzll(static_cast<long long>(z0));
-                    if (cs.minz == 0 || cs.maxz == nz - 1) cs.side = 1;
-                    // isMatch, IslandSearchEngine.java:3972-4003
-                    const int width = cs.maxx - cs.minx + 1;
-                    const int depth = cs.maxz - cs.minz + 1;
-                    const bool match = cs.blocks >= p.min_blocks && cs.columns >= p.min_columns &&
-                                       cs.maxy >= p.min_y_for_match && !cs.bottom && width >= p.min_width &&
-                                       depth >= p.min_depth &&
-                                       cs.blocks / static_cast<double>(cs.columns > 1 ? cs.columns : 1) >=
-                                               p.min_avg_thickness;
-                    if (match && (!have_best || comp_better(cs, best))) {
-                        best = cs;
-                        have_best = true;
-                    }
-                    __syncthreads();
-                }
-            }
-        }
-        if (tid == 0 && have_best) {
+        CompStats best;
+        const bool found = flood_window<W>(base, a.lattice + static_cast<size_t>(sv) * a.lat_stride, p, sh, &best);
+        if (tid == 0 && found) {
             const unsigned long long slot = atomicAdd(&a.ctr[kCtrHits], 1ULL);
             atomicAdd(&a.jobs[jix].hits, 1ULL);
-            if (static_cast<int64_t>(slot) < a.pass_slots) {
-                bsf_island_hit h;
-                h.seed = a.surv_seed[sv];
-                h.min_x = minWX + best.minx;
-                h.max_x = minWX + best.maxx;
-                h.min_z = minWZ + best.minz;
-                h.max_z = minWZ + best.maxz;
-                h.min_y = best.miny;
-                h.max_y = best.maxy;
-                h.blocks = best.blocks;
-                h.columns = best.columns;
-                h.center_x = java_round(static_cast<double>(best.sumx) / best.columns);
-                h.center_z = java_round(static_cast<double>(best.sumz) / best.columns);
-                h.chunk_x = h.center_x >> 4;
-                h.chunk_z = h.center_z >> 4;
-                h.flags = best.side ? 1 : 0;
-                h.window_index = a.jobs[jix].tag;
-                a.hits[slot] = h;
-            }
+            if (static_cast<int64_t>(slot) < a.pass_slots) a.hits[slot] = make_hit(a.surv_seed[sv], p, best, a.jobs[jix].tag);
         }
         __syncthreads();
     }
 }
+
+// Region scanner: one workgroup per window of one flood bucket, bitsets in the workgroup's
+// slice of the global scratch.
+template <int W>
+__global__ __launch_bounds__(256) void region_flood_kernel(RegionArgs a, int bucket) {
+    __shared__ FloodShared<W> sh;
+    __shared__ unsigned long long s_item;
+    const int tid = threadIdx.x;
+    unsigned long long* const base = a.flood_scratch + static_cast<size_t>(blockIdx.x) * a.flood_scratch_words;
+    const int32_t* list = a.flood_list + static_cast<size_t>(bucket) * a.ntasks;
+    const unsigned long long n_items = static_cast<unsigned long long>(a.flood_count[bucket]);
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.ctr[kRegionCtrFlood + bucket], 1ULL);
+        __syncthreads();
+        const unsigned long long item = s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const int k = list[item];
+        const RegionTask& t = a.tasks[k];
+        const bsf_island_params p = t.p;
+        CompStats best;
+        const bool found = flood_window<W>(base, a.lattice + t.lat_off, p, sh, &best);
+        if (tid == 0) {
+            a.found[k] = found ? 1 : 0;
+            if (found) a.hits[k] = make_hit(t.seed, p, best, 0);
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void region_flood_kernel<1>(RegionArgs, int);
+template __global__ void region_flood_kernel<2>(RegionArgs, int);
+template __global__ void region_flood_kernel<4>(RegionArgs, int);
+template __global__ void region_flood_kernel<7>(RegionArgs, int);
+template __global__ void region_flood_kernel<13>(RegionArgs, int);
 
 template __global__ void pass_flood_kernel<1, false>(EngineArgs, int);
 template __global__ void pass_flood_kernel<2, false>(EngineArgs, int);

@@ -50,10 +50,12 @@ struct Context {
     DeviceBuffer seeds, out, scratch, scratch2, survivors, counters, lattice, hits, small;
     DeviceBuffer cols_noise, cols_dens, cols_noise2, cols_dens2;
     DeviceBuffer s0_a, s0_b, s0_c, s0_d, s0_e, s0_f, s0_g, s0_h;
+    DeviceBuffer rg_seeds, rg_perms, rg_offs, rg_tasks, rg_lattice, rg_lists, rg_ctr, rg_scratch, rg_hits, rg_found;
     void release_all() {
         for (DeviceBuffer* b : {&seeds, &out, &scratch, &scratch2, &survivors, &counters, &lattice, &hits, &small,
                                 &cols_noise, &cols_dens, &cols_noise2, &cols_dens2, &s0_a, &s0_b, &s0_c, &s0_d, &s0_e,
-                                &s0_f, &s0_g, &s0_h})
+                                &s0_f, &s0_g, &s0_h, &rg_seeds, &rg_perms, &rg_offs, &rg_tasks, &rg_lattice, &rg_lists,
+                                &rg_ctr, &rg_scratch, &rg_hits, &rg_found})
             b->release();
     }
 };

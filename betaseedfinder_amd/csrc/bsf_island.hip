@@ -495,28 +495,6 @@ __global__ __launch_bounds__(64) void pass_stage1b_kernel(EngineArgs a) {
 // ---------------------------------------------------------------------------
 // Stage 2
 // ---------------------------------------------------------------------------
-// Densities of levels [Y0, Y1) for every density column (chunk-local lattice column) that the
-// noise column (u, v) of the window lattice belongs to.
-template <int Y0, int Y1>
-__device__ __forceinline__ void stage2_emit(const bsf_island_params& p, double* lat, int u, int v, const double* n1,
-                                            const double* n2, const double* n3, double n4, double n5,
-                                            const double* s_temp, const double* s_rain) {
-    const int cx_n = p.chunks_x, cz_n = p.chunks_z;
-    for (int ca = (u >> 2) - 1; ca <= (u >> 2); ++ca) {
-        const int i = u - 4 * ca;
-        if (ca < 0 || ca >= cx_n || i > 4) continue;
-        for (int cb = (v >> 2) - 1; cb <= (v >> 2); ++cb) {
-            const int l = v - 4 * cb;
-            if (cb < 0 || cb >= cz_n || l > 4) continue;
-            const int d = (ca * cz_n + cb) * 25 + i * 5 + l;
-            const ColumnShape cs = column_shape(s_temp[d], s_rain[d], n4, n5);
-            double* o = lat + static_cast<size_t>(d) * 8 + (Y0 - 8);
-#pragma unroll
-            for (int k = 0; k < Y1 - Y0; ++k) o[k] = lattice_density(cs, Y0 + k, n1[k], n2[k], n3[k]);
-        }
-    }
-}
-
 constexpr int kS2Threads = 64;
 
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
@@ -572,7 +550,7 @@ __global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a
             ColAcc<8, 16> acc;
             colpar_noise<8, 16, true>(s, X, Z, active, acc);
             __syncthreads();
-            if (active) stage2_emit<8, 16>(p, lat, u, v, acc.n1, acc.n2, acc.n3, acc.n4, acc.n5, s_temp, s_rain);
+            if (active) emit_window_densities<8, 16>(p, lat, u, v, acc.n1, acc.n2, acc.n3, acc.n4, acc.n5, s_temp, s_rain);
         }
         __syncthreads();
         if (tid == 0) {

@@ -134,6 +134,30 @@ struct EngineArgs {
     int32_t pad;
 };
 
+// Densities of levels [Y0, Y1) for every density column (chunk-local lattice column) that the
+// noise column (u, v) of the window lattice belongs to: density column d = (chunk x * chunks_z
+// + chunk z) * 25 + i * 5 + l of lattice cell (i, l) of its chunk, 8 doubles per column from
+// level 8, with each chunk's own climate sample s_temp[d] / s_rain[d].
+template <int Y0, int Y1>
+__device__ __forceinline__ void emit_window_densities(const bsf_island_params& p, double* lat, int u, int v, const double* n1,
+                                            const double* n2, const double* n3, double n4, double n5,
+                                            const double* s_temp, const double* s_rain) {
+    const int cx_n = p.chunks_x, cz_n = p.chunks_z;
+    for (int ca = (u >> 2) - 1; ca <= (u >> 2); ++ca) {
+        const int i = u - 4 * ca;
+        if (ca < 0 || ca >= cx_n || i > 4) continue;
+        for (int cb = (v >> 2) - 1; cb <= (v >> 2); ++cb) {
+            const int l = v - 4 * cb;
+            if (cb < 0 || cb >= cz_n || l > 4) continue;
+            const int d = (ca * cz_n + cb) * 25 + i * 5 + l;
+            const ColumnShape cs = column_shape(s_temp[d], s_rain[d], n4, n5);
+            double* o = lat + static_cast<size_t>(d) * 8 + (Y0 - 8);
+#pragma unroll
+            for (int k = 0; k < Y1 - Y0; ++k) o[k] = lattice_density(cs, Y0 + k, n1[k], n2[k], n3[k]);
+        }
+    }
+}
+
 __global__ void pass_reset_kernel(EngineArgs a);
 __global__ void pass_build_kernel(EngineArgs a);
 __global__ void pass_cache_kernel(EngineArgs a);

@@ -137,6 +137,43 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
 int bsf_island_get_tuning(const bsf_island_params* p, bsf_island_tuning* out);
 int bsf_island_set_tuning(const bsf_island_params* p, const bsf_island_tuning* t);
 
+/* ---- region scans: the exact scan of Stage-0 survivors that the Java side runs on the CPU
+ * (IslandSearchEngine.processGpuRegionAttempt, :1878-1971, over findFloatingIslandInSeed,
+ * :2861-2977).  Windows of 1..49 chunks per axis, the radius-24 border-verification window.
+ * ---------------------------------------------------------------------------------------- */
+/* Exact scan of one window per seed: out_found[i] = 1 and out_hits[i] = the best matching
+ * component of windows[i] (compareComponents order), else out_found[i] = 0. */
+int bsf_window_scan(const int64_t* seeds, const bsf_island_params* windows, int64_t count,
+                    bsf_island_hit* out_hits, int32_t* out_found);
+
+/* One Stage-0 survivor: the region around a centre chunk (GpuSeedWork's seed, centre chunk and
+ * coarse score). */
+typedef struct bsf_region_item {
+    int64_t seed;
+    int32_t center_chunk_x, center_chunk_z;
+    int32_t coarse_score;
+    int32_t pad;
+} bsf_region_item;
+
+typedef struct bsf_region_result {
+    bsf_island_hit hit;               /* the resolved match (valid when found) */
+    int32_t found;
+    int32_t radius_used;              /* MatchResult.chunkRadiusUsed */
+    int32_t normal_found;             /* the scan at the search radius found a match */
+    int32_t hot_scan_ran;             /* coarse >= 120 and radius < 16: radius-16 rescan ran */
+    int32_t hot_won;                  /* the rescan's match replaced the normal one */
+    int32_t pad;
+} bsf_region_result;
+
+/* processGpuRegionAttempt for each item: the scan of the (2 radius + 1)^2-chunk window around
+ * the centre; a match touching the window side is rescanned at radii 10, 13, 16, 20, 24 until
+ * it no longer touches it (verifySideBorderMatch, :3188-3259; no match at a wider radius drops
+ * it); items with coarse >= 120 are also scanned at radius 16 (runHotSeedScan, :3131-3186),
+ * verified the same way, and that match is kept when compareComponents prefers it.  Only the
+ * threshold fields of *settings are used.  All scans of a call are batched per radius. */
+int bsf_region_attempts(const bsf_region_item* items, int64_t count, const bsf_island_params* settings,
+                        int32_t radius, bsf_region_result* out);
+
 /* ---- serving (no counterpart in the reference).  A request loop on its own thread, stream
  * and engine: jobs are admitted as job-table entries free up (state memory) into passes whose
  * hit buffers bound every pass's output (hit-buffer memory); each pass mixes all resident jobs,

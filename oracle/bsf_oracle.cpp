@@ -1304,6 +1304,75 @@ int bsfo_mega_topology_reject(double score, int extreme, const bsfo_p19_features
 
 int bsfo_coarse_score_signs(const uint8_t* signs) { return coarseScoreSigns(signs); }
 
+// Exact scan of the window of chunk radius r around a centre chunk (findFloatingIslandInSeed
+// with hunterMode off, :2861-2977).
+static int regionScan(int64_t seed, int cx, int cz, int r, const bsfo_island_params& s, bsfo_island_hit* hit) {
+    bsfo_island_params p = s;
+    p.chunk_x0 = cx - r;
+    p.chunk_z0 = cz - r;
+    p.chunks_x = 2 * r + 1;
+    p.chunks_z = 2 * r + 1;
+    return islandScan(seed, p, hit);
+}
+
+// compareComponents(a, b) > 0 on two hits (:4079-4094): blocks, then columns, then the
+// bounding-box area.
+static bool hitBetter(const bsfo_island_hit& a, const bsfo_island_hit& b) {
+    if (a.blocks != b.blocks) return a.blocks > b.blocks;
+    if (a.columns != b.columns) return a.columns > b.columns;
+    const int areaA = (a.max_x - a.min_x + 1) * (a.max_z - a.min_z + 1);
+    const int areaB = (b.max_x - b.min_x + 1) * (b.max_z - b.min_z + 1);
+    return areaA > areaB;
+}
+
+static const int kBorderVerifyRadii[5] = {10, 13, 16, 20, 24};  // :106-107
+
+// verifySideBorderMatch (:3188-3259): widen through the checkpoint radii above the result's
+// own until the component no longer touches the side; a wider scan without a match drops it.
+static bool borderVerify(int64_t seed, int cx, int cz, const bsfo_island_params& s, bsfo_island_hit* best,
+                         int* radius) {
+    for (int r : kBorderVerifyRadii) {
+        if (r <= *radius) continue;
+        bsfo_island_hit e;
+        if (!regionScan(seed, cx, cz, r, s, &e)) return false;
+        *best = e;
+        *radius = r;
+        if (!(e.flags & 1)) return true;
+    }
+    return true;
+}
+
+void bsfo_region_attempt(int64_t seed, int32_t cx, int32_t cz, int32_t coarse, const bsfo_island_params* s,
+                         int32_t radius, bsfo_region_result* out) {
+    std::memset(out, 0, sizeof(*out));
+    bsfo_island_hit normal;
+    int normal_r = radius;
+    bool have_normal = regionScan(seed, cx, cz, radius, *s, &normal) != 0;
+    out->normal_found = have_normal ? 1 : 0;
+    if (have_normal && (normal.flags & 1) && radius < 24) have_normal = borderVerify(seed, cx, cz, *s, &normal, &normal_r);
+    bool have = have_normal;
+    bsfo_island_hit match = normal;
+    int match_r = normal_r;
+    if (coarse >= 120 && radius < 16) {  // shouldRunHotSeedScan, :3131-3136
+        out->hot_scan_ran = 1;
+        bsfo_island_hit hot;
+        int hot_r = 16;
+        bool have_hot = regionScan(seed, cx, cz, 16, *s, &hot) != 0;
+        if (have_hot && (hot.flags & 1) && hot_r < 24) have_hot = borderVerify(seed, cx, cz, *s, &hot, &hot_r);
+        if (have_hot && (!have || hitBetter(hot, match))) {
+            match = hot;
+            match_r = hot_r;
+            have = true;
+            out->hot_won = 1;
+        }
+    }
+    out->found = have ? 1 : 0;
+    if (have) {
+        out->hit = match;
+        out->radius_used = match_r;
+    }
+}
+
 // HuntProfileConfig (gpu_p20_benchmark.cpp:7270-7290) and the production centre chunk offsets
 // (:7316-7317).
 struct Stage0Profile {

@@ -100,6 +100,25 @@ int      bsfo_island_from_stone(int64_t seed, const uint8_t* stone, int32_t size
  * has stone at y>=68 with non-stone directly below (a superset of all hits) */
 int      bsfo_island_has_candidate(int64_t seed, const bsfo_island_params* p);
 
+/* ---- region attempt of one Stage-0 survivor (IslandSearchEngine.processGpuRegionAttempt,
+ * :1878-1971): the exact scan of the (2 radius + 1)^2-chunk window around the centre chunk,
+ * border verification of a side-touching result over radii {10, 13, 16, 20, 24}
+ * (verifySideBorderMatch, :3188-3259), and the hot-seed radius-16 rescan when the coarse score
+ * is at least 120 (runHotSeedScan, :3131-3186), kept when compareComponents prefers it.
+ * Only the threshold fields of *settings are read. ---- */
+typedef struct bsfo_region_result {
+    bsfo_island_hit hit;             /* the resolved match (valid when found) */
+    int32_t found;
+    int32_t radius_used;             /* chunkRadiusUsed of the match */
+    int32_t normal_found;            /* the radius scan itself found a match */
+    int32_t hot_scan_ran;
+    int32_t hot_won;                 /* the hot-seed result replaced the normal one */
+    int32_t pad;
+} bsfo_region_result;
+
+void     bsfo_region_attempt(int64_t seed, int32_t center_chunk_x, int32_t center_chunk_z, int32_t coarse_score,
+                             const bsfo_island_params* settings, int32_t radius, bsfo_region_result* out);
+
 /* ---- multithreaded CPU baseline drivers (std::thread) ---- */
 uint64_t bsfo_run_density(int64_t first_seed, int64_t count, int threads, uint64_t* digests);
 int64_t  bsfo_run_scout(int64_t first_seed, int64_t count, double threshold, int threads,

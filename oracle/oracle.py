@@ -151,3 +151,18 @@ def run_stage0_records(seeds, profile="general", research=True, centers=8, threa
                               1 if research else 0, centers, threads, out.ctypes.data_as(ctypes.c_void_p),
                               counters.ctypes.data_as(ctypes.c_void_p))
     return out, counters
+
+
+class RegionResult(ctypes.Structure):
+    _fields_ = [("hit", IslandHit), ("found", ctypes.c_int32), ("radius_used", ctypes.c_int32),
+                ("normal_found", ctypes.c_int32), ("hot_scan_ran", ctypes.c_int32), ("hot_won", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+def region_attempt(seed, center_chunk_x, center_chunk_z, coarse_score, settings, radius=7):
+    """processGpuRegionAttempt of one Stage-0 survivor (same dict layout as the library's)."""
+    r = RegionResult()
+    lib().bsfo_region_attempt(ctypes.c_int64(seed), center_chunk_x, center_chunk_z, coarse_score,
+                              ctypes.byref(settings), radius, ctypes.byref(r))
+    return {"hit": r.hit.as_tuple() if r.found else None, "radius_used": r.radius_used,
+            "normal_found": r.normal_found, "hot_scan_ran": r.hot_scan_ran, "hot_won": r.hot_won}

@@ -513,3 +513,62 @@ def test_stage0_batch_driver_matches_stage_functions(oracle, profile, research, 
                     counts[5] += 1
             assert not np.isnan(struct.unpack("<d", exp[8:16])[0]) or not exp[3]
     assert ctr.tolist() == [len(seeds), len(seeds) * centers] + counts
+
+
+def _attempt_by_direct_scans(oracle, seed, cx, cz, coarse, s, radius):
+    """processGpuRegionAttempt (IslandSearchEngine.java:1878-1971) restated over direct window
+    scans, to pin the oracle's composition: verifySideBorderMatch (:3188-3259) and the hot-seed
+    rescan (:3131-3186)."""
+    def scan(r):
+        p = oracle.island_params(cx - r, cz - r, 2 * r + 1, 2 * r + 1, s.min_blocks, s.min_columns,
+                                 s.min_y_for_match, s.min_width, s.min_depth, s.min_avg_thickness)
+        return oracle.island_scan(seed, p)
+
+    def verify(hit, r0):
+        best, r_best = hit, r0
+        for r in (10, 13, 16, 20, 24):
+            if r <= r_best:
+                continue
+            e = scan(r)
+            if e is None:
+                return None, 0
+            best, r_best = e, r
+            if not (e[13] & 1):
+                break
+        return best, r_best
+
+    def better(a, b):  # compareComponents(a, b) > 0 on hit tuples
+        if a[9] != b[9]:
+            return a[9] > b[9]
+        if a[10] != b[10]:
+            return a[10] > b[10]
+        return (a[4] - a[3] + 1) * (a[6] - a[5] + 1) > (b[4] - b[3] + 1) * (b[6] - b[5] + 1)
+
+    normal = scan(radius)
+    out = {"normal_found": int(normal is not None), "hot_scan_ran": 0, "hot_won": 0}
+    match, r_match = normal, radius
+    if normal is not None and (normal[13] & 1) and radius < 24:
+        match, r_match = verify(normal, radius)
+    if coarse >= 120 and radius < 16:
+        out["hot_scan_ran"] = 1
+        hot, r_hot = scan(16), 16
+        if hot is not None and (hot[13] & 1):
+            hot, r_hot = verify(hot, 16)
+        if hot is not None and (match is None or better(hot, match)):
+            match, r_match = hot, r_hot
+            out["hot_won"] = 1
+    out["hit"] = match
+    out["radius_used"] = r_match if match is not None else 0
+    return out
+
+
+@pytest.mark.parametrize("item", [
+    (86, 0, 0, 0),    # contained at radius 7
+    (108, 0, 0, 0),   # touches the side at 7, contained at 10
+    (439, 0, 0, 0),   # touches the side at 7, no match at 10: dropped
+    (5, 0, 0, 121),   # no match at 7; the hot-seed rescan at 16 finds one
+])
+def test_region_attempt_composition(oracle, item):
+    s = oracle.island_params(0, 0, 1, 1, min_blocks=50, min_columns=10, min_width=3, min_depth=3)
+    got = oracle.region_attempt(*item, s, 7)
+    assert got == _attempt_by_direct_scans(oracle, *item, s, 7)
