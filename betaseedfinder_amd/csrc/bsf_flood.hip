@@ -315,6 +315,7 @@ struct FloodShared {
     CompStats cs;
     unsigned long long zor[W];
     int total;
+    unsigned long long starts[256];  // candidate-start levels of a slice of sample columns
 };
 
 // The flood of one window: upsampling of its lattice (levels 8..15 of every density column,
@@ -382,27 +383,49 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
 
     bool have_best = false;
     CompStats best{};
-    // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up
-    for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
-        const int cxi = ch / p.chunks_z, czi = ch % p.chunks_z;
-        for (int sc = 0; sc < 4; ++sc) {
-            const int x = cxi * 16 + ((sc >> 1) ? 12 : 4);
-            const int z = czi * 16 + ((sc & 1) ? 12 : 4);
+    // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up.
+    //    Sample columns are taken in slices of blockDim.x: each thread finds the start levels
+    //    of one column (stone over non-stone: y = 68 over the y = 67 row, above that over a
+    //    block that is not in rem -- a stone block below a floating block is floating too),
+    //    then the slice's starts are walked in order, skipping blocks an earlier flood took.
+    const int npairs = p.chunks_x * p.chunks_z * 4;
+    for (int p0 = 0; p0 < npairs; p0 += blockDim.x) {
+        {
+            const int pi = p0 + tid;
+            unsigned long long m = 0;
+            if (pi < npairs) {
+                const int ch = pi >> 2, sc = pi & 3;
+                const int x = (ch / p.chunks_z) * 16 + ((sc >> 1) ? 12 : 4);
+                const int z = (ch % p.chunks_z) * 16 + ((sc & 1) ? 12 : 4);
+                const int zw = z >> 6;
+                const unsigned long long bitz = 1ULL << (z & 63);
+                const unsigned long long* col = rem + static_cast<size_t>(x) * kRowStride * W + zw;
+                bool below = (st67[static_cast<size_t>(x) * W + zw] & bitz) != 0;
+                for (int yy = 0; yy < kLevels; ++yy) {
+                    const bool here = (col[static_cast<size_t>(yy) * W] & bitz) != 0;
+                    if (here && !below) m |= 1ULL << yy;
+                    below = here;
+                }
+            }
+            sh.starts[tid] = m;
+        }
+        __syncthreads();
+        const int slice = npairs - p0 < static_cast<int>(blockDim.x) ? npairs - p0 : static_cast<int>(blockDim.x);
+        for (int k = 0; k < slice; ++k) {
+            unsigned long long todo = sh.starts[k];
+            const int ch = (p0 + k) >> 2, sc = (p0 + k) & 3;
+            const int x = (ch / p.chunks_z) * 16 + ((sc >> 1) ? 12 : 4);
+            const int z = (ch % p.chunks_z) * 16 + ((sc & 1) ? 12 : 4);
             const int zw = z >> 6;
             const unsigned long long bitz = 1ULL << (z & 63);
-            for (int y = kFloorY; y < 128; ++y) {
+            while (todo) {
+                const int yy = __ffsll(static_cast<long long>(todo)) - 1;
+                todo &= todo - 1;
                 // stone not yet visited (visited or grounded blocks are no longer in rem)
-                const unsigned long long here = rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw];
-                if ((here & bitz) == 0) continue;
-                // the block below is not stone: below y = 68 read the y = 67 row; above, a
-                // stone block below would share this block's component, so it is still in rem
-                const unsigned long long below =
-                        y == kFloorY ? st67[static_cast<size_t>(x) * W + zw]
-                                     : rem[(static_cast<size_t>(x) * kRowStride + (y - 1 - kFloorY)) * W + zw];
-                if ((below & bitz) != 0) continue;
+                if ((rem[(static_cast<size_t>(x) * kRowStride + yy) * W + zw] & bitz) == 0) continue;
                 for (int r = tid; r < rows * W; r += blockDim.x) comp[r] = 0;
                 __syncthreads();
-                if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + zw] = bitz;
+                if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + yy) * W + zw] = bitz;
                 __syncthreads();
                 grow_component<W>(comp, rem, nx);
                 // statistics of the component
@@ -488,6 +511,7 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
                 __syncthreads();
             }
         }
+        __syncthreads();  // the slice's start masks are read before the next slice rewrites them
     }
     *best_out = best;
     return have_best;
