@@ -263,11 +263,14 @@ def run_density_scout(bsf, args):
     bsf.stage0_records(worlds[:1024], "general", True, centers=8)
     t = time.perf_counter()
     totals = np.zeros(11, dtype=np.int64)
+    batches = []
     for b in range(1, args.stage0_batches + 1):
-        _, c = bsf.stage0_records(worlds[b * 32768:(b + 1) * 32768], "general", True, centers=8)
+        recs, c = bsf.stage0_records(worlds[b * 32768:(b + 1) * 32768], "general", True, centers=8)
         totals += np.array([getattr(c, n) for n, _ in c._fields_], dtype=np.int64)
+        batches.append(recs)
     s0_s = time.perf_counter() - t
     n_worlds = 32768 * args.stage0_batches
+    regions = run_regions(bsf, args, worlds, batches)
     return {
         "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
                          "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
@@ -282,7 +285,46 @@ def run_density_scout(bsf, args):
                          "regions_per_s": round(8 * n_worlds / s0_s, 1), "ms_per_batch":
                          round(s0_s * 1e3 / args.stage0_batches, 2),
                          "counters": dict(zip([n for n, _ in bsf.Stage0Counters._fields_], totals.tolist()))},
+        "region_attempts": regions,
     }
+
+
+CENTER_X = [0, 15, 0, 15, -15, -15, 0, 15]  # production centres, chunk offsets
+CENTER_Z = [0, 0, 15, 15, 0, 15, -15, -15]
+
+
+def run_regions(bsf, args, worlds, batches):
+    """Region attempts (exact radius-7 scan, border verification, hot-seed rescan) of the
+    stage0stream survivors of the timed batches: P19 pass, no mega reject, coarse >= 85."""
+    import numpy as np
+    items = []
+    for b, recs in enumerate(batches):
+        coarse = recs[:, :, 4].astype(np.int32) | (recs[:, :, 5].astype(np.int32) << 8)
+        ok = (recs[:, :, 3] == 1) & (recs[:, :, 7] == 0) & (coarse >= 85)
+        for wi, ci in zip(*np.nonzero(ok)):
+            items.append((int(worlds[(b + 1) * 32768 + wi]), CENTER_X[ci], CENTER_Z[ci], int(coarse[wi, ci])))
+    settings = bsf.island_params(0, 0, 1, 1)  # SearchSettings defaults: 1000 blocks, 100 columns, ...
+    bsf.region_attempts(items[:2], settings, 7)
+    t = time.perf_counter()
+    res = bsf.region_attempts(items, settings, 7)
+    gpu_s = time.perf_counter() - t
+    out = {"workload": "region attempts of the stage0stream survivors above (radius 7, SearchSettings defaults)",
+           "regions": len(items), "regions_per_s": round(len(items) / gpu_s, 1), "ms": round(gpu_s * 1e3, 2),
+           "matches": sum(r["hit"] is not None for r in res), "hot_scans": sum(r["hot_scan_ran"] for r in res),
+           "border_verified": sum(r["radius_used"] > 7 for r in res)}
+    if not args.no_cpu and items:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle
+        sample = items[:args.region_cpu_items]
+        os_ = oracle.island_params(0, 0, 1, 1)
+        t = time.perf_counter()
+        with ThreadPoolExecutor(16) as ex:
+            ref = list(ex.map(lambda it: oracle.region_attempt(*it, os_, 7), sample))
+        cpu_s = time.perf_counter() - t
+        out["cpu_oracle"] = {"regions_per_s": round(len(sample) / cpu_s, 2), "cores": 16,
+                             "sample": "first %d regions through the CPU oracle (16 threads, %.1f s); results equal: %s"
+                                       % (len(sample), cpu_s, ref == res[:len(sample)])}
+    return out
 
 
 def run_serving(bsf, args):
@@ -311,6 +353,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--scout-log2", type=int, default=30)
     ap.add_argument("--stage0-batches", type=int, default=4)
+    ap.add_argument("--region-cpu-items", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-seeds", type=int, default=1 << 15)

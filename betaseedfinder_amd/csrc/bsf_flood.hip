@@ -313,6 +313,8 @@ __device__ int count_outside(const unsigned long long* rem, const unsigned long 
 template <int W>
 struct FloodShared {
     CompStats cs;
+    CompStats best;                  // best matching component so far (written by thread 0)
+    int have_best;
     unsigned long long zor[W];
     int total;
     unsigned long long starts[256];  // candidate-start levels of a slice of sample columns
@@ -381,8 +383,10 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
     }
     __syncthreads();
 
-    bool have_best = false;
-    CompStats best{};
+    // The best component lives in LDS, updated by thread 0: kept in registers across the
+    // candidate loop, hipcc (ROCm 7.2, gfx950) dropped the copy of its min y on the path where
+    // a later component replaces an earlier match.
+    if (tid == 0) sh.have_best = 0;
     // 4. candidate starts in Java order: chunk x, chunk z, x in {4,12}, z in {4,12}, y up.
     //    Sample columns are taken in slices of blockDim.x: each thread finds the start levels
     //    of one column (stone over non-stone: y = 68 over the y = 67 row, above that over a
@@ -504,17 +508,17 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
                                    depth >= p.min_depth &&
                                    cs.blocks / static_cast<double>(cs.columns > 1 ? cs.columns : 1) >=
                                            p.min_avg_thickness;
-                if (match && (!have_best || comp_better(cs, best))) {
-                    best = cs;
-                    have_best = true;
+                if (tid == 0 && match && (!sh.have_best || comp_better(cs, sh.best))) {
+                    sh.best = cs;
+                    sh.have_best = 1;
                 }
                 __syncthreads();
             }
         }
         __syncthreads();  // the slice's start masks are read before the next slice rewrites them
     }
-    *best_out = best;
-    return have_best;
+    *best_out = sh.best;
+    return sh.have_best != 0;
 }
 
 // The hit record of a window's best component.

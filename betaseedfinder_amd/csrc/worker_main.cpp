@@ -14,6 +14,9 @@
 //   stage0stream [capacity=32768] [general|mega|record60|record80] [research|lean] [centers=1|8]
 //   density [capacity=32768] [chunkX=0] [chunkZ=0]      magic DNSTY425: count x 425 LE f64
 //   island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]   hit lines (text)
+//   region [capacity=4096] [radius=7] [minBlocks minColumns minY minWidth minDepth minAvgThickness]
+//          magic REGION01; request LE u32 count, count x (i64 seed, i32 centre chunk x, i32 centre
+//          chunk z, i32 coarse score); response LE u32 count, count x 56-byte results
 #include <cinttypes>
 #include <cstdint>
 #include <cstdio>
@@ -229,6 +232,82 @@ int island_mode(int argc, char** argv) {
     return 0;
 }
 
+// region: region attempts of Stage-0 survivors (the exact scans that IslandSearchEngine.
+// processGpuRegionAttempt runs on the CPU, :1878-1971).  Each 56-byte result: u8 found, u8
+// radius used, u8 normal scan found, u8 hot scan ran, u8 hot scan won, u8 touches side, 2 zero
+// bytes, then LE i32 chunk x, chunk z, min x, max x, min z, max z, min y, max y, blocks,
+// columns, centre x, centre z of the match (zero when none).
+int region_mode(int argc, char** argv) {
+    const int capacity = argc > 2 ? parse_int(argv[2], "capacity") : 4096;
+    const int radius = argc > 3 ? parse_int(argv[3], "radius") : 7;
+    if (capacity <= 0) throw WorkerError("region capacity must be positive");
+    if (radius < 0 || radius > 24) throw WorkerError("region radius must be 0..24");
+    bsf_island_params s{0, 0, 1, 1, 1000, 100, 60, 10, 10, 2.0};  // SearchSettings defaults
+    if (argc >= 10) {
+        s.min_blocks = parse_int(argv[4], "minBlocks");
+        s.min_columns = parse_int(argv[5], "minColumns");
+        s.min_y_for_match = parse_int(argv[6], "minY");
+        s.min_width = parse_int(argv[7], "minWidth");
+        s.min_depth = parse_int(argv[8], "minDepth");
+        s.min_avg_thickness = std::strtod(argv[9], nullptr);
+    }
+    print_device();
+    std::fprintf(stderr, "Region stream ready | capacity=%d | radius=%d | minBlocks=%d\n", capacity, radius, s.min_blocks);
+    if (!write_all("REGION01", 8) || std::fflush(stdout) != 0) throw WorkerError("failed writing region magic");
+    std::vector<unsigned char> raw;
+    std::vector<bsf_region_item> items(static_cast<size_t>(capacity));
+    std::vector<bsf_region_result> res(static_cast<size_t>(capacity));
+    std::vector<unsigned char> out(static_cast<size_t>(capacity) * 56);
+    uint64_t total = 0, matches = 0, hot = 0;
+    for (;;) {
+        uint32_t count = 0;
+        if (!try_read_le32(count) || count == 0) break;
+        if (count > static_cast<uint32_t>(capacity)) throw WorkerError("region batch exceeds configured capacity");
+        raw.resize(static_cast<size_t>(count) * 20);
+        if (std::fread(raw.data(), 1, raw.size(), stdin) != raw.size()) throw WorkerError("truncated region batch");
+        auto le = [&](size_t at, int bytes) {
+            uint64_t v = 0;
+            for (int b = bytes - 1; b >= 0; --b) v = (v << 8) | raw[at + b];
+            return v;
+        };
+        for (uint32_t i = 0; i < count; ++i) {
+            const size_t at = static_cast<size_t>(i) * 20;
+            bsf_region_item& it = items[i];
+            it.seed = static_cast<int64_t>(le(at, 8));
+            it.center_chunk_x = static_cast<int32_t>(static_cast<uint32_t>(le(at + 8, 4)));
+            it.center_chunk_z = static_cast<int32_t>(static_cast<uint32_t>(le(at + 12, 4)));
+            it.coarse_score = static_cast<int32_t>(static_cast<uint32_t>(le(at + 16, 4)));
+            it.pad = 0;
+        }
+        check(bsf_region_attempts(items.data(), count, &s, radius, res.data()), "region batch");
+        std::memset(out.data(), 0, static_cast<size_t>(count) * 56);
+        for (uint32_t i = 0; i < count; ++i) {
+            unsigned char* r = out.data() + static_cast<size_t>(i) * 56;
+            const bsf_region_result& x = res[i];
+            r[0] = static_cast<unsigned char>(x.found);
+            r[1] = static_cast<unsigned char>(x.radius_used);
+            r[2] = static_cast<unsigned char>(x.normal_found);
+            r[3] = static_cast<unsigned char>(x.hot_scan_ran);
+            r[4] = static_cast<unsigned char>(x.hot_won);
+            hot += static_cast<uint64_t>(x.hot_scan_ran);
+            if (!x.found) continue;
+            r[5] = static_cast<unsigned char>(x.hit.flags & 1);
+            const int32_t v[12] = {x.hit.chunk_x, x.hit.chunk_z, x.hit.min_x, x.hit.max_x, x.hit.min_z, x.hit.max_z,
+                                   x.hit.min_y, x.hit.max_y, x.hit.blocks, x.hit.columns, x.hit.center_x, x.hit.center_z};
+            for (int k = 0; k < 12; ++k)
+                for (int b = 0; b < 4; ++b)
+                    r[8 + 4 * k + b] = static_cast<unsigned char>(static_cast<uint32_t>(v[k]) >> (8 * b));
+            ++matches;
+        }
+        if (!write_le32(count) || !write_all(out.data(), static_cast<size_t>(count) * 56) || std::fflush(stdout) != 0)
+            throw WorkerError("failed writing region response");
+        total += count;
+    }
+    std::fprintf(stderr, "Region stream closed | regions=%" PRIu64 " | matches=%" PRIu64 " | hot_scans=%" PRIu64 "\n",
+                 total, matches, hot);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -240,7 +319,9 @@ int main(int argc, char** argv) {
                          "  BetaSeedFinderWorker stage0stream [capacity=32768] [general|mega|record60|record80] "
                          "[research|lean] [centers=1|8]\n"
                          "  BetaSeedFinderWorker density [capacity=32768] [chunkX=0] [chunkZ=0]\n"
-                         "  BetaSeedFinderWorker island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]\n");
+                         "  BetaSeedFinderWorker island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]\n"
+                         "  BetaSeedFinderWorker region [capacity=4096] [radius=7] [minBlocks minColumns minY minWidth "
+                         "minDepth minAvgThickness]\n");
             return 1;
         }
         const std::string mode = argv[1];
@@ -258,6 +339,8 @@ int main(int argc, char** argv) {
                               argc > 4 ? parse_int(argv[4], "chunkZ") : 0);
         } else if (mode == "island") {
             rc = island_mode(argc, argv);
+        } else if (mode == "region") {
+            rc = region_mode(argc, argv);
         } else {
             throw WorkerError("unknown mode: " + mode);
         }

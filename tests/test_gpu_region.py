@@ -64,3 +64,17 @@ def test_region_scan_rejects_bad_windows(bsf):
         bsf.window_scan([1], [p])
     with pytest.raises(bsf.BsfError):
         bsf.region_attempts([(1, 0, 0, 0)], bsf.island_params(0, 0, 1, 1), radius=25)
+
+
+def test_best_component_after_an_earlier_match(bsf, oracle):
+    """A window whose largest island is found after a smaller matching one: every field of the
+    later component (its min y included) must replace the earlier one's, in the engine's flood
+    and in the region scanner's."""
+    seed = 272631846895339
+    windows = [(13, -14, 6, 5), (12, -16, 8, 8), (8, -22, 15, 15), (8, -16, 15, 8), (14, -13, 4, 4)]
+    for w in windows:
+        ref = oracle.island_scan(seed, oracle.island_params(*w))
+        assert ref is not None and ref[7] == 75
+        assert bsf.window_scan([seed], [bsf.island_params(*w)])[0] == ref, w
+        if w[2] <= 8 and w[3] <= 8:
+            assert bsf.island_seeds([seed], bsf.island_params(*w)) == [ref], w

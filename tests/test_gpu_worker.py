@@ -140,3 +140,32 @@ def test_worker_fails_loudly_without_gpu():
     p = spawn("stream", 8)
     out, err = p.communicate(timeout=60)
     assert p.returncode == 1 and out == b"" and b"ERROR: init" in err
+
+
+@pytest.mark.gpu
+def test_region_stream_matches_library(bsf):
+    """`region` mode: magic, framing and 56-byte results against bsf_region_attempts."""
+    items = [(108, 0, 0, 0), (146, 0, 0, 130), (86, 0, 0, 0), (5, 0, 0, 121), (439, 0, 0, 0), (229, 15, -15, 0)]
+    p = spawn("region", 16, 7, 50, 10, 60, 3, 3, "2.0")
+    assert read_exact(p, 8) == b"REGION01"
+    req = struct.pack("<I", len(items)) + b"".join(struct.pack("<qiii", *it) for it in items)
+    p.stdin.write(req)
+    p.stdin.flush()
+    (count,) = struct.unpack("<I", read_exact(p, 4))
+    body = read_exact(p, count * 56)
+    p.stdin.write(struct.pack("<I", 0))
+    p.stdin.flush()
+    out, err = p.communicate(timeout=60)
+    assert p.returncode == 0 and b"Region stream closed | regions=6" in err
+    ref = bsf.region_attempts(items, bsf.island_params(0, 0, 1, 1, min_blocks=50, min_columns=10, min_width=3,
+                                                       min_depth=3), radius=7)
+    for i, r in enumerate(ref):
+        rec = body[i * 56:(i + 1) * 56]
+        found, radius, normal, hot, hot_won, side = rec[:6]
+        assert (found, radius, normal, hot, hot_won) == (int(r["hit"] is not None), r["radius_used"],
+                                                          r["normal_found"], r["hot_scan_ran"], r["hot_won"])
+        if r["hit"] is None:
+            assert rec[5:] == bytes(51)
+        else:
+            assert side == r["hit"][13] & 1
+            assert list(struct.unpack("<12i", rec[8:])) == list(r["hit"][1:13])
