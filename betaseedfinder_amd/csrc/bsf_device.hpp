@@ -450,20 +450,16 @@ __device__ __forceinline__ int simplex_floor(double v) {
     const int t = static_cast<int>(v);
     return v > 0.0 ? t : t - 1;
 }
+// grad3[g][0] * x + grad3[g][1] * z of NoiseGenerator2173 (g < 12): each product is +/-x, or
+// 0.0 * x, the zero carrying x's sign.  g < 8: x * (+/-1 by bit 0); else 0.  g < 4: z * (+/-1 by
+// bit 1); 4..7: 0; 8..11: z * (+/-1 by bit 0).
 __device__ __forceinline__ double simplex_grad(int g, double x, double z) {
-    switch (g) {
-        case 0: return 1.0 * x + 1.0 * z;
-        case 1: return -1.0 * x + 1.0 * z;
-        case 2: return 1.0 * x + -1.0 * z;
-        case 3: return -1.0 * x + -1.0 * z;
-        case 4:
-        case 6: return 1.0 * x + 0.0 * z;
-        case 5:
-        case 7: return -1.0 * x + 0.0 * z;
-        case 8:
-        case 10: return 0.0 * x + 1.0 * z;
-        default: return 0.0 * x + -1.0 * z;
-    }
+    const uint32_t ug = static_cast<uint32_t>(g);
+    const double zero_x = __hiloint2double(__double2hiint(x) & static_cast<int>(0x80000000u), 0);
+    const double zero_z = __hiloint2double(__double2hiint(z) & static_cast<int>(0x80000000u), 0);
+    const double tx = g < 8 ? flip_sign(x, ug << 31) : zero_x;
+    const double tz = g < 4 ? flip_sign(z, (ug << 30) & 0x80000000u) : (g < 8 ? zero_z : flip_sign(z, ug << 31));
+    return tx + tz;
 }
 template <class Perm>
 __device__ __forceinline__ double simplex2(Perm p, double d5, double d6) {
@@ -487,15 +483,17 @@ __device__ __forceinline__ double simplex2(Perm p, double d5, double d6) {
     const int j2 = p.get(l1 + p.get(i2)) % 12;
     const int k2 = p.get(l1 + b0 + p.get(i2 + b1)) % 12;
     const int l2 = p.get(l1 + 1 + p.get(i2 + 1)) % 12;
+    // each corner contributes 0.0 when its falloff is negative; the products are formed either
+    // way and selected, so lanes stay converged
     double d17 = 0.5 - d11 * d11 - d12 * d12;
-    double d18 = 0.0;
-    if (!(d17 < 0.0)) { d17 *= d17; d18 = d17 * d17 * simplex_grad(j2, d11, d12); }
+    const double e17 = d17 * d17;
+    const double d18 = !(d17 < 0.0) ? e17 * e17 * simplex_grad(j2, d11, d12) : 0.0;
     double d19 = 0.5 - d13 * d13 - d14 * d14;
-    double d20 = 0.0;
-    if (!(d19 < 0.0)) { d19 *= d19; d20 = d19 * d19 * simplex_grad(k2, d13, d14); }
+    const double e19 = d19 * d19;
+    const double d20 = !(d19 < 0.0) ? e19 * e19 * simplex_grad(k2, d13, d14) : 0.0;
     double d21 = 0.5 - d15 * d15 - d16 * d16;
-    double d22 = 0.0;
-    if (!(d21 < 0.0)) { d21 *= d21; d22 = d21 * d21 * simplex_grad(l2, d15, d16); }
+    const double e21 = d21 * d21;
+    const double d22 = !(d21 < 0.0) ? e21 * e21 * simplex_grad(l2, d15, d16) : 0.0;
     return 70.0 * (d18 + d20 + d22);
 }
 
