@@ -196,27 +196,17 @@ __device__ __forceinline__ double grad3(int h, double x, double y, double z) {
     return flip_sign(u, uh << 31) + flip_sign(v, (uh << 30) & 0x80000000u);
 }
 
-// a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0.
+// a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0: Java
+// forms u = (1 - bit 3) * x and v = 0.0 for h < 4, x for h in {12, 14}, else z, then the same
+// sign flips as grad3.  That is grad3's selection with u's alternative the signed zero 0.0 * x
+// and v's the constant +0.0, so it is written the same way, without branches.
 __device__ __forceinline__ double grad2_legacy(int h, double x, double z) {
+    h &= 15;
     const double zeroX = 0.0 * x;
-    switch (h & 15) {
-        case 0: return x + 0.0;
-        case 1: return -x + 0.0;
-        case 2: return x + -0.0;
-        case 3: return -x + -0.0;
-        case 4: return x + z;
-        case 5: return -x + z;
-        case 6: return x + -z;
-        case 7: return -x + -z;
-        case 8: return zeroX + z;
-        case 9: return -zeroX + z;
-        case 10: return zeroX + -z;
-        case 11: return -zeroX + -z;
-        case 12: return zeroX + x;
-        case 13: return -zeroX + z;
-        case 14: return zeroX + -x;
-        default: return -zeroX + -z;
-    }
+    const double u = h < 8 ? x : zeroX;
+    const double v = h < 4 ? 0.0 : ((h == 12 || h == 14) ? x : z);
+    const uint32_t uh = static_cast<uint32_t>(h);
+    return flip_sign(u, uh << 31) + flip_sign(v, (uh << 30) & 0x80000000u);
 }
 
 // Axis sample: Java floor index (& 255), fraction, fraction - 1, fade.
