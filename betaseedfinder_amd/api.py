@@ -237,7 +237,7 @@ def island_batch(jobs, cap=1 << 20):
     return [out[i].as_tuple() for i in range(n.value)], per_job[:len(jobs)].copy()
 
 
-def engine_configure(pass_slots=1 << 17, max_jobs=1024, min_quantum=1024):
+def engine_configure(pass_slots=1 << 18, max_jobs=1024, min_quantum=1024):
     init()
     _check(lib().bsf_island_engine_configure(ctypes.c_int64(pass_slots), ctypes.c_int32(max_jobs),
                                              ctypes.c_int32(min_quantum)))

@@ -137,7 +137,7 @@ constexpr int kNdBucket[2] = {400, 1600};
 IslandEngine& engine();
 void ensure_engine();
 
-constexpr int64_t kDefaultPassSlots = int64_t(1) << 17;
+constexpr int64_t kDefaultPassSlots = int64_t(1) << 18;
 constexpr int kDefaultMaxJobs = 1024;
 constexpr int kDefaultMinQuantum = 1024;
 

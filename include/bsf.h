@@ -114,7 +114,7 @@ int bsf_island_batch(const bsf_island_job* jobs, int32_t njobs, bsf_island_hit* 
                      int64_t* n_found, int64_t* job_hits);
 
 /* Engine shape: seeds per pass (64 .. 2^22), job-table entries, smallest per-job share of a
- * pass.  Defaults 2^17, 1024, 1024. */
+ * pass.  Defaults 2^18, 1024, 1024. */
 int bsf_island_engine_configure(int64_t pass_slots, int32_t max_jobs, int32_t min_quantum);
 
 /* Totals of the library's engine since bsf_init (or the last configure): passes, seeds,
