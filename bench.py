@@ -11,7 +11,7 @@ batch of 2^B consecutive seeds of that range (inputs are a seed range: nothing i
 the device).  The timed region covers K steps; seeds/s = K * 2^B / elapsed.
 
 The JSON line also carries:
-  roofline      FP64 VALU roofline of the dominant kernel (island Stage 1), timed with HIP
+  roofline      FP64 VALU roofline of the dominant FP64 kernel (island Stage 2 here), timed with HIP
                 events on its own stream over the timed region;
   cpu_baseline  the CPU oracle on a bounded sample of the same range, all host threads
                 (rank 0, N = 1 only), with the sample's hit list compared to the GPU's.
@@ -32,7 +32,7 @@ FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no
 # HBM bytes per unit of work of the FP64 kernels: FETCH_SIZE x 1.99975 (the factor by which
 # a 256 MiB device copy under-reads) + WRITE_SIZE, from the rocprofv3 --pmc runs summarised in
 # profiles/island_pipeline_4x4_v1134.md.
-HBM_BYTES_PER_UNIT = {"island_stage1": 16711, "island_stage2": 109515}
+HBM_BYTES_PER_UNIT = {"island_stage1": 16714, "island_stage2": 113170}
 
 
 def splitmix64(x):
