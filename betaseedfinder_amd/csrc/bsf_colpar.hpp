@@ -77,13 +77,17 @@ struct OneWorld {
         __syncthreads();
         if (g + 1 < kGenCount) fetch(g + 1);
     }
-    // Generator g of a 3D octave with y scale sy, and its Y-axis table over [Y0, Y1).
-    template <int Y0, int Y1>
-    __device__ __forceinline__ void stage_y(int g, double sy) const {
+    // Generator g of a 3D octave with y scale sy, and its Y-axis table over [y0, y0 + NY).
+    template <int NY>
+    __device__ __forceinline__ void stage_yn(int g, double sy, int y0) const {
         publish(g);
-        if (threadIdx.x < 64) build_ytable<Y0, Y1>(st->yt, sy, pf_b);
+        if (threadIdx.x < 64) build_ytable_n<NY>(st->yt, sy, pf_b, y0);
         __syncthreads();
         if (g + 1 < kGenCount) fetch(g + 1);
+    }
+    template <int Y0, int Y1>
+    __device__ __forceinline__ void stage_y(int g, double sy) const {
+        stage_yn<Y1 - Y0>(g, sy, Y0);
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
@@ -172,6 +176,29 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
     }
 }
 
+// colpar_3d for one world per workgroup with the first level y0 a run-time value (levels
+// [y0, y0 + NY)), so that a kernel can walk level blocks in a loop.
+template <int NY>
+__device__ __forceinline__ void colpar_3d_n(const OneWorld& s, int g0, int octaves, double s_xz, double s_y, double X,
+                                            double Z, bool active, double* acc, int y0) {
+    double amp = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
+        s.template stage_yn<NY>(g0 + o, sy, y0);
+        if (active) {
+            const YTable& yt = s.ytable();
+            const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
+            const Axis ax = axis(X * sx + s.off(0));
+            const Axis az = axis(Z * sx + s.off(2));
+            double v[NY];
+            perlin3_column_table<0, NY>(s.perm(), ax, az, yt, fresh, w, v);
+#pragma unroll
+            for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+        }
+        amp /= 2.0;
+    }
+}
+
 template <class S>
 __device__ __forceinline__ double colpar_2d(const S& s, int g0, int octaves, double sc, double X, double Z,
                                             bool active) {
@@ -223,6 +250,15 @@ __device__ __forceinline__ void colpar_climate_all(const S& s, const double* BX,
 #pragma unroll
     for (int j = 0; j < N; ++j)
         if (j < n) climate_finish(t[j], r[j], c[j], temp[j], rain[j]);
+}
+
+// The three 3D octave groups of one lattice column over levels [y0, y0 + NY), y0 at run time.
+template <int NY>
+__device__ __forceinline__ void colpar_noise3_n(const OneWorld& s, double X, double Z, bool active, ColAcc<0, NY>& a,
+                                                int y0) {
+    colpar_3d_n<NY>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2, y0);
+    colpar_3d_n<NY>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3, y0);
+    colpar_3d_n<NY>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, a.n1, y0);
 }
 
 // The three 3D octave groups of one lattice column over levels [Y0, Y1) (noise2, noise3,

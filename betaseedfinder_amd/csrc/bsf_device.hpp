@@ -359,11 +359,11 @@ struct YTable {
     uint32_t fresh;
 };
 
-// Called by every lane of wave 0.
-template <int Y0, int Y1>
-__device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
-    constexpr int NY = Y1 - Y0;
+// Called by every lane of wave 0; levels [y0, y0 + NY).
+template <int NY>
+__device__ __forceinline__ void build_ytable_n(YTable& t, double sy, double b, int y0) {
     static_assert(NY <= kMaxLevels, "too many levels");
+    const int Y0 = y0;
     const int k = static_cast<int>(threadIdx.x);
     const bool mine = k < NY;
     const double v = static_cast<double>(Y0 + (mine ? k : 0)) * sy + b;
@@ -396,6 +396,10 @@ __device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
         t.yi[k] = idx;
     }
     if (k == 0) t.fresh = mask;
+}
+template <int Y0, int Y1>
+__device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
+    build_ytable_n<Y1 - Y0>(t, sy, b, Y0);
 }
 
 // perlin3_column_cells with the Y axis read from a shared table; `fresh` must be uniform.

@@ -10,8 +10,9 @@
 //  * the staged gates (P20 -> upper -> lower re-entry + P19 / mega -> coarse): one workgroup per
 //    item; each stage appends the items that pass its gate to a device list that the next
 //    stage walks, so a batch needs no host round trip until its records are assembled.
-//  * s0_coarse_kernel: one workgroup per item, the 61x61x17 coarse sign grid built in LDS
-//    and scored by bit-parallel component search (one uint64 z-mask per (x, y) row).
+//  * s0_coarse_signs_kernel: one 64-lane workgroup per item builds the 61x61x17 coarse sign
+//    grid in LDS (one uint64 z-mask per (x, y) row); s0_coarse_score_kernel scores it by
+//    bit-parallel component search.
 #include "bsf_stage0.hpp"
 #include "bsf_cache.hpp"
 
@@ -321,14 +322,13 @@ __global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
 // Lower re-entry (Y 0..10 of the columns whose upper mask is non-zero) and, for the items
 // whose high count reaches high_min, the P19 features, score, extreme signal and mega
 // topology reject (gpu_p20_benchmark.cpp:7473-7489).  Items that pass and are not rejected
-// are appended for the coarse stage.  Needs blockDim.x == 256 (one thread per column).
-__global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
+// are appended for the coarse stage.  One 64-lane wave per item: it compacts the item's
+// active columns (lane t reads the masks of columns 4t .. 4t + 3) and walks them 64 at a time.
+__global__ __launch_bounds__(64, 2) void s0_lower_kernel(S0Args a) {
     __shared__ GenStage st;
-    __shared__ int s_active[256];
+    __shared__ uint8_t s_active[256];
     __shared__ uint8_t s_highest[256];
-    __shared__ int s_nactive, s_high;
-    __shared__ unsigned long long s_b88[4], s_b96[4];
-    __shared__ int s_c104[4], s_c112[4];
+    __shared__ int s_high;
     __shared__ P19Features s_f;
     __shared__ double s_leaf[16];
     const int tid = threadIdx.x;
@@ -339,61 +339,69 @@ __global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
         const int center = static_cast<int>(item % a.centers);
         const uint32_t* perms = a.perms + static_cast<size_t>(world) * kGenCount * 64;
         const double* offs = a.offsets + static_cast<size_t>(world) * kGenCount * 3;
-        s_highest[tid] = 0xFF;  // no re-entry / not a candidate column
-        if (tid == 0) {
-            int cnt = 0;
-            for (int c = 0; c < 256; ++c)
-                if (a.upper_mask[static_cast<size_t>(item) * 256 + c]) s_active[cnt++] = c;
-            s_nactive = cnt;
-            s_high = 0;
-        }
-        __syncthreads();
-        const int nact = s_nactive;
-        const bool active = tid < nact;
-        const int col = active ? s_active[tid] : 0;
-        const double X = -28.0 + static_cast<double>((col >> 4) * 4 + a.center_dx[center] * 4);
-        const double Z = -28.0 + static_cast<double>((col & 15) * 4 + a.center_dz[center] * 4);
-        S0Column<0, 11> cc;
-        evaluate_cached_column<0, 11, false>(st, perms, offs, X, Z, active, cc);
-        const ColAcc<0, 11>& acc = cc.acc;
-        if (active) {
-            const size_t ci = static_cast<size_t>(item) * 256 + col;
-            ColumnShape cs;
-            cs.d5 = a.d5[ci];
-            cs.d7 = a.d7[ci];
-            uint32_t positive = static_cast<uint32_t>(a.upper_mask[ci]) << 11;
+        const uint32_t m4 = reinterpret_cast<const uint32_t*>(a.upper_mask + static_cast<size_t>(item) * 256)[tid];
+        int cnt = 0;
 #pragma unroll
-            for (int y = 0; y < 11; ++y)
-                if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0) positive |= 1u << y;
-            const int h = highest_reentry(positive);
-            s_highest[col] = h < 0 ? 0xFF : static_cast<uint8_t>(h);
-            if (h >= 11) atomicAdd(&s_high, 1);
+        for (int k = 0; k < 4; ++k) cnt += ((m4 >> (8 * k)) & 0xFFu) != 0u;
+        int incl = cnt;  // inclusive prefix count over the lanes
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (tid >= off) incl += v;
+        }
+        const int nact = __shfl(incl, 63);
+        int at = incl - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (((m4 >> (8 * k)) & 0xFFu) != 0u) s_active[at++] = static_cast<uint8_t>(4 * tid + k);
+        for (int c = tid; c < 256; c += 64) s_highest[c] = 0xFF;  // no re-entry / not a candidate column
+        if (tid == 0) s_high = 0;
+        __syncthreads();
+        for (int base = 0; base < nact; base += 64) {
+            const bool active = base + tid < nact;
+            const int col = active ? s_active[base + tid] : 0;
+            const double X = -28.0 + static_cast<double>((col >> 4) * 4 + a.center_dx[center] * 4);
+            const double Z = -28.0 + static_cast<double>((col & 15) * 4 + a.center_dz[center] * 4);
+            S0Column<0, 11> cc;
+            evaluate_cached_column<0, 11, false>(st, perms, offs, X, Z, active, cc);
+            const ColAcc<0, 11>& acc = cc.acc;
+            if (active) {
+                const size_t ci = static_cast<size_t>(item) * 256 + col;
+                ColumnShape cs;
+                cs.d5 = a.d5[ci];
+                cs.d7 = a.d7[ci];
+                uint32_t positive = static_cast<uint32_t>(a.upper_mask[ci]) << 11;
+#pragma unroll
+                for (int y = 0; y < 11; ++y)
+                    if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0) positive |= 1u << y;
+                const int h = highest_reentry(positive);
+                s_highest[col] = h < 0 ? 0xFF : static_cast<uint8_t>(h);
+                if (h >= 11) atomicAdd(&s_high, 1);
+            }
         }
         __syncthreads();
         const int high = s_high;
         if (tid == 0) a.high[item] = high;
-        if (high >= a.high_min) {  // buildMonsterFeatures (:1090-1114): one ballot per wave
-            const int h = s_highest[tid];
-            const bool valid = h != 0xFF;
-            const unsigned long long m88 = __ballot(valid && h >= 11);
-            const unsigned long long m96 = __ballot(valid && h >= 12);
-            const unsigned long long m104 = __ballot(valid && h >= 13);
-            const unsigned long long m112 = __ballot(valid && h >= 14);
-            if ((tid & 63) == 0) {
-                s_b88[tid >> 6] = m88;
-                s_b96[tid >> 6] = m96;
-                s_c104[tid >> 6] = __popcll(m104);
-                s_c112[tid >> 6] = __popcll(m112);
+        if (high >= a.high_min) {  // buildMonsterFeatures (:1090-1114): one ballot per 64 cells
+            unsigned long long b88[4], b96[4];
+            int c104 = 0, c112 = 0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int h = s_highest[64 * w + tid];
+                const bool valid = h != 0xFF;
+                b88[w] = __ballot(valid && h >= 11);
+                b96[w] = __ballot(valid && h >= 12);
+                c104 += __popcll(__ballot(valid && h >= 13));
+                c112 += __popcll(__ballot(valid && h >= 14));
             }
-            __syncthreads();
             if (tid == 0) {
                 P19Features f;
-                f.full_y88 = __popcll(s_b88[0]) + __popcll(s_b88[1]) + __popcll(s_b88[2]) + __popcll(s_b88[3]);
-                f.full_y96 = __popcll(s_b96[0]) + __popcll(s_b96[1]) + __popcll(s_b96[2]) + __popcll(s_b96[3]);
-                f.full_y104 = s_c104[0] + s_c104[1] + s_c104[2] + s_c104[3];
-                f.full_y112 = s_c112[0] + s_c112[1] + s_c112[2] + s_c112[3];
-                board_shape(s_b88, f.y88_largest, f.y88_width, f.y88_depth, f.y88_touches);
-                board_shape(s_b96, f.y96_largest, f.y96_width, f.y96_depth, f.y96_touches);
+                f.full_y88 = __popcll(b88[0]) + __popcll(b88[1]) + __popcll(b88[2]) + __popcll(b88[3]);
+                f.full_y96 = __popcll(b96[0]) + __popcll(b96[1]) + __popcll(b96[2]) + __popcll(b96[3]);
+                f.full_y104 = c104;
+                f.full_y112 = c112;
+                board_shape(b88, f.y88_largest, f.y88_width, f.y88_depth, f.y88_touches);
+                board_shape(b96, f.y96_largest, f.y96_width, f.y96_depth, f.y96_touches);
                 s_f = f;
             }
             __syncthreads();
@@ -420,17 +428,11 @@ __global__ __launch_bounds__(256) void s0_lower_kernel(S0Args a) {
     }
 }
 
-// Exact radius-7 coarse score of one item: the 61x61x17 sign grid (coarse_exact_core.hpp:
-// 218-386) built in LDS as one 61-bit z mask per (x, y) row, then scoreSigns (:83-169): the
-// largest 6-connected positive component with maxY >= 8 that touches neither the bottom layer
-// nor a side and has a positive cell below its lowest cell in some column.  Components are
-// grown by bit-parallel dilation.
-__global__ __launch_bounds__(256) void s0_coarse_kernel(S0Args a) {
+// The 61x61x17 coarse sign grid of one item (coarse_exact_core.hpp:218-386), built in LDS as
+// one 61-bit z mask per (x, y) row and written to coarse_bits[list index].
+__global__ __launch_bounds__(64, 2) void s0_coarse_signs_kernel(S0Args a) {
     __shared__ GenStage st;
     __shared__ unsigned long long rem[kCoarseRows];
-    __shared__ unsigned long long sig[kCoarseRows];
-    __shared__ unsigned long long comp[kCoarseRows];
-    __shared__ int s_changed, s_seed, s_cells, s_maxy, s_bottom, s_side, s_reentry;
     const int tid = threadIdx.x;
     const int64_t n = stage_items(a);
     for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
@@ -447,17 +449,53 @@ __global__ __launch_bounds__(256) void s0_coarse_kernel(S0Args a) {
             const int cx = active ? col / 61 : 0, cz = active ? col % 61 : 0;
             const double X = static_cast<double>(-28 + cx + a.center_dx[center] * 4);
             const double Z = static_cast<double>(-28 + cz + a.center_dz[center] * 4);
-            S0Column<0, 16> cc;
-            evaluate_cached_column<0, 16, true>(st, perms, offs, X, Z, active, cc);
-            const ColAcc<0, 16>& acc = cc.acc;
-            if (active) {
-                const ColumnShape cs = column_shape(cc.temp, cc.rain, acc.n4, acc.n5);
+            // the column shape (2D noise and climate) first, then the 3D sums of levels 0..7 and
+            // of levels 8..15 in one loop body: eight levels of three octave sums at a time, with
+            // nothing else live, fit two waves per SIMD
+            const OneWorld s{&st, perms, offs};
+            ColumnShape cs;
+            {
+                const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
+                const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
+                const double BX = X * 4.0 + 2.0, BZ = Z * 4.0 + 2.0;
+                double temp = 0.0, rain = 0.0;
+                colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain);
+                cs = column_shape(temp, rain, n4, n5);
+            }
+#pragma unroll 1
+            for (int y0 = 0; y0 < 16; y0 += 8) {
+                ColAcc<0, 8> acc;
+                colpar_noise3_n<8>(s, X, Z, active, acc, y0);
+                if (active) {
 #pragma unroll
-                for (int y = 0; y < 16; ++y)
-                    if (lattice_density(cs, y, acc.n1[y], acc.n2[y], acc.n3[y]) > 0.0)
-                        atomicOr(&rem[cx * 17 + y], 1ULL << cz);
+                    for (int k = 0; k < 8; ++k)
+                        if (lattice_density(cs, y0 + k, acc.n1[k], acc.n2[k], acc.n3[k]) > 0.0)
+                            atomicOr(&rem[cx * 17 + y0 + k], 1ULL << cz);
+                }
             }
         }
+        __syncthreads();
+        unsigned long long* bits = a.coarse_bits + static_cast<size_t>(it) * kCoarseRows;
+        for (int r = tid; r < kCoarseRows; r += blockDim.x) bits[r] = rem[r];
+        __syncthreads();
+    }
+}
+
+// scoreSigns (coarse_exact_core.hpp:83-169) of one item's sign grid: the largest 6-connected
+// positive component with maxY >= 8 that touches neither the bottom layer nor a side and has a
+// positive cell below its lowest cell in some column.  Components are grown by bit-parallel
+// dilation.
+__global__ __launch_bounds__(256) void s0_coarse_score_kernel(S0Args a) {
+    __shared__ unsigned long long rem[kCoarseRows];
+    __shared__ unsigned long long sig[kCoarseRows];
+    __shared__ unsigned long long comp[kCoarseRows];
+    __shared__ int s_changed, s_seed, s_cells, s_maxy, s_bottom, s_side, s_reentry;
+    const int tid = threadIdx.x;
+    const int64_t n = stage_items(a);
+    for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
+        const int64_t item = a.items[it];
+        const unsigned long long* bits = a.coarse_bits + static_cast<size_t>(it) * kCoarseRows;
+        for (int r = tid; r < kCoarseRows; r += blockDim.x) rem[r] = bits[r];
         __syncthreads();
         for (int r = tid; r < kCoarseRows; r += blockDim.x) sig[r] = rem[r];
         __syncthreads();

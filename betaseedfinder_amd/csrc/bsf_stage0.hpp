@@ -54,6 +54,7 @@ struct S0Args {
     double* score;               // [item] P19 score (quiet NaN when not evaluated)
     uint8_t* flags;              // [item] S0F_*
     P19Features* feats;          // [item]
+    unsigned long long* coarse_bits;  // [list index][kCoarseRows] coarse sign grids
 };
 
 // Record assembly of a batch: item = world * centers + centre.
@@ -76,7 +77,8 @@ __global__ void gen_cache_kernel(GenCacheArgs a);
 template <int MODE>
 __global__ void s0_upper_kernel(S0Args a);
 __global__ void s0_lower_kernel(S0Args a);
-__global__ void s0_coarse_kernel(S0Args a);
+__global__ void s0_coarse_signs_kernel(S0Args a);
+__global__ void s0_coarse_score_kernel(S0Args a);
 __global__ void s0_records_kernel(S0RecordArgs a);
 
 // nextInt magic numbers and the P19 tree table.

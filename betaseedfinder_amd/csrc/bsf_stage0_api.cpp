@@ -156,7 +156,7 @@ void stage0_batch(const int64_t* h_seeds, int count, const Profile& prof, int ce
     a.next_count = b.list_count + 2;
     {
         LaunchTimer t("stage0_lower", s);
-        hipLaunchKernelGGL(s0_lower_kernel, dim3(grid_for(s0_lower_kernel, 256, b.items)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(s0_lower_kernel, dim3(grid_for(s0_lower_kernel, 64, b.items)), dim3(64), 0, s, a);
         check(hipGetLastError(), "lower launch");
         t.stop(0);
     }
@@ -164,10 +164,13 @@ void stage0_batch(const int64_t* h_seeds, int count, const Profile& prof, int ce
     a.n_dev = b.list_count + 2;
     a.next_items = nullptr;
     a.next_count = nullptr;
+    a.coarse_bits = static_cast<unsigned long long*>(g_ctx.lattice.ensure(static_cast<size_t>(b.items) * kCoarseRows * 8));
     {
         LaunchTimer t("stage0_coarse", s);
-        hipLaunchKernelGGL(s0_coarse_kernel, dim3(grid_for(s0_coarse_kernel, 256, b.items)), dim3(256), 0, s, a);
-        check(hipGetLastError(), "coarse launch");
+        hipLaunchKernelGGL(s0_coarse_signs_kernel, dim3(grid_for(s0_coarse_signs_kernel, 64, b.items)), dim3(64), 0, s, a);
+        check(hipGetLastError(), "coarse signs launch");
+        hipLaunchKernelGGL(s0_coarse_score_kernel, dim3(grid_for(s0_coarse_score_kernel, 256, b.items)), dim3(256), 0, s, a);
+        check(hipGetLastError(), "coarse score launch");
         t.stop(0);
     }
     const int rb = research ? 34 : 16;
