@@ -285,6 +285,7 @@ int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t cx, int32_t cz
     return guarded_call([&] {
         require_init();
         if (count < 0 || (count > 0 && (!seeds || !out))) throw ArgError("bad density arguments");
+        if (!chunk_span_ok(cx, 1) || !chunk_span_ok(cz, 1)) throw ArgError("chunk outside coordinates -2^19 .. 2^19");
         if (count == 0) return;
         hipStream_t s = g_ctx.stream;
         const int64_t batch = int64_t(1) << 20;
@@ -304,6 +305,7 @@ int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t cx, i
     return guarded_call([&] {
         require_init();
         if (count < 0 || (count > 0 && (!d_seeds || !d_out))) throw ArgError("bad density arguments");
+        if (!chunk_span_ok(cx, 1) || !chunk_span_ok(cz, 1)) throw ArgError("chunk outside coordinates -2^19 .. 2^19");
         run_density(d_seeds, count, cx, cz, d_out, stream ? static_cast<hipStream_t>(stream) : g_ctx.stream);
     });
 }

@@ -467,9 +467,8 @@ void validate_island_params(const bsf_island_params* p) {
     if (!p) throw ArgError("null island params");
     if (p->chunks_x < 1 || p->chunks_z < 1 || p->chunks_x > 8 || p->chunks_z > 8)
         throw ArgError("island window must be 1..8 chunks on each axis");
-    const long long lim = 1LL << 24;
-    if (std::llabs(static_cast<long long>(p->chunk_x0)) > lim || std::llabs(static_cast<long long>(p->chunk_z0)) > lim)
-        throw ArgError("island window origin out of range");
+    if (!chunk_span_ok(p->chunk_x0, p->chunks_x) || !chunk_span_ok(p->chunk_z0, p->chunks_z))
+        throw ArgError("island window outside chunk coordinates -2^19 .. 2^19");
 }
 
 }  // namespace bsf

@@ -80,6 +80,14 @@ bool timing_enabled();
 
 void require_init();
 
+// Chunk coordinates the noise keeps inside int range: the octave-0 coordinate of noise2 /
+// noise3 is (4 chunk + i) * 684.412 + offset < 2^31 for |chunk| <= 2^19.  Beyond that Java's
+// (int) casts saturate (the Far Lands); windows and chunks there are refused.
+constexpr long long kMaxChunkCoord = 1LL << 19;
+inline bool chunk_span_ok(long long first, long long count) {
+    return first >= -kMaxChunkCoord && first + count - 1 <= kMaxChunkCoord;
+}
+
 extern std::mutex g_mutex;
 
 // Runs f under the library lock and maps exceptions to the C ABI's status codes.

@@ -32,9 +32,8 @@ constexpr int kHotSeedCoarse = 120, kHotSeedRadius = 16;     // :110-112
 void validate_window(const bsf_island_params& p) {
     if (p.chunks_x < 1 || p.chunks_z < 1 || p.chunks_x > kRegionMaxChunks || p.chunks_z > kRegionMaxChunks)
         throw ArgError("region window must be 1..49 chunks on each axis");
-    const long long lim = 1LL << 24;
-    if (std::llabs(static_cast<long long>(p.chunk_x0)) > lim || std::llabs(static_cast<long long>(p.chunk_z0)) > lim)
-        throw ArgError("region window origin out of range");
+    if (!chunk_span_ok(p.chunk_x0, p.chunks_x) || !chunk_span_ok(p.chunk_z0, p.chunks_z))
+        throw ArgError("region window outside chunk coordinates -2^19 .. 2^19");
 }
 
 template <class K>

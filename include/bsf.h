@@ -46,7 +46,8 @@ int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uin
 
 /* ---- terrain density grid (BetaTerrain173.generateTerrainNoise, BetaTerrain173.java:667-834)
  * For each seed, the 5x17x5 FP64 density lattice of chunk (chunk_x, chunk_z), index
- * (x*5 + z)*17 + y, bit-exact to Java.  out holds count*425 doubles. */
+ * (x*5 + z)*17 + y, bit-exact to Java.  out holds count*425 doubles.  chunk_x and chunk_z
+ * lie in -2^19 .. 2^19 (BSF_ERR_ARG otherwise). */
 int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t chunk_x, int32_t chunk_z, double* out);
 /* Same with device-resident seeds/output on a caller stream (hipStream_t as void*). */
 int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t chunk_x, int32_t chunk_z,
@@ -66,7 +67,9 @@ int bsf_scout_range_device(int64_t first_seed, int64_t count, double threshold,
  * block scan, IslandSearchEngine.java:2861-2977, with isMatch :3972-4003, floodFillFlat
  * :4096-4202, compareComponents :4079-4094) over a chunk window. --------------------- */
 typedef struct bsf_island_params {
-    int32_t chunk_x0, chunk_z0;      /* first chunk of the window */
+    int32_t chunk_x0, chunk_z0;      /* first chunk of the window; every chunk of a window (and of a
+                                        density grid) lies in -2^19 .. 2^19 on each axis, where the
+                                        main noise's lattice coordinate still fits a Java int */
     int32_t chunks_x, chunks_z;      /* window size in chunks, 1..8 each */
     int32_t min_blocks;              /* SearchSettings.minBlocks (minimum island volume) */
     int32_t min_columns;             /* SearchSettings.minColumns */
