@@ -271,6 +271,7 @@ def run_density_scout(bsf, args):
     s0_s = time.perf_counter() - t
     n_worlds = 32768 * args.stage0_batches
     regions = run_regions(bsf, args, worlds, batches)
+    chain = run_mega_chain(bsf, args, start + 32768 * (args.stage0_batches + 1))
     return {
         "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
                          "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
@@ -286,6 +287,7 @@ def run_density_scout(bsf, args):
                          round(s0_s * 1e3 / args.stage0_batches, 2),
                          "counters": dict(zip([n for n, _ in bsf.Stage0Counters._fields_], totals.tolist()))},
         "region_attempts": regions,
+        "mega_chain": chain,
     }
 
 
@@ -293,16 +295,51 @@ CENTER_X = [0, 15, 0, 15, -15, -15, 0, 15]  # production centres, chunk offsets
 CENTER_Z = [0, 0, 15, 15, 0, 15, -15, -15]
 
 
-def run_regions(bsf, args, worlds, batches):
-    """Region attempts (exact radius-7 scan, border verification, hot-seed rescan) of the
-    stage0stream survivors of the timed batches: P19 pass, no mega reject, coarse >= 85."""
+def region_items(worlds, batches, coarse_min=85):
+    """(seed, centre chunk x, centre chunk z, coarse score) of the records the Java loop sends to
+    region attempts (IslandSearchEngine.java:1366-1376, reject stage 0): P19 pass (given only
+    after the P20, upper and high gates), no mega-topology reject, coarse >= the hunter
+    threshold (SearchSettings.hunterCoarseMinCells, 85).  batches[b] holds worlds[b*32768:]."""
     import numpy as np
     items = []
     for b, recs in enumerate(batches):
         coarse = recs[:, :, 4].astype(np.int32) | (recs[:, :, 5].astype(np.int32) << 8)
-        ok = (recs[:, :, 3] == 1) & (recs[:, :, 7] == 0) & (coarse >= 85)
+        ok = (recs[:, :, 3] == 1) & (recs[:, :, 7] == 0) & (coarse >= coarse_min)
         for wi, ci in zip(*np.nonzero(ok)):
-            items.append((int(worlds[(b + 1) * 32768 + wi]), CENTER_X[ci], CENTER_Z[ci], int(coarse[wi, ci])))
+            items.append((int(worlds[b * 32768 + wi]), CENTER_X[ci], CENTER_Z[ci], int(coarse[wi, ci])))
+    return items
+
+
+def run_mega_chain(bsf, args, first_world):
+    """The MEGA production chain end to end: stage0stream batches (mega profile, lean 16-byte
+    records, 8 centres) and the region attempts of their survivors, timed together."""
+    import numpy as np
+    n_batches = args.stage0_batches
+    worlds = np.arange(first_world, first_world + 32768 * n_batches, dtype=np.int64)
+    bsf.stage0_records(worlds[:1024], "mega", False, centers=8)
+    settings = bsf.island_params(0, 0, 1, 1)
+    t = time.perf_counter()
+    batches = []
+    for b in range(n_batches):
+        recs, _ = bsf.stage0_records(worlds[b * 32768:(b + 1) * 32768], "mega", False, centers=8)
+        batches.append(recs)
+    t_s0 = time.perf_counter()
+    items = region_items(worlds, batches)
+    res = bsf.region_attempts(items, settings, 7) if items else []
+    total = time.perf_counter() - t
+    n = 32768 * n_batches
+    return {"workload": "%d stage0stream batches of 32768 worlds x 8 centres (mega profile, lean records) from "
+                        "splitmix64(0xB5F00002) & (2^48-1) + %d, then region attempts of their survivors "
+                        "(radius 7, SearchSettings defaults); host in/out" % (n_batches, first_world - (splitmix64(0xB5F00002) & MASK48)),
+            "worlds_per_s": round(n / total, 1), "regions_per_s": round(8 * n / total, 1),
+            "stage0_ms": round((t_s0 - t) * 1e3, 2), "region_ms": round((time.perf_counter() - t_s0) * 1e3, 2),
+            "survivors": len(items), "matches": sum(r["hit"] is not None for r in res)}
+
+
+def run_regions(bsf, args, worlds, batches):
+    """Region attempts (exact radius-7 scan, border verification, hot-seed rescan) of the
+    stage0stream survivors of the timed batches: P19 pass, no mega reject, coarse >= 85."""
+    items = region_items(worlds[32768:], batches)
     settings = bsf.island_params(0, 0, 1, 1)  # SearchSettings defaults: 1000 blocks, 100 columns, ...
     bsf.region_attempts(items[:2], settings, 7)
     t = time.perf_counter()

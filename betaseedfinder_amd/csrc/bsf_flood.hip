@@ -201,17 +201,20 @@ __device__ __forceinline__ bool comp_better(const CompStats& a, const CompStats&
     return areaA > areaB;
 }
 
-// One sweep along y of each x slice (up, then down; thread t takes slices t, t + blockDim.x,
-// ...): adds the blocks of rem that touch comp within the slice or in the neighbouring
-// slices' rows.  Returns whether it grew.
+// One sweep along y of the x slices x0 .. x1 (up, then down; thread t takes slices x0 + t,
+// x0 + t + blockDim.x, ...): adds the blocks of rem that touch comp within the slice or in the
+// neighbouring slices' rows.  Returns whether it grew.  With s_lo, a slice outside
+// [lo, hi] that gains blocks widens *s_lo / *s_hi.
 template <int W>
-__device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, int nx) {
+__device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, int nx, int x0, int x1,
+                       int* s_lo = nullptr, int* s_hi = nullptr, int lo = 0, int hi = 0) {
     int changed = 0;
-    for (int t = threadIdx.x; t < nx; t += blockDim.x) {
+    for (int t = x0 + threadIdx.x; t <= x1; t += blockDim.x) {
         unsigned long long* row = comp + static_cast<size_t>(t) * kRowStride * W;
         const unsigned long long* mrow = rem + static_cast<size_t>(t) * kRowStride * W;
         const unsigned long long* left = t > 0 ? row - kRowStride * W : nullptr;
         const unsigned long long* right = t < nx - 1 ? row + kRowStride * W : nullptr;
+        int grew = 0;
         Z<W> prev = zzero<W>();
         for (int y = 0; y < kLevels; ++y) {
             const Z<W> m = zload<W>(mrow + y * W);
@@ -222,7 +225,7 @@ __device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, 
             const Z<W> n = zfill(zand(nb, m), m);
             if (!zeq(n, c)) {
                 zstore(row + y * W, n);
-                changed = 1;
+                grew = 1;
             }
             prev = n;
         }
@@ -233,22 +236,32 @@ __device__ int sweep_y(unsigned long long* comp, const unsigned long long* rem, 
             const Z<W> n = zfill(zand(zor(c, prev), m), m);
             if (!zeq(n, c)) {
                 zstore(row + y * W, n);
-                changed = 1;
+                grew = 1;
             }
             prev = n;
         }
+        if (grew && s_lo && (t < lo || t > hi)) {
+            atomicMin(s_lo, t);
+            atomicMax(s_hi, t);
+        }
+        changed |= grew;
     }
     return changed;
 }
 
-// One sweep of thread y along x over its y level (forward, then back).
+// One sweep of thread y along x over its y level (forward, then back) over x0 .. nx - 1 and
+// back.  With s_lo (comp lies in slices x0 .. x1), the forward walk ends past x1 once nothing
+// propagates, the backward walk likewise below x0, and the slices reached widen *s_lo / *s_hi.
 template <int W>
-__device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, int nx) {
+__device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, int nx, int x0 = 0, int x1 = 0,
+                       int* s_lo = nullptr, int* s_hi = nullptr) {
     const int t = threadIdx.x;
     int changed = 0;
     if (t < kLevels) {
+        const bool bounded = s_lo != nullptr;
+        int xmin = x0, xmax = x1;
         Z<W> prev = zzero<W>();
-        for (int x = 0; x < nx; ++x) {
+        for (int x = bounded ? x0 : 0; x < nx; ++x) {
             const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
             const Z<W> m = zload<W>(rem + r);
             const Z<W> c = zload<W>(comp + r);
@@ -259,11 +272,13 @@ __device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, 
             if (!zeq(n, c)) {
                 zstore(comp + r, n);
                 changed = 1;
+                xmax = x > xmax ? x : xmax;
             }
             prev = n;
+            if (bounded && x >= xmax && !zany(n)) break;
         }
         prev = zzero<W>();
-        for (int x = nx - 1; x >= 0; --x) {
+        for (int x = bounded ? xmax : nx - 1; x >= 0; --x) {
             const size_t r = (static_cast<size_t>(x) * kRowStride + t) * W;
             const Z<W> m = zload<W>(rem + r);
             const Z<W> c = zload<W>(comp + r);
@@ -271,8 +286,14 @@ __device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, 
             if (!zeq(n, c)) {
                 zstore(comp + r, n);
                 changed = 1;
+                xmin = x < xmin ? x : xmin;
             }
             prev = n;
+            if (bounded && x <= xmin && !zany(n)) break;
+        }
+        if (bounded && changed) {
+            atomicMin(s_lo, xmin);
+            atomicMax(s_hi, xmax);
         }
     }
     return changed;
@@ -285,9 +306,28 @@ __device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, 
 template <int W>
 __device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
     for (;;) {
-        int changed = sweep_y<W>(comp, rem, nx);
+        int changed = sweep_y<W>(comp, rem, nx, 0, nx - 1);
         __syncthreads();
         changed |= sweep_x<W>(comp, rem, nx);
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
+// The same for a connected comp whose blocks lie in slices *s_lo .. *s_hi (a connected set's
+// x projection is an interval): each round sweeps y over that interval widened by one slice
+// and x from it outwards while blocks propagate, and widens the interval as comp grows.  A
+// round without change has looked at every slice that holds comp or borders it, so comp is
+// closed.
+template <int W>
+__device__ void grow_component_in(unsigned long long* comp, const unsigned long long* rem, int nx, int* s_lo,
+                                  int* s_hi) {
+    for (;;) {
+        const int lo = *s_lo, hi = *s_hi;
+        __syncthreads();  // every thread has the same slices before any widens them
+        int changed = sweep_y<W>(comp, rem, nx, lo > 0 ? lo - 1 : 0, hi < nx - 1 ? hi + 1 : nx - 1, s_lo, s_hi,
+                                 lo, hi);
+        __syncthreads();
+        changed |= sweep_x<W>(comp, rem, nx, *s_lo, *s_hi, s_lo, s_hi);
         if (!__syncthreads_or(changed)) break;
     }
 }
@@ -317,6 +357,7 @@ struct FloodShared {
     int have_best;
     unsigned long long zor[W];
     int total;
+    int lo, hi;                      // x slices of the growing component
     unsigned long long starts[256];  // candidate-start levels of a slice of sample columns
 };
 
@@ -366,7 +407,7 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
     // 2a. one sweep along y from the contacts already reaches most grounded stone; what is
     //     left bounds the floating stone from above, and fewer than minBlocks floating
     //     blocks cannot hold a match
-    sweep_y<W>(comp, rem, nx);
+    sweep_y<W>(comp, rem, nx, 0, nx - 1);
     __syncthreads();
     if (count_outside<W>(rem, comp, rows, &sh.total) < p.min_blocks) return false;
     // 2b. the ground components in full
@@ -380,6 +421,7 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
 #pragma unroll
         for (int i = 0; i < W; ++i) left.v[i] &= ~c.v[i];
         zstore(rem + static_cast<size_t>(r) * W, left);
+        zstore(comp + static_cast<size_t>(r) * W, zzero<W>());
     }
     __syncthreads();
 
@@ -427,11 +469,15 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
                 todo &= todo - 1;
                 // stone not yet visited (visited or grounded blocks are no longer in rem)
                 if ((rem[(static_cast<size_t>(x) * kRowStride + yy) * W + zw] & bitz) == 0) continue;
-                for (int r = tid; r < rows * W; r += blockDim.x) comp[r] = 0;
+                // comp is all clear here: each component's slices are cleared after its statistics
+                if (tid == 0) {
+                    comp[(static_cast<size_t>(x) * kRowStride + yy) * W + zw] = bitz;
+                    sh.lo = x;
+                    sh.hi = x;
+                }
                 __syncthreads();
-                if (tid == 0) comp[(static_cast<size_t>(x) * kRowStride + yy) * W + zw] = bitz;
-                __syncthreads();
-                grow_component<W>(comp, rem, nx);
+                grow_component_in<W>(comp, rem, nx, &sh.lo, &sh.hi);
+                const int xlo = sh.lo, xhi = sh.hi;
                 // statistics of the component
                 if (tid == 0) {
                     sh.cs.blocks = 0;
@@ -450,7 +496,7 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
                     for (int i = 0; i < W; ++i) sh.zor[i] = 0;
                 }
                 __syncthreads();
-                for (int rx = tid; rx < nx; rx += blockDim.x) {
+                for (int rx = xlo + tid; rx <= xhi; rx += blockDim.x) {
                     Z<W> col = zzero<W>();
                     int blk = 0, ymin = 0x7fffffff, ymax = -0x7fffffff;
                     for (int ry = 0; ry < kLevels; ++ry) {
@@ -485,7 +531,12 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
                         if (rx == 0 || rx == nx - 1) atomicOr(&sh.cs.side, 1);
                     }
                 }
-                for (int r = tid; r < rows * W; r += blockDim.x) rem[r] &= ~comp[r];
+                __syncthreads();  // the statistics have read comp
+                for (size_t r = static_cast<size_t>(xlo) * kRowStride * W + tid;
+                     r < static_cast<size_t>(xhi + 1) * kRowStride * W; r += blockDim.x) {
+                    rem[r] &= ~comp[r];
+                    comp[r] = 0;
+                }
                 __syncthreads();
                 CompStats cs = sh.cs;
                 int lo = -1, hi = -1;  // lowest and highest z of the component
@@ -608,6 +659,7 @@ template __global__ void region_flood_kernel<1>(RegionArgs, int);
 template __global__ void region_flood_kernel<2>(RegionArgs, int);
 template __global__ void region_flood_kernel<4>(RegionArgs, int);
 template __global__ void region_flood_kernel<7>(RegionArgs, int);
+template __global__ void region_flood_kernel<9>(RegionArgs, int);
 template __global__ void region_flood_kernel<13>(RegionArgs, int);
 
 template __global__ void pass_flood_kernel<1, false>(EngineArgs, int);

@@ -22,12 +22,13 @@ namespace bsf {
 constexpr int kRegionMaxChunks = 49;  // (2 * 24 + 1) chunks per axis
 constexpr int kRegionItemCols = 64;   // lattice or climate columns per work item
 
-// Flood word-count buckets (64-bit words per z row): windows up to 64, 128, 256, 448 and
-// 832 blocks deep in z.
-constexpr int kRegionBuckets = 5;
+// Flood word-count buckets (64-bit words per z row): windows up to 64, 128, 256, 448, 576
+// and 832 blocks deep in z (radius 7 is 240 blocks, radius 13 432, radius 16 528, radius 24
+// 784).
+constexpr int kRegionBuckets = 6;
 
 inline int region_bucket_words(int bucket) {
-    return bucket == 0 ? 1 : bucket == 1 ? 2 : bucket == 2 ? 4 : bucket == 3 ? 7 : 13;
+    return bucket == 0 ? 1 : bucket == 1 ? 2 : bucket == 2 ? 4 : bucket == 3 ? 7 : bucket == 4 ? 9 : 13;
 }
 inline int region_bucket_of(int chunks_z) {
     const int words = (16 * chunks_z + 63) / 64;

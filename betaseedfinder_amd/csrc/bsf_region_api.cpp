@@ -51,6 +51,7 @@ void launch_flood(int bucket, int grid, RegionArgs& a, hipStream_t s) {
         case 1: hipLaunchKernelGGL(region_flood_kernel<2>, dim3(grid), dim3(256), 0, s, a, bucket); break;
         case 2: hipLaunchKernelGGL(region_flood_kernel<4>, dim3(grid), dim3(256), 0, s, a, bucket); break;
         case 3: hipLaunchKernelGGL(region_flood_kernel<7>, dim3(grid), dim3(256), 0, s, a, bucket); break;
+        case 4: hipLaunchKernelGGL(region_flood_kernel<9>, dim3(grid), dim3(256), 0, s, a, bucket); break;
         default: hipLaunchKernelGGL(region_flood_kernel<13>, dim3(grid), dim3(256), 0, s, a, bucket); break;
     }
 }
@@ -62,6 +63,7 @@ int flood_resident(int bucket) {
         case 1: return resident_grid(region_flood_kernel<2>, 256, 2);
         case 2: return resident_grid(region_flood_kernel<4>, 256, 2);
         case 3: return resident_grid(region_flood_kernel<7>, 256, 2);
+        case 4: return resident_grid(region_flood_kernel<9>, 256, 2);
         default: return resident_grid(region_flood_kernel<13>, 256, 2);
     }
 }

@@ -18,14 +18,15 @@ def _parallel(fn, args):
 
 def test_window_scan_matches_oracle(bsf, oracle):
     # radius-7 windows around two centres; narrow, long and deep windows in every flood word
-    # bucket (1, 2, 4, 7 and 13 words per z row)
+    # bucket (1, 2, 4, 7, 9 and 13 words per z row)
     cases = []
     for seed in (38, 86, 108, 153, 229, 285, 307, 341):
         cases.append((seed, (-7, -7, 15, 15)))
     for seed in (59, 146):
         cases.append((seed, (8, -22, 15, 15)))
     cases += [(285, (-3, -24, 1, 49)), (86, (-24, 2, 49, 3)), (229, (-10, -10, 21, 21)),
-              (108, (-4, -16, 9, 33)), (153, (-1, -1, 3, 5)), (307, (-8, -8, 17, 8))]
+              (108, (-4, -16, 9, 33)), (153, (-1, -1, 3, 5)), (307, (-8, -8, 17, 8)), (38, (-3, -16, 5, 33)),
+              (86, (-3, -16, 5, 33)), (4, (-3, -16, 5, 33)), (146, (-2, -16, 4, 36))]
     seeds = [c[0] for c in cases]
     wins = [bsf.island_params(*w, **LOW) for _, w in cases]
     got = bsf.window_scan(seeds, wins)
