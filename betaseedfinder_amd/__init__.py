@@ -37,4 +37,13 @@ from .api import (  # noqa: F401
     scout_range,
     stage0_records,
     window_scan,
+    deterministic_seeds,
+    island_sequence,
+    region_dict,
+    region_struct,
+    RunSettings,
+    RunAttempt,
+    RunWriter,
+    run_settings,
+    hunt,
 )

@@ -134,6 +134,25 @@ def island_seeds(seeds, params, cap=1 << 16):
     return [out[i].as_tuple() for i in range(n.value)]
 
 
+def deterministic_seeds(sequence_seed, first_attempt, count):
+    """Seeds of attempts first_attempt .. first_attempt + count - 1 (SplitMix64), as int64."""
+    init()
+    out = np.empty(count, dtype=np.int64)
+    _check(lib().bsf_deterministic_seeds(ctypes.c_uint64(sequence_seed & (2**64 - 1)), ctypes.c_int64(first_attempt),
+                                         ctypes.c_int64(count), out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def island_sequence(sequence_seed, first_attempt, count, params, cap=1 << 16):
+    init()
+    out = (IslandHit * cap)()
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_island_sequence(ctypes.c_uint64(sequence_seed & (2**64 - 1)), ctypes.c_int64(first_attempt),
+                                     ctypes.c_int64(count), ctypes.byref(params), out, ctypes.c_int64(cap),
+                                     ctypes.byref(n)))
+    return [out[i].as_tuple() for i in range(n.value)]
+
+
 def island_stage1_range(first_seed, count, params, cap=1 << 24):
     init()
     out = np.empty(cap, dtype=np.int64)
@@ -175,7 +194,9 @@ class RegionItem(ctypes.Structure):
 class RegionResult(ctypes.Structure):
     _fields_ = [("hit", IslandHit), ("found", ctypes.c_int32), ("radius_used", ctypes.c_int32),
                 ("normal_found", ctypes.c_int32), ("hot_scan_ran", ctypes.c_int32), ("hot_won", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("normal_hit", IslandHit), ("normal_kept", ctypes.c_int32),
+                ("normal_radius", ctypes.c_int32), ("hot_hit", IslandHit), ("hot_found", ctypes.c_int32),
+                ("hot_radius", ctypes.c_int32)]
 
 
 def window_scan(seeds, windows):
@@ -203,9 +224,102 @@ def region_attempts(items, settings, radius=7):
     arr = (RegionItem * max(1, n))(*[RegionItem(int(sd), int(cx), int(cz), int(c), 0) for sd, cx, cz, c in items])
     out = (RegionResult * max(1, n))()
     _check(lib().bsf_region_attempts(arr, ctypes.c_int64(n), ctypes.byref(settings), ctypes.c_int32(radius), out))
-    return [{"hit": out[i].hit.as_tuple() if out[i].found else None, "radius_used": out[i].radius_used,
-             "normal_found": out[i].normal_found, "hot_scan_ran": out[i].hot_scan_ran, "hot_won": out[i].hot_won}
-            for i in range(n)]
+    return [region_dict(out[i]) for i in range(n)]
+
+
+def region_dict(r):
+    """The fields of a region result as a dict: the resolved hit (tuple or None), radius_used,
+    normal_found, hot_scan_ran, hot_won, the verified normal and hot results and their radii."""
+    return {"hit": r.hit.as_tuple() if r.found else None, "radius_used": r.radius_used,
+            "normal_found": r.normal_found, "hot_scan_ran": r.hot_scan_ran, "hot_won": r.hot_won,
+            "normal": r.normal_hit.as_tuple() if r.normal_kept else None,
+            "normal_radius": r.normal_radius if r.normal_kept else 0,
+            "hot": r.hot_hit.as_tuple() if r.hot_found else None,
+            "hot_radius": r.hot_radius if r.hot_found else 0}
+
+
+def region_struct(d):
+    """A RegionResult from a region dict (the inverse of region_dict)."""
+    r = RegionResult()
+    if d["hit"] is not None:
+        r.found = 1
+        r.hit = IslandHit(*d["hit"])
+        r.radius_used = d["radius_used"]
+    r.normal_found = d["normal_found"]
+    r.hot_scan_ran = d["hot_scan_ran"]
+    r.hot_won = d["hot_won"]
+    if d.get("normal") is not None:
+        r.normal_kept = 1
+        r.normal_hit = IslandHit(*d["normal"])
+        r.normal_radius = d["normal_radius"]
+    if d.get("hot") is not None:
+        r.hot_found = 1
+        r.hot_hit = IslandHit(*d["hot"])
+        r.hot_radius = d["hot_radius"]
+    return r
+
+
+# ---------------------------------------------------------------------------
+# run output files and the headless hunt
+# ---------------------------------------------------------------------------
+class RunSettings(ctypes.Structure):
+    _fields_ = [("profile", ctypes.c_int32), ("research", ctypes.c_int32), ("top_keep", ctypes.c_int32),
+                ("chunk_radius", ctypes.c_int32), ("coarse_min", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("regions", ctypes.c_int64), ("sequence_seed", ctypes.c_uint64), ("thresholds", IslandParams)]
+
+
+class RunAttempt(ctypes.Structure):
+    _fields_ = [("attempt", ctypes.c_int64), ("seed", ctypes.c_int64), ("center_chunk_x", ctypes.c_int32),
+                ("center_chunk_z", ctypes.c_int32), ("p20", ctypes.c_int32), ("upper", ctypes.c_int32),
+                ("high", ctypes.c_int32), ("coarse", ctypes.c_int32), ("p19_pass", ctypes.c_int32),
+                ("p19_extreme", ctypes.c_int32), ("p19_score", ctypes.c_double)]
+
+
+class HuntSummary(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("worlds", "regions", "survivors", "matches", "duplicates",
+                                              "top_updates")] + [("seconds", ctypes.c_double)]
+
+
+def run_settings(profile="general", research=False, top_keep=50, chunk_radius=7, coarse_min=85, regions=0,
+                 sequence_seed=0, thresholds=None):
+    """SearchSettings of a headless hunt; thresholds: IslandParams (SearchSettings defaults)."""
+    return RunSettings(PROFILE_NAMES[profile], int(research), top_keep, chunk_radius, coarse_min, 0, regions,
+                       sequence_seed & (2**64 - 1), thresholds if thresholds is not None else island_params(0, 0, 1, 1))
+
+
+class RunWriter:
+    """The run folder of a hunt fed attempt by attempt (host code; no GPU needed).  attempt:
+    (attempt, seed, centre x, centre z, p20, upper, high, coarse, p19 pass, p19 extreme,
+    p19 score); result: a region dict or RegionResult."""
+
+    def __init__(self, root, settings, label="", stamp=None, start_ms=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().bsf_run_open(os.fsencode(root), label.encode(), stamp.encode() if stamp else None,
+                                  ctypes.c_int64(start_ms), ctypes.byref(settings), ctypes.byref(self._h)))
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().bsf_run_dir(self._h, buf, ctypes.c_int64(4096)))
+        self.dir = buf.value.decode()
+
+    def add(self, attempt, result):
+        a = RunAttempt(*attempt)
+        r = region_struct(result) if isinstance(result, dict) else result
+        u = ctypes.c_int32(0)
+        _check(lib().bsf_run_add(self._h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(u)))
+        return bool(u.value)
+
+    def close(self, status="FINISHED", checked=0, end_ms=0):
+        _check(lib().bsf_run_close(self._h, status.encode(), ctypes.c_int64(checked), ctypes.c_int64(end_ms)))
+        self._h = None
+
+
+def hunt(root, settings, label="", stamp=None):
+    """The headless GPU hunt into a run folder: (summary dict, run folder)."""
+    init()
+    out = HuntSummary()
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().bsf_hunt(os.fsencode(root), label.encode(), stamp.encode() if stamp else None,
+                          ctypes.byref(settings), ctypes.byref(out), buf, ctypes.c_int64(4096)))
+    return {n: getattr(out, n) for n, _ in HuntSummary._fields_}, buf.value.decode()
 
 
 # ---------------------------------------------------------------------------

@@ -32,7 +32,7 @@ DEVICE = [
 ]
 
 LIB_SOURCES = ["bsf_kernels.hip", "bsf_stage0.hip", "bsf_island.hip", "bsf_flood.hip", "bsf_region.hip", "bsf_api.cpp",
-               "bsf_engine.cpp", "bsf_island_api.cpp", "bsf_server.cpp", "bsf_stage0_api.cpp", "bsf_region_api.cpp"]
+               "bsf_engine.cpp", "bsf_island_api.cpp", "bsf_server.cpp", "bsf_stage0_api.cpp", "bsf_region_api.cpp", "bsf_run.cpp"]
 WORKER_SOURCES = ["worker_main.cpp"]
 
 

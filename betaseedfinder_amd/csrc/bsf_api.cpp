@@ -310,6 +310,34 @@ int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t cx, i
     });
 }
 
+int bsf_deterministic_seeds_device(uint64_t sequence_seed, int64_t first_attempt, int64_t count, int64_t* d_out,
+                                   void* stream) {
+    return guarded_call([&] {
+        require_init();
+        if (count < 0 || (count > 0 && !d_out)) throw ArgError("bad seed arguments");
+        if (count == 0) return;
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g_ctx.stream;
+        const int grid = static_cast<int>(std::min<int64_t>((count + 255) / 256, int64_t(g_ctx.cu_count) * 8));
+        hipLaunchKernelGGL(splitmix_seeds_kernel, dim3(grid), dim3(256), 0, s, sequence_seed, first_attempt, count, d_out);
+        check(hipGetLastError(), "seed kernel launch");
+    });
+}
+
+int bsf_deterministic_seeds(uint64_t sequence_seed, int64_t first_attempt, int64_t count, int64_t* out) {
+    return guarded_call([&] {
+        require_init();
+        if (count < 0 || (count > 0 && !out)) throw ArgError("bad seed arguments");
+        if (count == 0) return;
+        hipStream_t s = g_ctx.stream;
+        int64_t* d = static_cast<int64_t*>(g_ctx.seeds.ensure(static_cast<size_t>(count) * 8));
+        const int grid = static_cast<int>(std::min<int64_t>((count + 255) / 256, int64_t(g_ctx.cu_count) * 8));
+        hipLaunchKernelGGL(splitmix_seeds_kernel, dim3(grid), dim3(256), 0, s, sequence_seed, first_attempt, count, d);
+        check(hipGetLastError(), "seed kernel launch");
+        check(hipMemcpyAsync(out, d, static_cast<size_t>(count) * 8, hipMemcpyDeviceToHost, s), "D2H seeds");
+        check(hipStreamSynchronize(s), "seed sync");
+    });
+}
+
 int bsf_scout_range(int64_t first, int64_t count, double thr, int64_t* out, int64_t cap, int64_t* n_found) {
     return guarded_call([&] {
         require_init();

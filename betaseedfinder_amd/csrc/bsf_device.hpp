@@ -22,6 +22,17 @@ constexpr uint64_t kLcgMul = 0x5DEECE66DULL;
 constexpr uint64_t kLcgAdd = 0xBULL;
 constexpr uint64_t kLcgMask = (1ULL << 48) - 1;
 
+// Seed of attempt `attempt` of a deterministic run: SplitMix64 of sequence_seed + attempt * phi
+// (IslandSearchEngine.deterministicSeedForAttempt, IslandSearchEngine.java:2551-2556; the
+// worker's splitMixDeterministicSeed, p20_exact_math.hpp:420-425).  Java numbers world
+// attempts from 1 (:1321-1325), the worker's benchmark modes from 0.
+__host__ __device__ __forceinline__ uint64_t splitmix_seed(uint64_t sequence_seed, uint64_t attempt) {
+    uint64_t z = sequence_seed + attempt * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 // Division table for Random.nextInt(bound), bound in [1, 256] (filled by the host).
 // For a non-power-of-two bound d with 2^k < d < 2^(k+1): m = ceil(2^(32+k) / d) and
 // floor(n / d) == umulhi(n, m) >> k for every n < 2^31 (error term n*(m - 2^(32+k)/d)

@@ -151,7 +151,7 @@ bool IslandEngine::has_work() const {
 }
 
 int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64_t* d_seeds, int64_t count,
-                        const bsf_island_params& p, const Tuning& tuning) {
+                        const bsf_island_params& p, const Tuning& tuning, const uint64_t* sequence_seed) {
     int slot = -1;
     for (int i = 0; i < cfg_jobs_; ++i)
         if (!host_jobs_[i].used) {
@@ -162,6 +162,8 @@ int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64
     IslandJob j{};
     j.first_seed = first_seed;
     j.seeds = d_seeds;
+    j.seed_source = sequence_seed ? kSeedSplitMix : (d_seeds ? kSeedList : kSeedRange);
+    j.sequence_seed = sequence_seed ? *sequence_seed : 0;
     j.count = count;
     j.cursor = 0;
     j.hits = 0;

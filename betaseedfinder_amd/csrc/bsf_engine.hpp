@@ -55,9 +55,10 @@ class IslandEngine {
     int free_slots() const;
 
     // Puts a job into a free job-table slot (returns the slot, -1 when the table is full).
-    // d_seeds: device seed list that outlives the job (or nullptr for the range).
+    // d_seeds: device seed list that outlives the job (or nullptr for the range).  With
+    // sequence_seed, seed i is splitmix_seed(*sequence_seed, first_seed + i).
     int admit(int64_t id, int32_t tag, int64_t first_seed, const int64_t* d_seeds, int64_t count,
-              const bsf_island_params& p, const Tuning& tuning);
+              const bsf_island_params& p, const Tuning& tuning, const uint64_t* sequence_seed = nullptr);
     // Returns a finished job's slot to the free pool.
     void retire(int slot);
     const HostJob& job(int slot) const { return host_jobs_[slot]; }

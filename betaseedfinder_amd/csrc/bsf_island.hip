@@ -196,7 +196,9 @@ __device__ __forceinline__ const PassSeg& seg_of_slot(const EngineArgs& a, int64
 }
 
 __device__ __forceinline__ int64_t job_seed(const IslandJob& j, int64_t i) {
-    return j.seeds != nullptr ? j.seeds[i] : j.first_seed + i;
+    if (j.seed_source == kSeedSplitMix)
+        return static_cast<int64_t>(splitmix_seed(j.sequence_seed, static_cast<uint64_t>(j.first_seed + i)));
+    return j.seed_source == kSeedList ? j.seeds[i] : j.first_seed + i;
 }
 
 // ---------------------------------------------------------------------------

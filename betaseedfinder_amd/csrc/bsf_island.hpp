@@ -21,9 +21,12 @@
 namespace bsf {
 
 // One search job in the device job table.
+// Where a job's seeds come from.
+enum : int32_t { kSeedRange = 0, kSeedList = 1, kSeedSplitMix = 2 };
+
 struct IslandJob {
-    int64_t first_seed;        // seeds first_seed + i (when seeds == nullptr)
-    const int64_t* seeds;      // explicit device seed list, or nullptr
+    int64_t first_seed;        // range: seeds first_seed + i; SplitMix: the first attempt
+    const int64_t* seeds;      // kSeedList: explicit device seed list
     int64_t count;             // seeds in the job
     unsigned long long cursor; // seeds handed to passes so far (pass_build)
     unsigned long long hits;   // hits found so far (flood kernels)
@@ -32,8 +35,11 @@ struct IslandJob {
     int32_t worlds_per_wave;   // Stage-1 worlds per 64-lane wave (1 .. kMaxWorldsPerWave)
     int32_t gate;              // 1: Stage-1 candidate gate; 0: every seed goes to Stage 2
     int32_t flood_class;       // kFloodClass* below
-    int32_t pad[2];
+    int32_t seed_source;       // kSeed*
+    int32_t pad;
+    unsigned long long sequence_seed;  // kSeedSplitMix: splitmix_seed(sequence_seed, first_seed + i)
 };
+static_assert(sizeof(IslandJob) == 120, "job-table entry size (server state memory is counted in entries)");
 
 // A contiguous run of one job's seeds inside a pass.
 struct PassSeg {

@@ -32,7 +32,7 @@ void finish_hits(std::vector<bsf_island_hit>& hits, bsf_island_hit* out, int64_t
 
 // One job through the engine, run to completion.
 void run_one(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p, const Tuning& t,
-             std::vector<bsf_island_hit>* hits, std::vector<int64_t>* stage1) {
+             std::vector<bsf_island_hit>* hits, std::vector<int64_t>* stage1, const uint64_t* sequence_seed = nullptr) {
     if (count == 0) return;
     ensure_engine();
     IslandEngine& e = engine();
@@ -42,7 +42,7 @@ void run_one(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_isl
         check(hipMemcpy(d, h_seeds, static_cast<size_t>(count) * sizeof(int64_t), hipMemcpyHostToDevice), "H2D seeds");
         d_seeds = d;
     }
-    const int slot = e.admit(0, 0, first, d_seeds, count, p, t);
+    const int slot = e.admit(0, 0, first, d_seeds, count, p, t, sequence_seed);
     if (slot < 0) throw CapacityError("island job table full");
     e.drive(stage1 != nullptr, [&](IslandEngine::PassResult& r) {
         if (hits) hits->insert(hits->end(), r.hits.begin(), r.hits.end());
@@ -80,6 +80,18 @@ int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_param
             throw ArgError("bad island arguments");
         std::vector<bsf_island_hit> hits;
         run_one(seeds, 0, count, *p, tuning_for(*p), &hits, nullptr);
+        finish_hits(hits, out, cap, n_found);
+    });
+}
+
+int bsf_island_sequence(uint64_t sequence_seed, int64_t first_attempt, int64_t count, const bsf_island_params* p,
+                        bsf_island_hit* out, int64_t cap, int64_t* n_found) {
+    return guarded_call([&] {
+        require_init();
+        validate_island_params(p);
+        if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad island arguments");
+        std::vector<bsf_island_hit> hits;
+        run_one(nullptr, first_attempt, count, *p, tuning_for(*p), &hits, nullptr, &sequence_seed);
         finish_hits(hits, out, cap, n_found);
     });
 }

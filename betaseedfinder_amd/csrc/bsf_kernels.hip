@@ -212,6 +212,14 @@ __global__ __launch_bounds__(64) void lattice_kernel(LatticeArgs a) {
 
 template __global__ void lattice_kernel<0, 17, MODE_DENSITY>(LatticeArgs);
 
+__global__ __launch_bounds__(256) void splitmix_seeds_kernel(uint64_t sequence_seed, int64_t first_attempt,
+                                                             int64_t count, int64_t* out) {
+    const int64_t step = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += step)
+        out[i] = static_cast<int64_t>(
+                splitmix_seed(sequence_seed, static_cast<uint64_t>(first_attempt) + static_cast<uint64_t>(i)));
+}
+
 // ---------------------------------------------------------------------------
 // Stage-0 scout: first terrain generator, octave 0, chunk (0,0) 5x17x5, S = sum.
 // ---------------------------------------------------------------------------

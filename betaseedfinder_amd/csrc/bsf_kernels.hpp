@@ -44,6 +44,8 @@ struct ScoutArgs {
 template <int Y0, int Y1, int MODE>
 __global__ void lattice_kernel(LatticeArgs a);
 __global__ void scout_kernel(ScoutArgs a);
+// out[i] = splitmix_seed(sequence_seed, first_attempt + i), i < count.
+__global__ void splitmix_seeds_kernel(uint64_t sequence_seed, int64_t first_attempt, int64_t count, int64_t* out);
 
 // Uploads the nextInt division table into the kernels' constant memory (current device).
 hipError_t upload_tables();

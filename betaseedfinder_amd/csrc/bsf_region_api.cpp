@@ -328,12 +328,20 @@ int bsf_region_attempts(const bsf_region_item* items, int64_t count, const bsf_i
             if (c.have) {
                 r.hit = c.hit;
                 r.radius_used = c.radius;
+                r.normal_hit = c.hit;
+                r.normal_kept = 1;
+                r.normal_radius = c.radius;
             }
         }
         for (size_t j = 0; j < hot_items.size(); ++j) {
             const Chain& h = chains[n + j];
             bsf_region_result& r = out[h.item];
             r.hot_scan_ran = 1;
+            if (h.have) {
+                r.hot_hit = h.hit;
+                r.hot_found = 1;
+                r.hot_radius = h.radius;
+            }
             if (h.have && (!r.found || hit_better(h.hit, r.hit))) {
                 r.hit = h.hit;
                 r.radius_used = h.radius;

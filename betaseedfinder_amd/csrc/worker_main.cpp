@@ -17,6 +17,10 @@
 //   region [capacity=4096] [radius=7] [minBlocks minColumns minY minWidth minDepth minAvgThickness]
 //          magic REGION01; request LE u32 count, count x (i64 seed, i32 centre chunk x, i32 centre
 //          chunk z, i32 coarse score); response LE u32 count, count x 56-byte results
+//   hunt <output_root> <regions> [general|mega|record60|record80] [research|lean] [sequence_seed=0]
+//        [label] [topResultsToKeep=50] [hunterCoarseMinCells=85]
+//          a headless hunt (bsf_hunt): deterministic world seeds, stage0stream, region attempts,
+//          and the Java search's run folder under <output_root>/runs; summary lines on stderr
 #include <cinttypes>
 #include <cstdint>
 #include <cstdio>
@@ -308,6 +312,53 @@ int region_mode(int argc, char** argv) {
     return 0;
 }
 
+// The Java GPU hunt without the Java: runGpuStage0Pipeline + processGpuRegionAttempt
+// (IslandSearchEngine.java:1206-1480, 1878-1971) and the run output files (:4230-5361).
+int hunt_mode(int argc, char** argv) {
+    if (argc < 4) throw WorkerError("hunt needs <output_root> <regions>");
+    bsf_run_settings s{};
+    char* end = nullptr;
+    s.regions = std::strtoll(argv[3], &end, 10);
+    if (!argv[3][0] || *end || s.regions < 0) throw WorkerError(std::string("invalid regions: ") + argv[3]);
+    const std::string profile = argc > 4 ? argv[4] : "mega";
+    int pid = -1;
+    for (int i = 0; i < 4; ++i)
+        if (profile == kProfiles[i].arg) pid = i;
+    if (pid < 0) throw WorkerError("hunt profile must be general, mega, record60, or record80");
+    const std::string telemetry = argc > 5 ? argv[5] : "lean";
+    if (telemetry != "research" && telemetry != "lean") throw WorkerError("hunt telemetry must be research or lean");
+    s.profile = pid;
+    s.research = telemetry == "research" ? 1 : 0;
+    if (argc > 6) {
+        s.sequence_seed = std::strtoull(argv[6], &end, 0);
+        if (!argv[6][0] || *end) throw WorkerError(std::string("invalid sequence seed: ") + argv[6]);
+    }
+    const char* label = argc > 7 ? argv[7] : "";
+    s.top_keep = argc > 8 ? parse_int(argv[8], "topResultsToKeep") : 50;
+    s.coarse_min = argc > 9 ? parse_int(argv[9], "hunterCoarseMinCells") : 85;
+    s.chunk_radius = 7;
+    // SearchSettings defaults (SearchSettings.java:39-45)
+    s.thresholds.min_blocks = 1000;
+    s.thresholds.min_columns = 100;
+    s.thresholds.min_y_for_match = 60;
+    s.thresholds.min_width = 10;
+    s.thresholds.min_depth = 10;
+    s.thresholds.min_avg_thickness = 2.0;
+    print_device();
+    std::fprintf(stderr, "Headless hunt | profile=%s | regions=%" PRId64 " | 8 regions/world | sequence=%" PRIu64 "\n",
+                 kProfiles[pid].arg, s.regions, s.sequence_seed);
+    bsf_hunt_summary sum{};
+    char dir[4096];
+    check(bsf_hunt(argv[2], label, nullptr, &s, &sum, dir, sizeof(dir)), "hunt");
+    std::fprintf(stderr,
+                 "Hunt finished | worlds=%" PRId64 " | regions=%" PRId64 " | survivors=%" PRId64 " | matches=%" PRId64
+                 " | duplicates=%" PRId64 " | topUpdates=%" PRId64 " | %.3f s | %.0f regions/s\n",
+                 sum.worlds, sum.regions, sum.survivors, sum.matches, sum.duplicates, sum.top_updates, sum.seconds,
+                 sum.seconds > 0 ? sum.regions / sum.seconds : 0.0);
+    std::fprintf(stderr, "Run folder: %s\n", dir);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -321,7 +372,9 @@ int main(int argc, char** argv) {
                          "  BetaSeedFinderWorker density [capacity=32768] [chunkX=0] [chunkZ=0]\n"
                          "  BetaSeedFinderWorker island <first_seed> <count> [chunkX0 chunkZ0 chunksX chunksZ minBlocks]\n"
                          "  BetaSeedFinderWorker region [capacity=4096] [radius=7] [minBlocks minColumns minY minWidth "
-                         "minDepth minAvgThickness]\n");
+                         "minDepth minAvgThickness]\n"
+                         "  BetaSeedFinderWorker hunt <output_root> <regions> [general|mega|record60|record80] "
+                         "[research|lean] [sequence_seed=0] [label] [topResultsToKeep=50] [hunterCoarseMinCells=85]\n");
             return 1;
         }
         const std::string mode = argv[1];
@@ -341,6 +394,8 @@ int main(int argc, char** argv) {
             rc = island_mode(argc, argv);
         } else if (mode == "region") {
             rc = region_mode(argc, argv);
+        } else if (mode == "hunt") {
+            rc = hunt_mode(argc, argv);
         } else {
             throw WorkerError("unknown mode: " + mode);
         }

@@ -44,6 +44,15 @@ int bsf_timing_enable(int on);
 int bsf_timing_reset(void);
 int bsf_kernel_stats(const char* name, double* total_ms, uint64_t* launches, uint64_t* units);
 
+/* ---- deterministic seed source (IslandSearchEngine.deterministicSeedForAttempt,
+ * IslandSearchEngine.java:2551-2556; splitMixDeterministicSeed, p20_exact_math.hpp:420-425;
+ * generateDeterministicSeedsKernel, gpu_p20_benchmark.cpp:855-858).  out[i] = SplitMix64 of
+ * sequence_seed + (first_attempt + i) * 0x9E3779B97F4A7C15.  Java numbers world attempts from
+ * 1, the worker's benchmark modes from 0. */
+int bsf_deterministic_seeds(uint64_t sequence_seed, int64_t first_attempt, int64_t count, int64_t* out);
+int bsf_deterministic_seeds_device(uint64_t sequence_seed, int64_t first_attempt, int64_t count, int64_t* d_out,
+                                   void* stream);
+
 /* ---- terrain density grid (BetaTerrain173.generateTerrainNoise, BetaTerrain173.java:667-834)
  * For each seed, the 5x17x5 FP64 density lattice of chunk (chunk_x, chunk_z), index
  * (x*5 + z)*17 + y, bit-exact to Java.  out holds count*425 doubles.  chunk_x and chunk_z
@@ -96,6 +105,11 @@ int bsf_island_range(int64_t first_seed, int64_t count, const bsf_island_params*
 /* Hits for an explicit seed list (host), sorted by seed. */
 int bsf_island_seeds(const int64_t* seeds, int64_t count, const bsf_island_params* p,
                      bsf_island_hit* out, int64_t cap, int64_t* n_found);
+/* Hits for the seeds of attempts first_attempt .. first_attempt + count - 1 of a deterministic
+ * run (bsf_deterministic_seeds; the Java search's deterministicSeedMode), generated on the
+ * device inside the passes; sorted by seed. */
+int bsf_island_sequence(uint64_t sequence_seed, int64_t first_attempt, int64_t count, const bsf_island_params* p,
+                        bsf_island_hit* out, int64_t cap, int64_t* n_found);
 /* Stage-1 survivors only (seeds with a candidate start at y >= 68 over non-stone in a
  * sample column), sorted.  Exposed for gate audits. */
 int bsf_island_stage1_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
@@ -166,6 +180,13 @@ typedef struct bsf_region_result {
     int32_t hot_scan_ran;             /* coarse >= 120 and radius < 16: radius-16 rescan ran */
     int32_t hot_won;                  /* the rescan's match replaced the normal one */
     int32_t pad;
+    bsf_island_hit normal_hit;        /* normalMatch: the search-radius scan after border
+                                         verification (valid when normal_kept) */
+    int32_t normal_kept;              /* normalMatch != null */
+    int32_t normal_radius;            /* its chunkRadiusUsed */
+    bsf_island_hit hot_hit;           /* hotResult after border verification (valid when hot_found) */
+    int32_t hot_found;                /* hotResult != null */
+    int32_t hot_radius;               /* its chunkRadiusUsed */
 } bsf_region_result;
 
 /* processGpuRegionAttempt for each item: the scan of the (2 radius + 1)^2-chunk window around
@@ -176,6 +197,61 @@ typedef struct bsf_region_result {
  * threshold fields of *settings are used.  All scans of a call are batched per radius. */
 int bsf_region_attempts(const bsf_region_item* items, int64_t count, const bsf_island_params* settings,
                         int32_t radius, bsf_region_result* out);
+
+/* ---- run output files of a headless hunt (IslandSearchEngine.java: initializeRunOutput
+ * :4230-4299, appendCandidateCsv :4826-4982, registerUniqueMatch :2062-2070, the leaderboards
+ * :4985-5361, writeManifest :4309-4449, publishLatestRunCopies :4483-4510).  A run folder
+ * <output_root>/runs/<stamp>[_<label>] holds hunter_candidates.csv (one row per region
+ * attempt), top_largest_results.txt (and its top_floating_results.txt mirror),
+ * top_filled_footprint_results.txt, top_raw_footprint_results.txt, top_island_sideboards.csv,
+ * manifest.txt and performance_profile.txt; the output root gets latest_run.txt and copies of
+ * the latest run's files.  Host code only: these calls need no GPU. */
+typedef struct bsf_run_settings {
+    int32_t profile;                  /* BSF_PROFILE_*: the gates and the coarse threshold */
+    int32_t research;                 /* 34-byte records (1) or lean 16-byte records (0) */
+    int32_t top_keep;                 /* SearchSettings.topResultsToKeep (50), >= 1 */
+    int32_t chunk_radius;             /* SearchSettings.chunkRadius (7), 0..24 */
+    int32_t coarse_min;               /* hunterCoarseMinCells for general and mega (default 85;
+                                         <= 0 means 180); record60 / record80 use 95 / 700 */
+    int32_t pad;
+    int64_t regions;                  /* seedsToCheck: the region budget (8 regions per world) */
+    uint64_t sequence_seed;           /* deterministicSeedSequenceSeed */
+    bsf_island_params thresholds;     /* minBlocks .. minAvgThickness (window fields unused) */
+} bsf_run_settings;
+
+/* One region attempt (GpuSeedWork, IslandSearchEngine.java:1124-1203). */
+typedef struct bsf_run_attempt {
+    int64_t attempt;                  /* 1-based region attempt */
+    int64_t seed;
+    int32_t center_chunk_x, center_chunk_z;
+    int32_t p20, upper, high, coarse; /* the record's counts and coarse score */
+    int32_t p19_pass, p19_extreme;
+    double p19_score;                 /* NaN when not evaluated */
+} bsf_run_attempt;
+
+typedef struct bsf_run bsf_run;
+/* stamp: the folder's "yyyy-MM-dd_HH-mm-ss" (NULL: local time now); start_ms <= 0: now. */
+int bsf_run_open(const char* output_root, const char* run_label, const char* stamp, int64_t start_ms,
+                 const bsf_run_settings* settings, bsf_run** out);
+/* Region attempts in attempt order; *unique_match (optional) = 1 for a new match. */
+int bsf_run_add(bsf_run* run, const bsf_run_attempt* attempt, const bsf_region_result* result,
+                int32_t* unique_match);
+int bsf_run_dir(const bsf_run* run, char* buf, int64_t cap);
+/* Writes the final files (status "FINISHED", "STOPPED" ...; end_ms <= 0: now) and frees run. */
+int bsf_run_close(bsf_run* run, const char* status, int64_t checked, int64_t end_ms);
+
+typedef struct bsf_hunt_summary {
+    int64_t worlds, regions, survivors, matches, duplicates, top_updates;
+    double seconds;
+} bsf_hunt_summary;
+
+/* The Java GPU hunt loop without the Java (runGpuStage0Pipeline, :1206-1480, with
+ * processGpuRegionAttempt, :1878-1971): world seeds of attempts 1, 2, ... of the sequence
+ * (deterministicSeedMode), stage0stream batches of 32,768 worlds x 8 centres, the region
+ * attempts of every record with reject stage 0, in attempt order, into a run folder.
+ * run_dir (optional) receives the folder's path. */
+int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, const bsf_run_settings* settings,
+             bsf_hunt_summary* out, char* run_dir, int64_t run_dir_cap);
 
 /* ---- serving (no counterpart in the reference).  A request loop on its own thread, stream
  * and engine: jobs are admitted as job-table entries free up (state memory) into passes whose

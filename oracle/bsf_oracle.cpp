@@ -1158,6 +1158,15 @@ void bsfo_upsample_density(const double* lattice425, double* out) {
 
 double bsfo_scout_statistic(int64_t seed) { return scoutStatistic(seed); }
 
+// Java: z = sequenceSeed + attempt * 0x9E3779B97F4A7C15L, two xor-shift-multiply rounds
+// (>>> 30, >>> 27) and a final xor-shift (>>> 31), all in wrapping 64-bit arithmetic.
+uint64_t bsfo_splitmix_seed(uint64_t sequence_seed, uint64_t attempt) {
+    uint64_t z = sequence_seed + attempt * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 int bsfo_island_scan(int64_t seed, const bsfo_island_params* p, bsfo_island_hit* hit) {
     return islandScan(seed, *p, hit);
 }
@@ -1350,6 +1359,11 @@ void bsfo_region_attempt(int64_t seed, int32_t cx, int32_t cz, int32_t coarse, c
     bool have_normal = regionScan(seed, cx, cz, radius, *s, &normal) != 0;
     out->normal_found = have_normal ? 1 : 0;
     if (have_normal && (normal.flags & 1) && radius < 24) have_normal = borderVerify(seed, cx, cz, *s, &normal, &normal_r);
+    if (have_normal) {
+        out->normal_hit = normal;
+        out->normal_kept = 1;
+        out->normal_radius = normal_r;
+    }
     bool have = have_normal;
     bsfo_island_hit match = normal;
     int match_r = normal_r;
@@ -1359,6 +1373,11 @@ void bsfo_region_attempt(int64_t seed, int32_t cx, int32_t cz, int32_t coarse, c
         int hot_r = 16;
         bool have_hot = regionScan(seed, cx, cz, 16, *s, &hot) != 0;
         if (have_hot && (hot.flags & 1) && hot_r < 24) have_hot = borderVerify(seed, cx, cz, *s, &hot, &hot_r);
+        if (have_hot) {
+            out->hot_hit = hot;
+            out->hot_found = 1;
+            out->hot_radius = hot_r;
+        }
         if (have_hot && (!have || hitBetter(hot, match))) {
             match = hot;
             match_r = hot_r;

@@ -90,6 +90,10 @@ void     bsfo_upsample_density(const double* lattice425, double* out32768);
  *      first terrain Perlin at chunk (0,0), index order) ---- */
 double   bsfo_scout_statistic(int64_t seed);
 
+/* SplitMix64 seed of an attempt: IslandSearchEngine.deterministicSeedForAttempt
+ * (IslandSearchEngine.java:2551-2556; native twin p20_exact_math.hpp:420-425). */
+uint64_t bsfo_splitmix_seed(uint64_t sequence_seed, uint64_t attempt);
+
 /* ---- config 3: exact island scan over a chunk window; returns 1 and fills *hit on a match ---- */
 int      bsfo_island_scan(int64_t seed, const bsfo_island_params* p, bsfo_island_hit* hit);
 /* flood fill + gates on an explicit stone volume laid out [x][y][z] with sizeY=128 (for tests) */
@@ -114,6 +118,12 @@ typedef struct bsfo_region_result {
     int32_t hot_scan_ran;
     int32_t hot_won;                 /* the hot-seed result replaced the normal one */
     int32_t pad;
+    bsfo_island_hit normal_hit;      /* normalMatch: the radius scan after border verification */
+    int32_t normal_kept;             /* normalMatch != null */
+    int32_t normal_radius;           /* its chunkRadiusUsed */
+    bsfo_island_hit hot_hit;         /* hotResult after border verification */
+    int32_t hot_found;               /* hotResult != null */
+    int32_t hot_radius;              /* its chunkRadiusUsed */
 } bsfo_region_result;
 
 void     bsfo_region_attempt(int64_t seed, int32_t center_chunk_x, int32_t center_chunk_z, int32_t coarse_score,

@@ -58,6 +58,8 @@ def lib():
         L.bsfo_density_digest.restype = ctypes.c_uint64
         L.bsfo_run_scout.restype = ctypes.c_int64
         L.bsfo_run_island.restype = ctypes.c_int64
+        L.bsfo_splitmix_seed.restype = ctypes.c_uint64
+        L.bsfo_splitmix_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         for name in ("bsfo_scout_statistic",):
             getattr(L, name).argtypes = [ctypes.c_int64]
         _LIB = L
@@ -74,6 +76,12 @@ def chunk_density(seed, cx=0, cz=0):
     out = np.zeros(425, dtype=np.float64)
     lib().bsfo_chunk_density(ctypes.c_int64(seed), cx, cz, out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+def splitmix_seed(sequence_seed, attempt):
+    """deterministicSeedForAttempt as a signed 64-bit seed."""
+    v = lib().bsfo_splitmix_seed(sequence_seed & (2**64 - 1), attempt & (2**64 - 1))
+    return v - (1 << 64) if v >= 1 << 63 else v
 
 
 def scout_statistic(seed):
@@ -156,7 +164,9 @@ def run_stage0_records(seeds, profile="general", research=True, centers=8, threa
 class RegionResult(ctypes.Structure):
     _fields_ = [("hit", IslandHit), ("found", ctypes.c_int32), ("radius_used", ctypes.c_int32),
                 ("normal_found", ctypes.c_int32), ("hot_scan_ran", ctypes.c_int32), ("hot_won", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("normal_hit", IslandHit), ("normal_kept", ctypes.c_int32),
+                ("normal_radius", ctypes.c_int32), ("hot_hit", IslandHit), ("hot_found", ctypes.c_int32),
+                ("hot_radius", ctypes.c_int32)]
 
 
 def region_attempt(seed, center_chunk_x, center_chunk_z, coarse_score, settings, radius=7):
@@ -164,5 +174,14 @@ def region_attempt(seed, center_chunk_x, center_chunk_z, coarse_score, settings,
     r = RegionResult()
     lib().bsfo_region_attempt(ctypes.c_int64(seed), center_chunk_x, center_chunk_z, coarse_score,
                               ctypes.byref(settings), radius, ctypes.byref(r))
+    return region_dict(r)
+
+
+def region_dict(r):
+    """The fields of a region result (library or oracle struct) as a dict."""
     return {"hit": r.hit.as_tuple() if r.found else None, "radius_used": r.radius_used,
-            "normal_found": r.normal_found, "hot_scan_ran": r.hot_scan_ran, "hot_won": r.hot_won}
+            "normal_found": r.normal_found, "hot_scan_ran": r.hot_scan_ran, "hot_won": r.hot_won,
+            "normal": r.normal_hit.as_tuple() if r.normal_kept else None,
+            "normal_radius": r.normal_radius if r.normal_kept else 0,
+            "hot": r.hot_hit.as_tuple() if r.hot_found else None,
+            "hot_radius": r.hot_radius if r.hot_found else 0}

@@ -256,6 +256,25 @@ void p19_fixture(const std::string& dir) {
 
 }  // namespace
 
+// SplitMix64 deterministic seeds (splitMixDeterministicSeed): sequence seeds x attempts.
+void splitmix_fixture(const std::string& dir) {
+    FILE* f = open_out(dir, "ref_splitmix.json");
+    const uint64_t seqs[] = {0ULL, 1ULL, 0xB5F00002ULL, 0x9E3779B97F4A7C15ULL, 0x8000000000000000ULL,
+                             0xFFFFFFFFFFFFFFFFULL, 123456789ULL};
+    const uint64_t attempts[] = {0ULL, 1ULL, 2ULL, 3ULL, 1000ULL, 1ULL << 32, (1ULL << 40) + 7, 0xFFFFFFFFFFFFFFFFULL};
+    std::fprintf(f, "{\"cases\": [\n");
+    bool first = true;
+    for (uint64_t q : seqs)
+        for (uint64_t a : attempts) {
+            std::fprintf(f, "%s {\"sequence_seed\": \"%016" PRIx64 "\", \"attempt\": \"%016" PRIx64
+                            "\", \"seed\": \"%016" PRIx64 "\"}",
+                         first ? "" : ",\n", q, a, p20::splitMixDeterministicSeed(q, a));
+            first = false;
+        }
+    std::fprintf(f, "\n]}\n");
+    std::fclose(f);
+}
+
 int main(int argc, char** argv) {
     const std::string dir = argc > 1 ? argv[1] : ".";
     const int n_coarse = argc > 2 ? std::atoi(argv[2]) : 24;
@@ -263,6 +282,7 @@ int main(int argc, char** argv) {
     perlin_fixture(dir);
     p19_fixture(dir);
     coarse_fixture(dir, n_coarse);
+    splitmix_fixture(dir);
     std::printf("fixtures written to %s\n", dir.c_str());
     return 0;
 }

@@ -141,7 +141,7 @@ def test_served_results_equal_alone_and_in_mix(bsf, fifo):
 def test_server_rejects_beyond_queue_and_tiny_table(bsf):
     """state memory for 2 job-table entries, no queue: later jobs are rejected, admitted ones finish."""
     p = bsf.island_params(**kw(0, 0, 2, 2))
-    with bsf.Server(state_bytes=2 * 112, hit_buffer_bytes=4096 * 128, queue_limit=0) as srv:
+    with bsf.Server(state_bytes=2 * 120, hit_buffer_bytes=4096 * 128, queue_limit=0) as srv:
         ids = [srv.submit(10 * i, 3000, p) for i in range(6)]
         kinds = {}
         deadline = time.time() + 60

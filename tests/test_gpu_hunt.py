@@ -1,0 +1,70 @@
+"""The headless GPU hunt against the oracle pipeline: deterministic world seeds, stage0stream
+records, the reject-stage-0 selection, region attempts and the run folder, file by file."""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from oracle import run_outputs as ro
+
+pytestmark = pytest.mark.gpu
+
+CX = [0, 15, 0, 15, -15, -15, 0, 15]
+CZ = [0, 0, 15, 15, 0, 15, -15, -15]
+LOW = dict(min_blocks=50, min_columns=10, min_width=3, min_depth=3)
+
+
+def test_hunt_matches_oracle_pipeline(bsf, oracle, tmp_path):
+    seq, worlds, keep = 0xB5F0, 512, 5
+    regions = 8 * worlds - 5  # the last world is cut short, as the Java loop cuts its last batch
+    settings = bsf.run_settings("general", research=False, top_keep=keep, chunk_radius=7, coarse_min=1,
+                                regions=regions, sequence_seed=seq, thresholds=bsf.island_params(0, 0, 1, 1, **LOW))
+    summary, run_dir = bsf.hunt(str(tmp_path), settings, label="gpu hunt", stamp="2026-05-06_07-08-09")
+    print("hunt:", summary, flush=True)
+
+    seeds = np.array([oracle.splitmix_seed(seq, a) for a in range(1, worlds + 1)], dtype=np.int64)
+    recs, _ = oracle.run_stage0_records(seeds, "general", research=False, threads=16)
+    attempts = []
+    for off in range(regions):
+        w, c = divmod(off, 8)
+        r = recs[w, c]
+        coarse = int(r[4]) | (int(r[5]) << 8)
+        score = float(np.frombuffer(bytes(r[8:16]), dtype="<f8")[0])
+        if r[3] == 1 and r[7] == 0 and coarse >= 1 and r[0] >= 1 and r[1] >= 5 and r[2] >= 5:
+            attempts.append((off + 1, int(seeds[w]), CX[c], CZ[c], int(r[0]), int(r[1]), int(r[2]), coarse, 1,
+                             int(r[6]), score))
+    s = oracle.island_params(0, 0, 1, 1, **LOW)
+    with ThreadPoolExecutor(16) as ex:
+        results = list(ex.map(lambda a: oracle.region_attempt(a[1], a[2], a[3], a[7], s, 7), attempts))
+    want = ro.run_files(attempts, results, run_dir, keep)
+    got = ro.read_run(run_dir)
+    for name in want:
+        assert got[name] == want[name], name
+    matches = sum(r["hit"] is not None for r in results)
+    assert summary["worlds"] == worlds and summary["regions"] == regions
+    assert summary["survivors"] == len(attempts) > 20
+    assert summary["matches"] + summary["duplicates"] == matches > 0
+    assert "status=FINISHED\n" in got["manifest.txt"] and "checked=%d\n" % regions in got["manifest.txt"]
+
+
+def test_worker_hunt_mode_writes_the_same_run(bsf, tmp_path):
+    """`BetaSeedFinderWorker hunt` (SearchSettings default thresholds) leaves the run folder the
+    library's hunt leaves for the same settings."""
+    import os
+    import subprocess
+    worker = os.path.join(os.path.dirname(bsf.lib_path()), "BetaSeedFinderWorker.exe")
+    seq, regions = 77, 8 * 300
+    p = subprocess.run([worker, "hunt", str(tmp_path / "w"), str(regions), "mega", "lean", str(seq), "cli run", "7",
+                        "1"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=300)
+    err = p.stderr.decode()
+    assert p.returncode == 0, err
+    assert "Hunt finished | worlds=300 | regions=2400 |" in err
+    cli_dir = err.split("Run folder: ")[1].strip()
+    settings = bsf.run_settings("mega", research=False, top_keep=7, coarse_min=1, regions=regions, sequence_seed=seq)
+    summary, lib_dir = bsf.hunt(str(tmp_path / "l"), settings, label="cli run")
+    a, b = ro.read_run(cli_dir), ro.read_run(lib_dir)
+    assert set(a) == set(b)
+    for name in a:  # preview paths name the run folder
+        if name != "manifest.txt":
+            assert a[name].replace(cli_dir, lib_dir) == b[name], name
+    assert summary["survivors"] == len(a["hunter_candidates.csv"].splitlines()) - 1 > 0

@@ -434,6 +434,21 @@ def test_island_stage1_predicate_is_superset_of_hits(oracle):
         assert oracle.island_has_candidate(h[0], p)
 
 
+def test_splitmix_seeds_match_reference_fixture(oracle):
+    """deterministicSeedForAttempt against the reference's splitMixDeterministicSeed outputs and
+    the published first SplitMix64 output for state 0 (0xE220A8397B1DCDAF)."""
+    cases = json.load(open(os.path.join(GOLDEN, "ref_splitmix.json")))["cases"]
+    assert len(cases) >= 50
+    for c in cases:
+        got = oracle.splitmix_seed(int(c["sequence_seed"], 16), int(c["attempt"], 16)) & (2**64 - 1)
+        assert got == int(c["seed"], 16), c
+    assert oracle.splitmix_seed(0, 1) & (2**64 - 1) == 0xE220A8397B1DCDAF
+    # a plausible slip (a dropped xor-shift round, a 32-bit multiply) changes the value
+    z = 0x9E3779B97F4A7C15
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+    assert (z ^ (z >> 31)) != 0xE220A8397B1DCDAF
+
+
 def test_scout_statistic_is_octave_zero_sum(oracle):
     """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
     generator's octave-0 values (0.0 + v), index order x, z, y."""
@@ -545,15 +560,19 @@ def _attempt_by_direct_scans(oracle, seed, cx, cz, coarse, s, radius):
         return (a[4] - a[3] + 1) * (a[6] - a[5] + 1) > (b[4] - b[3] + 1) * (b[6] - b[5] + 1)
 
     normal = scan(radius)
-    out = {"normal_found": int(normal is not None), "hot_scan_ran": 0, "hot_won": 0}
+    out = {"normal_found": int(normal is not None), "hot_scan_ran": 0, "hot_won": 0, "hot": None, "hot_radius": 0}
     match, r_match = normal, radius
     if normal is not None and (normal[13] & 1) and radius < 24:
         match, r_match = verify(normal, radius)
+    out["normal"] = match
+    out["normal_radius"] = r_match if match is not None else 0
     if coarse >= 120 and radius < 16:
         out["hot_scan_ran"] = 1
         hot, r_hot = scan(16), 16
         if hot is not None and (hot[13] & 1):
             hot, r_hot = verify(hot, 16)
+        out["hot"] = hot
+        out["hot_radius"] = r_hot if hot is not None else 0
         if hot is not None and (match is None or better(hot, match)):
             match, r_match = hot, r_hot
             out["hot_won"] = 1
