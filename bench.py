@@ -271,7 +271,7 @@ def run_density_scout(bsf, args):
     s0_s = time.perf_counter() - t
     n_worlds = 32768 * args.stage0_batches
     regions = run_regions(bsf, args, worlds, batches)
-    chain = run_mega_chain(bsf, args, start + 32768 * (args.stage0_batches + 1))
+    chain = run_mega_hunt(bsf, args)
     return {
         "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
                          "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
@@ -287,7 +287,7 @@ def run_density_scout(bsf, args):
                          round(s0_s * 1e3 / args.stage0_batches, 2),
                          "counters": dict(zip([n for n, _ in bsf.Stage0Counters._fields_], totals.tolist()))},
         "region_attempts": regions,
-        "mega_chain": chain,
+        "mega_hunt": chain,
     }
 
 
@@ -310,30 +310,25 @@ def region_items(worlds, batches, coarse_min=85):
     return items
 
 
-def run_mega_chain(bsf, args, first_world):
-    """The MEGA production chain end to end: stage0stream batches (mega profile, lean 16-byte
-    records, 8 centres) and the region attempts of their survivors, timed together."""
-    import numpy as np
-    n_batches = args.stage0_batches
-    worlds = np.arange(first_world, first_world + 32768 * n_batches, dtype=np.int64)
-    bsf.stage0_records(worlds[:1024], "mega", False, centers=8)
-    settings = bsf.island_params(0, 0, 1, 1)
-    t = time.perf_counter()
-    batches = []
-    for b in range(n_batches):
-        recs, _ = bsf.stage0_records(worlds[b * 32768:(b + 1) * 32768], "mega", False, centers=8)
-        batches.append(recs)
-    t_s0 = time.perf_counter()
-    items = region_items(worlds, batches)
-    res = bsf.region_attempts(items, settings, 7) if items else []
-    total = time.perf_counter() - t
-    n = 32768 * n_batches
-    return {"workload": "%d stage0stream batches of 32768 worlds x 8 centres (mega profile, lean records) from "
-                        "splitmix64(0xB5F00002) & (2^48-1) + %d, then region attempts of their survivors "
-                        "(radius 7, SearchSettings defaults); host in/out" % (n_batches, first_world - (splitmix64(0xB5F00002) & MASK48)),
-            "worlds_per_s": round(n / total, 1), "regions_per_s": round(8 * n / total, 1),
-            "stage0_ms": round((t_s0 - t) * 1e3, 2), "region_ms": round((time.perf_counter() - t_s0) * 1e3, 2),
-            "survivors": len(items), "matches": sum(r["hit"] is not None for r in res)}
+def run_mega_hunt(bsf, args):
+    """The MEGA production hunt end to end (bsf_hunt): SplitMix world seeds, stage0stream batches
+    (mega profile, lean records, 8 centres), region attempts of every reject-stage-0 record, and
+    the Java search's run folder written to a temporary directory."""
+    import tempfile
+    regions = 32768 * 8 * args.stage0_batches
+    seq = splitmix64(0xB5F00007)
+    settings = bsf.run_settings("mega", research=False, regions=regions, sequence_seed=seq)
+    with tempfile.TemporaryDirectory() as d:
+        bsf.hunt(d, bsf.run_settings("mega", research=False, regions=8 * 1024, sequence_seed=seq))  # warm-up
+        t = time.perf_counter()
+        summary, _ = bsf.hunt(d, settings, label="bench")
+        wall = time.perf_counter() - t
+    return {"workload": "headless MEGA hunt: %d regions (%d worlds of attempts 1.. of SplitMix sequence "
+                        "splitmix64(0xB5F00007)), lean records, region attempts at SearchSettings defaults, run "
+                        "folder written" % (regions, regions // 8),
+            "regions_per_s": round(regions / wall, 1), "worlds_per_s": round(regions / 8 / wall, 1),
+            "seconds": round(wall, 3), "survivors": summary["survivors"], "matches": summary["matches"],
+            "duplicates": summary["duplicates"]}
 
 
 def run_regions(bsf, args, worlds, batches):
