@@ -364,8 +364,7 @@ RunWriter::RunWriter(const std::string& root, const std::string& label, const st
     write_file(path("top_raw_footprint_results.txt"), "Top raw-footprint floating islands\nNo results yet.\n");
     write_file(path("top_island_sideboards.csv"), kTopHeader);
     write_file(root_ + "/latest_run.txt", absolute(dir_) + "\n");
-    csv_ = kCandidateHeader;
-    write_file(path("hunter_candidates.csv"), csv_);
+    write_file(path("hunter_candidates.csv"), kCandidateHeader);
 }
 
 bool RunWriter::add(const bsf_run_attempt& a, const bsf_region_result& r) {
@@ -487,7 +486,16 @@ bool RunWriter::add(const bsf_run_attempt& a, const bsf_region_result& r) {
     row += source;
     row += '\n';
     csv_ += row;
+    if (csv_.size() >= (size_t(1) << 20)) flush_rows();
     return unique;
+}
+
+void RunWriter::flush_rows() {
+    if (csv_.empty()) return;
+    std::ofstream f(path("hunter_candidates.csv"), std::ios::binary | std::ios::app);
+    f << csv_;
+    if (!f) throw ArgError("cannot write " + path("hunter_candidates.csv"));
+    csv_.clear();
 }
 
 void RunWriter::write_boards() {
@@ -515,7 +523,7 @@ void RunWriter::write_boards() {
 }
 
 void RunWriter::close(const std::string& status, int64_t checked, int64_t start_ms, int64_t end_ms) {
-    write_file(path("hunter_candidates.csv"), csv_);
+    flush_rows();
     write_boards();
     write_file(path("performance_profile.txt"), "Performance profiler disabled.\n");
     const int prof = s_.profile;

@@ -72,11 +72,12 @@ class RunWriter {
    private:
     std::string path(const char* name) const { return dir_ + "/" + name; }
     void write_boards();
+    void flush_rows();  // appends the buffered candidate rows to hunter_candidates.csv
     std::string root_, label_, dir_;
     bsf_run_settings s_;
     std::set<std::string> seen_;
     JavaHeap top_, filled_, raw_;
-    std::string csv_;  // hunter_candidates.csv, header and rows
+    std::string csv_;  // candidate rows not yet appended to hunter_candidates.csv
     bool boards_dirty_ = false;
 };
 

@@ -86,7 +86,7 @@ def _synthetic_run(n, seed=1):
     return attempts, results
 
 
-@pytest.mark.parametrize("n,keep", [(0, 50), (40, 50), (400, 5), (400, 1)])
+@pytest.mark.parametrize("n,keep", [(0, 50), (40, 50), (400, 5), (400, 1), (5000, 20)])
 def test_run_folder_matches_restatement(tmp_path, n, keep):
     import betaseedfinder_amd as bsf
     attempts, results = _synthetic_run(n, seed=n + keep)
