@@ -342,7 +342,7 @@ __device__ __forceinline__ void append_open(const EngineArgs& a, bool open, int6
 // worlds side by side, or no gate.  Separate kernels keep the register budget of the common
 // one-world form (and so its occupancy) free of the other form.
 template <int KIND>
-__global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
+__global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
     __shared__ GenStage st;
     __shared__ GenStageMulti stm;
     __shared__ unsigned long long s_item;
@@ -501,7 +501,7 @@ constexpr int kS2Threads = 64;
 
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
-__global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a) {
+__global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a) {
     __shared__ GenStage st;
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns

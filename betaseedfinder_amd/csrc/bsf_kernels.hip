@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void splitmix_seeds_kernel(uint64_t sequence_s
 // ---------------------------------------------------------------------------
 // Stage-0 scout: first terrain generator, octave 0, chunk (0,0) 5x17x5, S = sum.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void scout_kernel(ScoutArgs a) {
+__global__ __launch_bounds__(64, 2) void scout_kernel(ScoutArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
     const int lane = threadIdx.x;
     LanePerm P;

@@ -430,7 +430,7 @@ __global__ __launch_bounds__(64, 2) void s0_lower_kernel(S0Args a) {
 
 // The 61x61x17 coarse sign grid of one item (coarse_exact_core.hpp:218-386), built in LDS as
 // one 61-bit z mask per (x, y) row and written to coarse_bits[list index].
-__global__ __launch_bounds__(64, 2) void s0_coarse_signs_kernel(S0Args a) {
+__global__ __launch_bounds__(64, 3) void s0_coarse_signs_kernel(S0Args a) {
     __shared__ GenStage st;
     __shared__ unsigned long long rem[kCoarseRows];
     const int tid = threadIdx.x;
