@@ -40,7 +40,27 @@ __host__ __device__ __forceinline__ uint64_t splitmix_seed(uint64_t sequence_see
 struct DivMagic {
     uint32_t m;
     uint32_t shift;
+    uint32_t limit;  // nextInt(d) redraws bits >= limit: d * floor(2^31 / d), 2^31 (never) for d = 2^k
+    uint32_t pad;
 };
+
+// The table for d = 1 .. 256 (entry 0 unused).  Java redraws when bits - bits % d + (d - 1)
+// overflows an int, that is when d * (q + 1) > 2^31 for q = bits / d, which for d not a power
+// of two is q >= floor(2^31 / d), i.e. bits >= d * floor(2^31 / d).
+inline void fill_div_magic(DivMagic* t) {
+    for (uint32_t d = 0; d <= 256; ++d) t[d] = DivMagic{0, 0, 0x80000000u, 0};
+    for (uint32_t d = 1; d <= 256; ++d) {
+        uint32_t k = 0;
+        while ((2u << k) <= d) ++k;  // 2^k <= d < 2^(k+1)
+        if ((d & (d - 1)) == 0) {
+            t[d].shift = 31 - k;
+        } else {
+            t[d].m = static_cast<uint32_t>(((1ULL << (32 + k)) + d - 1) / d);
+            t[d].shift = k;
+            t[d].limit = d * ((1u << 31) / d);
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // java.util.Random, 48-bit LCG (call sites NoiseGeneratorPerlin173.java:63-78,
@@ -68,15 +88,10 @@ struct Lcg {
 __device__ __forceinline__ uint32_t next_int_uniform_bound(Lcg& r, uint32_t bound, DivMagic mg) {
     uint32_t bits = r.next(31);
     if ((bound & (bound - 1)) == 0) return bits >> mg.shift;  // (bound * bits) >> 31
-    uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
-    uint32_t val = bits - q * bound;
     // Rejection when bits - val + (bound - 1) overflows int32 (probability < bound / 2^31).
-    while (static_cast<int32_t>(bits - val + (bound - 1)) < 0) {
-        bits = r.next(31);
-        q = __umulhi(bits, mg.m) >> mg.shift;
-        val = bits - q * bound;
-    }
-    return val;
+    while (bits >= mg.limit) bits = r.next(31);
+    const uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
+    return bits - q * bound;
 }
 
 // ---------------------------------------------------------------------------
@@ -112,28 +127,29 @@ struct GenOffsets {
 constexpr uint64_t kLcgMul2 = (kLcgMul * kLcgMul) & kLcgMask;
 constexpr uint64_t kLcgAdd2 = (kLcgMul * kLcgAdd + kLcgAdd) & kLcgMask;
 
-__device__ __forceinline__ bool int_from_bits(uint32_t bits, uint32_t bound, DivMagic mg, uint32_t& val) {
-    if ((bound & (bound - 1)) == 0) {
-        val = bits >> mg.shift;
-        return true;
-    }
-    const uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
-    val = bits - q * bound;
-    return static_cast<int32_t>(bits - val + (bound - 1)) >= 0;
+// nextInt(bound) of bits that are not rejected.
+__device__ __forceinline__ uint32_t int_from_bits(uint32_t bits, uint32_t bound, DivMagic mg) {
+    if ((bound & (bound - 1)) == 0) return bits >> mg.shift;
+    return bits - (__umulhi(bits, mg.m) >> mg.shift) * bound;
 }
 
+// Two consecutive nextInt draws, bounds b0 then b1, from states s1 = A s + C and
+// s2 = A^2 s + (A + 1) C formed side by side.  When a draw of any lane of the wave would be
+// rejected (probability < 2^-23 per draw) the whole wave redoes the pair one draw at a time,
+// as Java does, so the common path has no per-lane branch.
 __device__ __forceinline__ void next_two_ints(Lcg& r, uint32_t b0, DivMagic m0, uint32_t b1, DivMagic m1,
                                               uint32_t& v0, uint32_t& v1) {
     const uint64_t s1 = (r.s * kLcgMul + kLcgAdd) & kLcgMask;
     const uint64_t s2 = (r.s * kLcgMul2 + kLcgAdd2) & kLcgMask;
-    const bool ok0 = int_from_bits(static_cast<uint32_t>(s1 >> 17), b0, m0, v0);
-    const bool ok1 = int_from_bits(static_cast<uint32_t>(s2 >> 17), b1, m1, v1);
-    if (ok0 && ok1) {
-        r.s = s2;
+    const uint32_t u0 = static_cast<uint32_t>(s1 >> 17), u1 = static_cast<uint32_t>(s2 >> 17);
+    if (__builtin_expect(__any(u0 >= m0.limit || u1 >= m1.limit), 0)) {
+        v0 = next_int_uniform_bound(r, b0, m0);
+        v1 = next_int_uniform_bound(r, b1, m1);
         return;
     }
-    v0 = next_int_uniform_bound(r, b0, m0);
-    v1 = next_int_uniform_bound(r, b1, m1);
+    v0 = int_from_bits(u0, b0, m0);
+    v1 = int_from_bits(u1, b1, m1);
+    r.s = s2;
 }
 
 // The shuffle walks two steps (i, i + 1) per LDS round trip: the four bytes are read first
@@ -172,12 +188,14 @@ __device__ __forceinline__ GenOffsets build_generator(Lcg& r, LanePerm p, const 
 
 // Advance the stream exactly like one Perlin constructor without keeping the table
 // (NoiseGeneratorPerlin173.consumeConstructorRandom, :81-88).
+// Only the draws' count matters here, so each draw is just a state step and the rejection test.
 __device__ __forceinline__ void consume_generator(Lcg& r, const DivMagic* __restrict__ magic) {
     r.step(); r.step(); r.step(); r.step(); r.step(); r.step();  // three nextDouble()
 #pragma unroll 4
     for (int i = 0; i < 256; ++i) {
-        const uint32_t bound = 256u - static_cast<uint32_t>(i);
-        (void)next_int_uniform_bound(r, bound, magic[bound]);
+        const uint32_t limit = magic[256u - static_cast<uint32_t>(i)].limit;
+        while (r.next(31) >= limit) {
+        }
     }
 }
 

@@ -21,17 +21,8 @@ namespace bsf {
 __constant__ DivMagic isl_magic[257];
 
 hipError_t upload_island_tables() {
-    DivMagic t[257] = {};
-    for (uint32_t d = 1; d <= 256; ++d) {
-        uint32_t k = 0;
-        while ((2u << k) <= d) ++k;
-        if ((d & (d - 1)) == 0) {
-            t[d].shift = 31 - k;
-        } else {
-            t[d].m = static_cast<uint32_t>(((1ULL << (32 + k)) + d - 1) / d);
-            t[d].shift = k;
-        }
-    }
+    DivMagic t[257];
+    fill_div_magic(t);
     return hipMemcpyToSymbol(HIP_SYMBOL(isl_magic), t, sizeof(t));
 }
 

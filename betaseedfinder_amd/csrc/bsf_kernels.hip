@@ -14,19 +14,8 @@ namespace bsf {
 __constant__ DivMagic c_magic[257];
 
 hipError_t upload_tables() {
-    DivMagic t[257] = {};
-    for (uint32_t d = 1; d <= 256; ++d) {
-        uint32_t k = 0;
-        while ((2u << k) <= d) ++k;  // 2^k <= d < 2^(k+1)
-        if ((d & (d - 1)) == 0) {
-            t[d].m = 0;
-            t[d].shift = 31 - k;
-        } else {
-            const unsigned long long num = 1ULL << (32 + k);
-            t[d].m = static_cast<uint32_t>((num + d - 1) / d);
-            t[d].shift = k;
-        }
-    }
+    DivMagic t[257];
+    fill_div_magic(t);
     return hipMemcpyToSymbol(HIP_SYMBOL(c_magic), t, sizeof(t));
 }
 

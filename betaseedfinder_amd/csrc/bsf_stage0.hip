@@ -48,17 +48,8 @@ hipError_t upload_p19_tables() {
 }
 
 hipError_t upload_stage0_tables() {
-    DivMagic t[257] = {};
-    for (uint32_t d = 1; d <= 256; ++d) {
-        uint32_t k = 0;
-        while ((2u << k) <= d) ++k;
-        if ((d & (d - 1)) == 0) {
-            t[d].shift = 31 - k;
-        } else {
-            t[d].m = static_cast<uint32_t>(((1ULL << (32 + k)) + d - 1) / d);
-            t[d].shift = k;
-        }
-    }
+    DivMagic t[257];
+    fill_div_magic(t);
     const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(s0_magic), t, sizeof(t));
     if (e != hipSuccess) return e;
     return upload_p19_tables();

@@ -62,3 +62,17 @@ def test_windows_at_the_limit_match_oracle(bsf, oracle):
     for s in (11, 12, 13):
         r = oracle.island_scan(s, oracle.island_params(*w_region, **LOW))
         assert bsf.window_scan([s], [bsf.island_params(*w_region, **LOW)])[0] == r, s
+
+
+# Seeds whose terrain generator builds hit a nextInt redraw (a draw at or above
+# bound * floor(2^31 / bound)): in a built generator or in a consumed one.
+REDRAW_SEEDS = [874, 1621, 4428, 5748, 6158, 7040, 9535, 12700, 15279, 15514]
+
+
+def test_redraw_seeds_match_oracle(bsf, oracle):
+    got = bsf.density_grid(REDRAW_SEEDS)
+    for i, s in enumerate(REDRAW_SEEDS):
+        assert np.array_equal(got[i], np.asarray(oracle.chunk_density(s, 0, 0))), s
+    p = bsf.island_params(-2, -2, 4, 4, **LOW)
+    ref = [oracle.island_scan(s, oracle.island_params(-2, -2, 4, 4, **LOW)) for s in REDRAW_SEEDS]
+    assert bsf.island_seeds(REDRAW_SEEDS, p) == sorted((r for r in ref if r is not None), key=lambda h: h[0])
