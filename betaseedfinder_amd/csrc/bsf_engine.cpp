@@ -129,10 +129,10 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
     for (int b = 0; b < 2; ++b)
         grid_s2_[b] = occupancy_grid(pass_stage2_kernel, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
-    grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, 256, flood_lds_bytes(0), 0);
-    grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, 256, flood_lds_bytes(1), 0);
-    grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, 256, flood_lds_bytes(2), 0);
-    grid_flood_[kFloodClassLarge] = occupancy_grid(pass_flood_kernel<2, true>, 256, 0, 2);
+    grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(0), 0);
+    grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(1), 0);
+    grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, kFloodThreads, flood_lds_bytes(2), 0);
+    grid_flood_[kFloodClassLarge] = occupancy_grid(pass_flood_kernel<2, true>, kFloodThreads, 0, 2);
     flood_scratch_words_ = static_cast<int64_t>(flood_words(128, 2));
     flood_scratch_.ensure(static_cast<size_t>(grid_flood_[kFloodClassLarge]) * flood_scratch_words_ * 8);
     check(hipDeviceSynchronize(), "engine setup");
@@ -301,13 +301,13 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
         auto flood1 = pass_flood_kernel<1, false>;
         auto flood2 = pass_flood_kernel<2, false>;
         auto flood2g = pass_flood_kernel<2, true>;
-        hipLaunchKernelGGL(flood1, dim3(grid_flood_[0]), dim3(256), flood_lds_bytes(0), s, a,
+        hipLaunchKernelGGL(flood1, dim3(grid_flood_[0]), dim3(kFloodThreads), flood_lds_bytes(0), s, a,
                            static_cast<int>(kFloodClassSmall));
-        hipLaunchKernelGGL(flood1, dim3(grid_flood_[1]), dim3(256), flood_lds_bytes(1), s, a,
+        hipLaunchKernelGGL(flood1, dim3(grid_flood_[1]), dim3(kFloodThreads), flood_lds_bytes(1), s, a,
                            static_cast<int>(kFloodClassWideX));
-        hipLaunchKernelGGL(flood2, dim3(grid_flood_[2]), dim3(256), flood_lds_bytes(2), s, a,
+        hipLaunchKernelGGL(flood2, dim3(grid_flood_[2]), dim3(kFloodThreads), flood_lds_bytes(2), s, a,
                            static_cast<int>(kFloodClassWideZ));
-        hipLaunchKernelGGL(flood2g, dim3(grid_flood_[3]), dim3(256), 0, s, a, static_cast<int>(kFloodClassLarge));
+        hipLaunchKernelGGL(flood2g, dim3(grid_flood_[3]), dim3(kFloodThreads), 0, s, a, static_cast<int>(kFloodClassLarge));
         check(hipGetLastError(), "flood launch");
         t.stop(0);
     }

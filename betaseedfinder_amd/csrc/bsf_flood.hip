@@ -358,7 +358,7 @@ struct FloodShared {
     unsigned long long zor[W];
     int total;
     int lo, hi;                      // x slices of the growing component
-    unsigned long long starts[256];  // candidate-start levels of a slice of sample columns
+    unsigned long long starts[kFloodThreads];  // candidate-start levels of a slice of sample columns
 };
 
 // The flood of one window: upsampling of its lattice (levels 8..15 of every density column,
@@ -595,7 +595,7 @@ __device__ bsf_island_hit make_hit(int64_t seed, const bsf_island_params& p, con
 }
 
 template <int W, bool GLOBAL>
-__global__ __launch_bounds__(256) void pass_flood_kernel(EngineArgs a, int cls) {
+__global__ __launch_bounds__(kFloodThreads) void pass_flood_kernel(EngineArgs a, int cls) {
     extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
     __shared__ FloodShared<W> sh;
     __shared__ unsigned long long s_item;

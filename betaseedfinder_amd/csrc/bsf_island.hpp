@@ -106,6 +106,7 @@ struct OpenColumn {
 };
 
 constexpr int kFloorY = 68;
+constexpr int kFloodThreads = 512;  // engine flood workgroup: 8 waves per survivor window
 constexpr int kLevels = 128 - kFloorY;   // 60 block rows per x
 constexpr int kRowStride = kLevels + 1;  // 61: odd stride spreads the x rows over the LDS banks
 
