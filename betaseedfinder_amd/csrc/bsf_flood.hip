@@ -1,4 +1,4 @@
-// bsf_flood.hip -- island flood (gfx950): one 256-thread workgroup per window, block bitsets
+// bsf_flood.hip -- island flood (gfx950): one 512-thread workgroup per window, block bitsets
 // of the window in LDS (or, for the larger windows, in a per-workgroup global scratch that
 // stays in L2).  Used by the search engine (one window per Stage-2 survivor) and by the region
 // scanner (windows of up to 49 x 49 chunks).
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(kFloodThreads) void pass_flood_kernel(EngineArgs a,
 // Region scanner: one workgroup per window of one flood bucket, bitsets in the workgroup's
 // slice of the global scratch.
 template <int W>
-__global__ __launch_bounds__(256) void region_flood_kernel(RegionArgs a, int bucket) {
+__global__ __launch_bounds__(kFloodThreads) void region_flood_kernel(RegionArgs a, int bucket) {
     __shared__ FloodShared<W> sh;
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;

@@ -47,24 +47,24 @@ int resident_grid(K kernel, int block, int cap_per_cu) {
 
 void launch_flood(int bucket, int grid, RegionArgs& a, hipStream_t s) {
     switch (bucket) {
-        case 0: hipLaunchKernelGGL(region_flood_kernel<1>, dim3(grid), dim3(256), 0, s, a, bucket); break;
-        case 1: hipLaunchKernelGGL(region_flood_kernel<2>, dim3(grid), dim3(256), 0, s, a, bucket); break;
-        case 2: hipLaunchKernelGGL(region_flood_kernel<4>, dim3(grid), dim3(256), 0, s, a, bucket); break;
-        case 3: hipLaunchKernelGGL(region_flood_kernel<7>, dim3(grid), dim3(256), 0, s, a, bucket); break;
-        case 4: hipLaunchKernelGGL(region_flood_kernel<9>, dim3(grid), dim3(256), 0, s, a, bucket); break;
-        default: hipLaunchKernelGGL(region_flood_kernel<13>, dim3(grid), dim3(256), 0, s, a, bucket); break;
+        case 0: hipLaunchKernelGGL(region_flood_kernel<1>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
+        case 1: hipLaunchKernelGGL(region_flood_kernel<2>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
+        case 2: hipLaunchKernelGGL(region_flood_kernel<4>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
+        case 3: hipLaunchKernelGGL(region_flood_kernel<7>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
+        case 4: hipLaunchKernelGGL(region_flood_kernel<9>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
+        default: hipLaunchKernelGGL(region_flood_kernel<13>, dim3(grid), dim3(kFloodThreads), 0, s, a, bucket); break;
     }
 }
 
 int flood_resident(int bucket) {
     // at most two windows per CU: each holds megabytes of bitsets in the scratch
     switch (bucket) {
-        case 0: return resident_grid(region_flood_kernel<1>, 256, 2);
-        case 1: return resident_grid(region_flood_kernel<2>, 256, 2);
-        case 2: return resident_grid(region_flood_kernel<4>, 256, 2);
-        case 3: return resident_grid(region_flood_kernel<7>, 256, 2);
-        case 4: return resident_grid(region_flood_kernel<9>, 256, 2);
-        default: return resident_grid(region_flood_kernel<13>, 256, 2);
+        case 0: return resident_grid(region_flood_kernel<1>, kFloodThreads, 2);
+        case 1: return resident_grid(region_flood_kernel<2>, kFloodThreads, 2);
+        case 2: return resident_grid(region_flood_kernel<4>, kFloodThreads, 2);
+        case 3: return resident_grid(region_flood_kernel<7>, kFloodThreads, 2);
+        case 4: return resident_grid(region_flood_kernel<9>, kFloodThreads, 2);
+        default: return resident_grid(region_flood_kernel<13>, kFloodThreads, 2);
     }
 }
 
