@@ -25,7 +25,8 @@ constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
 struct GenStage {
     uint32_t perm[128];
     double off[3];
-    YTable yt;  // Y axis of the staged 3D octave (stage_generator_y)
+    YTable yt;         // Y axis of the staged 3D octave (stage_generator_y)
+    uint2 gcoef[256];  // grad3_coefficients of every permutation byte (filled by OneWorld)
 };
 
 // Stage generator g of the current world into LDS (every thread of the block must call this).
@@ -45,8 +46,10 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
 // Stager for one world per workgroup: every lane reads the same staged generator.  The
 // cache entry of the next generator (g + 1, the usual successor) is loaded into registers
 // while generator g is evaluated, so a stage only waits for its LDS stores.
-struct OneWorld {
+template <bool FMA_GRAD>
+struct OneWorldT {
     static constexpr bool kUniform = true;
+    static constexpr bool kFmaGrad = FMA_GRAD;  // 3D corner gradients from the coefficient table
     GenStage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
@@ -54,6 +57,7 @@ struct OneWorld {
     mutable uint32_t pf_perm = 0;
     mutable double pf_off = 0.0;  // offset[threadIdx.x] for threads 0..2
     mutable double pf_b = 0.0;    // offset[1] in every thread (the y offset)
+    mutable bool gc_ready = false;  // st->gcoef written by this stager
 
     __device__ __forceinline__ void fetch(int g) const {
         const int t = threadIdx.x;
@@ -66,6 +70,10 @@ struct OneWorld {
         if (pf_g != g) fetch(g);
         __syncthreads();
         const int t = threadIdx.x;
+        if (FMA_GRAD && !gc_ready) {  // published by the barrier that follows every publish
+            for (int k = t; k < 256; k += blockDim.x) st->gcoef[k] = grad3_coefficients(k);
+            gc_ready = true;
+        }
         if (t < 64) {
             st->perm[t] = pf_perm;
             st->perm[t + 64] = pf_perm;
@@ -90,9 +98,14 @@ struct OneWorld {
         stage_yn<Y1 - Y0>(g, sy, Y0);
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
+    __device__ __forceinline__ const uint2* gcoef() const { return FMA_GRAD ? st->gcoef : nullptr; }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
     __device__ __forceinline__ double off(int i) const { return st->off[i]; }
 };
+// OneWorld evaluates the 3D corners by case selection, OneWorldFma by the coefficient table
+// (fewer VALU instructions, one more LDS read per corner; which is faster depends on the kernel).
+using OneWorld = OneWorldT<false>;
+using OneWorldFma = OneWorldT<true>;
 
 // Stager for up to kMaxWorldsPerWave consecutive cache slots in one 64-lane workgroup: lane
 // group `sub` evaluates world first_slot + sub.  Used when a world needs few columns.
@@ -155,7 +168,7 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
                 const Axis ax = axis(X * sx + s.off(0));
                 const Axis az = axis(Z * sx + s.off(2));
                 double v[NY];
-                perlin3_column_table<Y0, Y1>(s.perm(), ax, az, yt, fresh, w, v);
+                perlin3_column_table<Y0, Y1>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
                 for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
@@ -178,8 +191,8 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
 
 // colpar_3d for one world per workgroup with the first level y0 a run-time value (levels
 // [y0, y0 + NY)), so that a kernel can walk level blocks in a loop.
-template <int NY>
-__device__ __forceinline__ void colpar_3d_n(const OneWorld& s, int g0, int octaves, double s_xz, double s_y, double X,
+template <int NY, class S>
+__device__ __forceinline__ void colpar_3d_n(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
                                             double Z, bool active, double* acc, int y0) {
     double amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
@@ -191,7 +204,7 @@ __device__ __forceinline__ void colpar_3d_n(const OneWorld& s, int g0, int octav
             const Axis ax = axis(X * sx + s.off(0));
             const Axis az = axis(Z * sx + s.off(2));
             double v[NY];
-            perlin3_column_table<0, NY>(s.perm(), ax, az, yt, fresh, w, v);
+            perlin3_column_table<0, NY>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
             for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
         }
@@ -253,8 +266,8 @@ __device__ __forceinline__ void colpar_climate_all(const S& s, const double* BX,
 }
 
 // The three 3D octave groups of one lattice column over levels [y0, y0 + NY), y0 at run time.
-template <int NY>
-__device__ __forceinline__ void colpar_noise3_n(const OneWorld& s, double X, double Z, bool active, ColAcc<0, NY>& a,
+template <int NY, class S>
+__device__ __forceinline__ void colpar_noise3_n(const S& s, double X, double Z, bool active, ColAcc<0, NY>& a,
                                                 int y0) {
     colpar_3d_n<NY>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, a.n2, y0);
     colpar_3d_n<NY>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, a.n3, y0);

@@ -225,6 +225,28 @@ __device__ __forceinline__ double grad3(int h, double x, double y, double z) {
     return flip_sign(u, uh << 31) + flip_sign(v, (uh << 30) & 0x80000000u);
 }
 
+// grad3 as (a x + b y) + c z with a, b, c in {-1, 0, +1}: for coordinates that are all nonzero
+// the two products that matter are exact and the zero term cannot change the sum (a nonzero
+// sum is unchanged by adding +/-0, an exactly cancelling one is +0 either way), so
+// fma(a, x, fma(b, y, c * z)) is bit for bit grad3's (+/-u) + (+/-v).  The coefficients of
+// table entry k (a permutation byte, h = k & 15) are packed as x: high word of a | high 12
+// bits of c, y: high word of b (all low words are 0).
+__host__ __device__ inline uint2 grad3_coefficients(int k) {
+    const int h = k & 15;
+    uint32_t hi[3] = {0, 0, 0};                       // x, y, z
+    const int u = h < 8 ? 0 : 1;                      // u = x for h < 8, else y
+    const int v = h < 4 ? 1 : ((h == 12 || h == 14) ? 0 : 2);
+    hi[u] = 0x3FF00000u | ((h & 1) ? 0x80000000u : 0u);
+    hi[v] = 0x3FF00000u | ((h & 2) ? 0x80000000u : 0u);
+    return uint2{hi[0] | (hi[2] >> 20), hi[1]};
+}
+__device__ __forceinline__ double grad3_fma(uint2 c, double x, double y, double z) {
+    const double a = __hiloint2double(static_cast<int>(c.x & 0xFFF00000u), 0);
+    const double b = __hiloint2double(static_cast<int>(c.y), 0);
+    const double cz = __hiloint2double(static_cast<int>(c.x << 20), 0);
+    return __fma_rn(a, x, __fma_rn(b, y, cz * z));
+}
+
 // a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0: Java
 // forms u = (1 - bit 3) * x and v = 0.0 for h < 4, x for h in {12, 14}, else z, then the same
 // sign flips as grad3.  That is grad3's selection with u's alternative the signed zero 0.0 * x
@@ -386,6 +408,7 @@ struct YTable {
     double cell_f[kMaxLevels], cell_fm1[kMaxLevels], fd[kMaxLevels];
     int yi[kMaxLevels];
     uint32_t fresh;
+    uint32_t yzero;  // levels whose cell starts at a zero fraction (no FMA gradients there)
 };
 
 // Called by every lane of wave 0; levels [y0, y0 + NY).
@@ -424,7 +447,11 @@ __device__ __forceinline__ void build_ytable_n(YTable& t, double sy, double b, i
         t.fd[k] = fade(f);
         t.yi[k] = idx;
     }
-    if (k == 0) t.fresh = mask;
+    const uint32_t zero = static_cast<uint32_t>(__ballot(mine && cf == 0.0));
+    if (k == 0) {
+        t.fresh = mask;
+        t.yzero = zero;
+    }
 }
 template <int Y0, int Y1>
 __device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
@@ -432,9 +459,12 @@ __device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
 }
 
 // perlin3_column_cells with the Y axis read from a shared table; `fresh` must be uniform.
-template <int Y0, int Y1, class Perm>
-__device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YTable& yt,
-                                                     uint32_t fresh, double weight, double* out) {
+// With gc (the grad3_coefficients table of the 256 permutation bytes, in LDS) the corner
+// gradients are three FP64 ops each instead of the case selection, whenever no corner
+// coordinate of the wave's cells is zero (checked per call; the selection path otherwise).
+template <int Y0, int Y1, bool FMA, class Perm>
+__device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
+                                                    const YTable& yt, uint32_t fresh, double weight, double* out) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
     double q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
@@ -450,16 +480,37 @@ __device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, con
             const int p1 = xp1 + yi;
             const int p10 = p.get(p1) + az.i;
             const int p11 = p.get(p1 + 1) + az.i;
-            q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
-            q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
-            q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
-            q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+            if constexpr (FMA) {
+                q00 = lerp(ax.fd, grad3_fma(gc[p.get(p00)], ax.f, yf, az.f), grad3_fma(gc[p.get(p10)], ax.fm1, yf, az.f));
+                q01 = lerp(ax.fd, grad3_fma(gc[p.get(p01)], ax.f, yfm1, az.f),
+                           grad3_fma(gc[p.get(p11)], ax.fm1, yfm1, az.f));
+                q10 = lerp(ax.fd, grad3_fma(gc[p.get(p00 + 1)], ax.f, yf, az.fm1),
+                           grad3_fma(gc[p.get(p10 + 1)], ax.fm1, yf, az.fm1));
+                q11 = lerp(ax.fd, grad3_fma(gc[p.get(p01 + 1)], ax.f, yfm1, az.fm1),
+                           grad3_fma(gc[p.get(p11 + 1)], ax.fm1, yfm1, az.fm1));
+            } else {
+                q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
+                q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
+                q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
+                q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1),
+                           grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+            }
         }
         const double fd = yt.fd[k];
         const double r0 = lerp(fd, q00, q01);
         const double r1 = lerp(fd, q10, q11);
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
+}
+template <int Y0, int Y1, class Perm>
+__device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YTable& yt,
+                                                     uint32_t fresh, double weight, double* out,
+                                                     const uint2* gc = nullptr) {
+    // x and z cells start at ax.f / az.f; their other corners (f - 1) are never zero
+    if (gc != nullptr && (yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0))
+        perlin3_table_cells<Y0, Y1, true>(p, gc, ax, az, yt, fresh, weight, out);
+    else
+        perlin3_table_cells<Y0, Y1, false>(p, gc, ax, az, yt, fresh, weight, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;
         const int nc = (4 * cx_n + 1) * nzl;
-        const OneWorld s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+        const OneWorldFma s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
                          a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
         double* lat = a.lattice + static_cast<size_t>(it) * a.lat_stride;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)

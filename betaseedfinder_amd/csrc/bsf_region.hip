@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kRegionItemCols, 3) void region_lattice_kernel(Regi
         const int u = active ? static_cast<int>(c / nzl) : 0, v = active ? static_cast<int>(c % nzl) : 0;
         const double X = static_cast<double>(p.chunk_x0 * 4 + u);
         const double Z = static_cast<double>(p.chunk_z0 * 4 + v);
-        const OneWorld s{&st, a.perms + static_cast<size_t>(t.slot) * kGenCount * 64,
+        const OneWorldFma s{&st, a.perms + static_cast<size_t>(t.slot) * kGenCount * 64,
                          a.offsets + static_cast<size_t>(t.slot) * kGenCount * 3};
         ColAcc<8, 16> acc;
         colpar_noise<8, 16, true>(s, X, Z, active, acc);

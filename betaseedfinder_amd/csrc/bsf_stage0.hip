@@ -88,7 +88,7 @@ struct S0Column {
 template <int Y0, int Y1, bool WITH_SHAPE>
 __device__ __forceinline__ void evaluate_cached_column(GenStage& st, const uint32_t* perms, const double* offs,
                                                        double X, double Z, bool active, S0Column<Y0, Y1>& c) {
-    const OneWorld s{&st, perms, offs};
+    const OneWorldFma s{&st, perms, offs};
     colpar_noise<Y0, Y1, WITH_SHAPE>(s, X, Z, active, c.acc);
     if (WITH_SHAPE) {
         const double BX = X * 4.0 + 2.0, BZ = Z * 4.0 + 2.0;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(64, 3) void s0_coarse_signs_kernel(S0Args a) {
             // the column shape (2D noise and climate) first, then the 3D sums of levels 0..7 and
             // of levels 8..15 in one loop body: eight levels of three octave sums at a time, with
             // nothing else live, fit two waves per SIMD
-            const OneWorld s{&st, perms, offs};
+            const OneWorldFma s{&st, perms, offs};
             ColumnShape cs;
             {
                 const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
