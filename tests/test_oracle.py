@@ -449,6 +449,46 @@ def test_splitmix_seeds_match_reference_fixture(oracle):
     assert (z ^ (z >> 31)) != 0xE220A8397B1DCDAF
 
 
+def test_fma_gradient_form_is_exact():
+    """The kernels' fma(a, x, fma(b, y, c * z)) with a, b, c in {-1, 0, 1} equals Java's
+    grad (+/-u) + (+/-v) (NoiseGeneratorPerlin173.java:145-165) bit for bit when no coordinate
+    is zero, including exactly cancelling sums; with a zero coordinate it can differ in the
+    sign of a zero result, which is why the kernels check for zero coordinates."""
+    import random
+    import struct
+    libm = ctypes.CDLL("libm.so.6")
+    libm.fma.restype = ctypes.c_double
+    libm.fma.argtypes = [ctypes.c_double] * 3
+
+    def bits(v):
+        return struct.unpack("<Q", struct.pack("<d", v))[0]
+
+    def java_grad(h, x, y, z):
+        u = x if h < 8 else y
+        v = y if h < 4 else (x if h in (12, 14) else z)
+        return (-u if h & 1 else u) + (-v if h & 2 else v)
+
+    def coefficients(h):  # the same rule, as the kernels' table holds it
+        c = [0.0, 0.0, 0.0]
+        c[0 if h < 8 else 1] = -1.0 if h & 1 else 1.0
+        c[1 if h < 4 else (0 if h in (12, 14) else 2)] = -1.0 if h & 2 else 1.0
+        return c
+
+    rng = random.Random(3)
+    coords = [rng.random() for _ in range(200)] + [rng.random() - 1.0 for _ in range(200)]
+    coords += [2.0 ** -52, 1.0 - 2.0 ** -53, -1.0, -2.0 ** -53, 0.5, -0.5, 0.25]
+    for h in range(16):
+        a, b, c = coefficients(h)
+        for _ in range(400):
+            x, y, z = rng.choice(coords), rng.choice(coords), rng.choice(coords)
+            for xx, yy, zz in ((x, y, z), (x, x, x), (y, y, z), (x, -x, x)):
+                got = libm.fma(a, xx, libm.fma(b, yy, c * zz))
+                assert bits(got) == bits(java_grad(h, xx, yy, zz)), (h, xx, yy, zz)
+    # a zero coordinate: h = 3 gives -x - y = -0.0 for x = y = +0.0, the FMA form +0.0
+    a, b, c = coefficients(3)
+    assert bits(java_grad(3, 0.0, 0.0, 0.5)) != bits(libm.fma(a, 0.0, libm.fma(b, 0.0, c * 0.5)))
+
+
 def test_scout_statistic_is_octave_zero_sum(oracle):
     """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
     generator's octave-0 values (0.0 + v), index order x, z, y."""
