@@ -489,6 +489,20 @@ def test_fma_gradient_form_is_exact():
     assert bits(java_grad(3, 0.0, 0.0, 0.5)) != bits(libm.fma(a, 0.0, libm.fma(b, 0.0, c * 0.5)))
 
 
+def test_nextint_redraw_threshold_matches_java_test():
+    """The kernels redraw nextInt(d) when bits >= d * floor(2^31 / d) (never for d = 2^k);
+    Java redraws when bits - bits % d + (d - 1) overflows an int (Random.nextInt)."""
+    import random
+    rng = random.Random(9)
+    for d in range(1, 257):
+        limit = (1 << 31) if d & (d - 1) == 0 else d * ((1 << 31) // d)
+        cand = [0, 1, (1 << 31) - 1] + [min((1 << 31) - 1, max(0, limit + k)) for k in range(-260, 260)]
+        cand += [rng.randrange(1 << 31) for _ in range(300)]
+        for b in cand:
+            java = d & (d - 1) != 0 and ((b - b % d + d - 1) & 0xFFFFFFFF) >= 0x80000000
+            assert java == (b >= limit), (d, b)
+
+
 def test_scout_statistic_is_octave_zero_sum(oracle):
     """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
     generator's octave-0 values (0.0 + v), index order x, z, y."""
