@@ -302,6 +302,7 @@ struct YAxis {
     double cell_f[Y1 - Y0];
     double cell_fm1[Y1 - Y0];
     uint32_t fresh;  // bit k: level Y0 + k starts the cell run inside [Y0, Y1) (always bit 0)
+    uint32_t yzero;  // bit k: level Y0 + k's cell starts at a zero fraction
 
     __device__ __forceinline__ void build(double sy, double b) {
         // walk back from Y0 to the start of its cell (only levels below Y0 that share Y0's index)
@@ -319,6 +320,7 @@ struct YAxis {
         }
         int prev_i = fl0 & 255;
         fresh = 1u;
+        yzero = 0u;
 #pragma unroll
         for (int k = 0; k < Y1 - Y0; ++k) {
             const int y = Y0 + k;
@@ -335,15 +337,17 @@ struct YAxis {
             fd[k] = fade(f);
             cell_f[k] = start_f;
             cell_fm1[k] = start_f - 1.0;
+            if (start_f == 0.0) yzero |= 1u << k;
         }
     }
 };
 
 // One 3D octave at one lattice column, all levels [Y0, Y1): out[k] = value * weight
-// (the per-point addend of :281-282).
-template <int Y0, int Y1, class Perm>
-__device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
-                                               double weight, double* out) {
+// (the per-point addend of :281-282).  FMA: corner gradients from the grad3_coefficients table
+// gc (only for nonzero corner coordinates; see grad3_coefficients).
+template <int Y0, int Y1, bool FMA, class Perm>
+__device__ __forceinline__ void perlin3_column_g(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
+                                                 const YAxis<Y0, Y1>& ya, double weight, double* out) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
 #pragma unroll
@@ -357,14 +361,32 @@ __device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axi
         const int p1 = xp1 + yi;
         const int p10 = p.get(p1) + az.i;
         const int p11 = p.get(p1 + 1) + az.i;
-        const double q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
-        const double q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
-        const double q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
-        const double q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        double q00, q01, q10, q11;
+        if constexpr (FMA) {
+            q00 = lerp(ax.fd, grad3_fma(gc[p.get(p00)], ax.f, yf, az.f), grad3_fma(gc[p.get(p10)], ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, grad3_fma(gc[p.get(p01)], ax.f, yfm1, az.f), grad3_fma(gc[p.get(p11)], ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, grad3_fma(gc[p.get(p00 + 1)], ax.f, yf, az.fm1),
+                       grad3_fma(gc[p.get(p10 + 1)], ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, grad3_fma(gc[p.get(p01 + 1)], ax.f, yfm1, az.fm1),
+                       grad3_fma(gc[p.get(p11 + 1)], ax.fm1, yfm1, az.fm1));
+        } else {
+            q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+        }
         const double r0 = lerp(ya.fd[k], q00, q01);
         const double r1 = lerp(ya.fd[k], q10, q11);
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
+}
+template <int Y0, int Y1, class Perm>
+__device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
+                                               double weight, double* out, const uint2* gc = nullptr) {
+    if (gc != nullptr && __all(ya.yzero == 0u && ax.f != 0.0 && az.f != 0.0))
+        perlin3_column_g<Y0, Y1, true>(p, gc, ax, az, ya, weight, out);
+    else
+        perlin3_column_g<Y0, Y1, false>(p, gc, ax, az, ya, weight, out);
 }
 
 // Same values as perlin3_column, with the corner lerps computed once per cell run as the

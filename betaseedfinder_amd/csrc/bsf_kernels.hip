@@ -214,7 +214,10 @@ __global__ __launch_bounds__(256) void splitmix_seeds_kernel(uint64_t sequence_s
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64, 2) void scout_kernel(ScoutArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    __shared__ uint2 s_gc[256];
     const int lane = threadIdx.x;
+    for (int k = lane; k < 256; k += 64) s_gc[k] = grad3_coefficients(k);
+    __syncthreads();
     LanePerm P;
     P.base = perm_lds + lane * 4;
     const int64_t groups = (a.count + 63) >> 6;
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(64, 2) void scout_kernel(ScoutArgs a) {
             for (int z = 0; z < 5; ++z) {
                 const Axis az = axis(static_cast<double>(z) * 684.412 + g.c);
                 double v[17];
-                perlin3_column<0, 17>(P, ax, az, ya, 1.0, v);
+                perlin3_column<0, 17>(P, ax, az, ya, 1.0, v, s_gc);
 #pragma unroll
                 for (int k = 0; k < 17; ++k) s += 0.0 + v[k];
             }
