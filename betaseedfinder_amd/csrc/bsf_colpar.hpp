@@ -9,6 +9,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 #include "bsf_device.hpp"
@@ -25,9 +27,20 @@ constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
 struct GenStage {
     uint32_t perm[128];
     double off[3];
-    YTable yt;         // Y axis of the staged 3D octave (stage_generator_y)
-    uint2 gcoef[256];  // grad3_coefficients of every permutation byte (filled by OneWorld)
+    YTable yt;  // Y axis of the staged 3D octave (stage_generator_y)
 };
+// The same with the kernel's grad3_coefficients table (OneWorldFma; filled by init_gcoef).
+struct GenStageGc : GenStage {
+    uint2 gcoef[256];
+};
+
+// Fills a kernel's 256-entry grad3_coefficients table in LDS; kernels that evaluate with
+// OneWorldFma call it once before their item loop (every thread of the block; the barrier
+// publishes the table).
+__device__ __forceinline__ void init_gcoef(uint2* t) {
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) t[k] = grad3_coefficients(k);
+    __syncthreads();
+}
 
 // Stage generator g of the current world into LDS (every thread of the block must call this).
 __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __restrict__ perms,
@@ -49,15 +62,15 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
 template <bool FMA_GRAD>
 struct OneWorldT {
     static constexpr bool kUniform = true;
-    static constexpr bool kFmaGrad = FMA_GRAD;  // 3D corner gradients from the coefficient table
-    GenStage* st;
+    static constexpr bool kFmaGrad = FMA_GRAD;  // 3D corner gradients from st->gcoef
+    using Stage = std::conditional_t<FMA_GRAD, GenStageGc, GenStage>;
+    Stage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
     mutable int pf_g = -1;  // generator held in the prefetch registers
     mutable uint32_t pf_perm = 0;
     mutable double pf_off = 0.0;  // offset[threadIdx.x] for threads 0..2
     mutable double pf_b = 0.0;    // offset[1] in every thread (the y offset)
-    mutable bool gc_ready = false;  // st->gcoef written by this stager
 
     __device__ __forceinline__ void fetch(int g) const {
         const int t = threadIdx.x;
@@ -70,10 +83,7 @@ struct OneWorldT {
         if (pf_g != g) fetch(g);
         __syncthreads();
         const int t = threadIdx.x;
-        if (FMA_GRAD && !gc_ready) {  // published by the barrier that follows every publish
-            for (int k = t; k < 256; k += blockDim.x) st->gcoef[k] = grad3_coefficients(k);
-            gc_ready = true;
-        }
+
         if (t < 64) {
             st->perm[t] = pf_perm;
             st->perm[t + 64] = pf_perm;
@@ -98,7 +108,10 @@ struct OneWorldT {
         stage_yn<Y1 - Y0>(g, sy, Y0);
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
-    __device__ __forceinline__ const uint2* gcoef() const { return FMA_GRAD ? st->gcoef : nullptr; }
+    __device__ __forceinline__ const uint2* gcoef() const {
+        if constexpr (FMA_GRAD) return st->gcoef;
+        else return nullptr;
+    }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
     __device__ __forceinline__ double off(int i) const { return st->off[i]; }
 };

@@ -128,7 +128,8 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
     for (int b = 0; b < 2; ++b)
-        grid_s2_[b] = occupancy_grid(pass_stage2_kernel, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
+        grid_s2_[b] = b == 0 ? occupancy_grid(pass_stage2_kernel<true>, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0)
+                             : occupancy_grid(pass_stage2_kernel<false>, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
     grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(0), 0);
     grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(1), 0);
     grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, kFloodThreads, flood_lds_bytes(2), 0);
@@ -291,7 +292,8 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
     if (stage1_only) return;
     {
         LaunchTimer t("island_stage2", s, timers_);
-        hipLaunchKernelGGL(pass_stage2_kernel, dim3(grid_s2_[bucket]), dim3(64),
+        auto s2 = bucket == 0 ? pass_stage2_kernel<true> : pass_stage2_kernel<false>;
+        hipLaunchKernelGGL(s2, dim3(grid_s2_[bucket]), dim3(64),
                            static_cast<size_t>(2 * kNdBucket[bucket]) * sizeof(double), s, a);
         check(hipGetLastError(), "stage 2 launch");
         t.stop(0);

@@ -492,12 +492,14 @@ constexpr int kS2Threads = 64;
 
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
+template <bool FMA>
 __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a) {
-    __shared__ GenStage st;
+    __shared__ typename OneWorldT<FMA>::Stage st;
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
     if (a.stage1_only) return;
+    if constexpr (FMA) init_gcoef(st.gcoef);
     unsigned long long nsurv = a.ctr[kCtrSurv];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
     for (;;) {
@@ -515,8 +517,8 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;
         const int nc = (4 * cx_n + 1) * nzl;
-        const OneWorldFma s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
-                         a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
+        const OneWorldT<FMA> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                               a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
         double* lat = a.lattice + static_cast<size_t>(it) * a.lat_stride;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)
         for (int base = 0; base < nd; base += kS2Threads) {
@@ -553,5 +555,8 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         }
     }
 }
+
+template __global__ void pass_stage2_kernel<true>(EngineArgs);
+template __global__ void pass_stage2_kernel<false>(EngineArgs);
 
 }  // namespace bsf

@@ -67,9 +67,10 @@ __global__ __launch_bounds__(kRegionItemCols, 2) void region_climate_kernel(Regi
 // The window's noise lattice, levels 8..15, one noise column per lane, emitted into the
 // density columns it belongs to (with the climate the previous kernel wrote).
 __global__ __launch_bounds__(kRegionItemCols, 3) void region_lattice_kernel(RegionArgs a) {
-    __shared__ GenStage st;
+    __shared__ GenStageGc st;
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
+    init_gcoef(st.gcoef);
     for (;;) {
         const int64_t item = region_next_item(&a.ctr[kRegionCtrLattice], &s_item);
         if (item >= a.lat_items) break;
