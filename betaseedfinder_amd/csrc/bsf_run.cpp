@@ -526,6 +526,25 @@ void RunWriter::close(const std::string& status, int64_t checked, int64_t start_
     flush_rows();
     write_boards();
     write_file(path("performance_profile.txt"), "Performance profiler disabled.\n");
+    write_manifest(status, checked, start_ms, end_ms);
+    // publishLatestRunCopies (:4483-4510)
+    copy_file(path("hunter_candidates.csv"), root_ + "/hunter_candidates.csv");
+    copy_file(path("top_largest_results.txt"), root_ + "/top_largest_results.txt");
+    copy_file(path("top_largest_results.txt"), root_ + "/top_floating_results.txt");
+    copy_file(path("top_filled_footprint_results.txt"), root_ + "/top_filled_footprint_results.txt");
+    copy_file(path("top_raw_footprint_results.txt"), root_ + "/top_raw_footprint_results.txt");
+    copy_file(path("top_island_sideboards.csv"), root_ + "/top_island_sideboards.csv");
+    copy_file(path("manifest.txt"), root_ + "/latest_manifest.txt");
+    copy_file(path("performance_profile.txt"), root_ + "/latest_performance_profile.txt");
+}
+
+void RunWriter::fail(int64_t checked, int64_t start_ms, int64_t end_ms) {
+    flush_rows();
+    write_boards();
+    write_manifest("ERROR", checked, start_ms, end_ms);
+}
+
+void RunWriter::write_manifest(const std::string& status, int64_t checked, int64_t start_ms, int64_t end_ms) {
     const int prof = s_.profile;
     const HuntGates& g = kHuntGates[prof];
     const bsf_island_params& t = s_.thresholds;
@@ -589,15 +608,6 @@ void RunWriter::close(const std::string& status, int64_t checked, int64_t start_
     kv("deterministicSeedMode", "true");
     kv("deterministicSeedSequenceSeed", std::to_string(static_cast<int64_t>(s_.sequence_seed)));
     write_file(path("manifest.txt"), m);
-    // publishLatestRunCopies (:4483-4510)
-    copy_file(path("hunter_candidates.csv"), root_ + "/hunter_candidates.csv");
-    copy_file(path("top_largest_results.txt"), root_ + "/top_largest_results.txt");
-    copy_file(path("top_largest_results.txt"), root_ + "/top_floating_results.txt");
-    copy_file(path("top_filled_footprint_results.txt"), root_ + "/top_filled_footprint_results.txt");
-    copy_file(path("top_raw_footprint_results.txt"), root_ + "/top_raw_footprint_results.txt");
-    copy_file(path("top_island_sideboards.csv"), root_ + "/top_island_sideboards.csv");
-    copy_file(path("manifest.txt"), root_ + "/latest_manifest.txt");
-    copy_file(path("performance_profile.txt"), root_ + "/latest_performance_profile.txt");
 }
 
 namespace {
@@ -657,8 +667,11 @@ int bsf_run_open(const char* output_root, const char* run_label, const char* sta
     return unlocked_call([&] {
         if (!output_root || !out) throw ArgError("null run argument");
         validate_settings(s);
-        *out = new bsf_run{RunWriter(output_root, run_label ? run_label : "", stamp ? stamp : stamp_now(), *s),
-                           start_ms > 0 ? start_ms : now_ms()};
+        std::unique_ptr<bsf_run> r(new bsf_run{
+            RunWriter(output_root, run_label ? run_label : "", stamp ? stamp : stamp_now(), *s),
+            start_ms > 0 ? start_ms : now_ms()});
+        r->w.write_manifest("RUNNING", 0, r->start_ms, 0);  // at search start (:761)
+        *out = r.release();
     });
 }
 
@@ -683,7 +696,12 @@ int bsf_run_close(bsf_run* r, const char* status, int64_t checked, int64_t end_m
     return unlocked_call([&] {
         if (!r) throw ArgError("null run argument");
         std::unique_ptr<bsf_run> owned(r);
-        owned->w.close(status ? status : "FINISHED", checked, owned->start_ms, end_ms > 0 ? end_ms : now_ms());
+        const std::string st = status ? status : "FINISHED";
+        const int64_t end = end_ms > 0 ? end_ms : now_ms();
+        if (st == "ERROR")
+            owned->w.fail(checked, owned->start_ms, end);
+        else
+            owned->w.close(st, checked, owned->start_ms, end);
     });
 }
 
@@ -695,6 +713,7 @@ int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, 
         const int64_t t0 = now_ms();
         const auto c0 = std::chrono::steady_clock::now();
         RunWriter w(output_root, run_label ? run_label : "", stamp ? stamp : stamp_now(), *s);
+        w.write_manifest("RUNNING", 0, t0, 0);  // at search start (:761)
         const HuntGates& g = kHuntGates[s->profile];
         const int coarse_min = coarse_threshold(*s);
         const int centers = 8, batch = 32768;
@@ -704,6 +723,8 @@ int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, 
         int64_t issued_regions = 0, issued_worlds = 0, survivors = 0;
         static const int kCx[8] = {0, 15, 0, 15, -15, -15, 0, 15};
         static const int kCz[8] = {0, 0, 15, 15, 0, 15, -15, -15};
+        // a failed batch leaves the files of the batches before it and an ERROR manifest (:925-934)
+        try {
         while (issued_regions < s->regions) {
             const int64_t regions = std::min<int64_t>(int64_t(batch) * centers, s->regions - issued_regions);
             const int count = static_cast<int>((regions + centers - 1) / centers);
@@ -745,6 +766,10 @@ int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, 
             survivors += static_cast<int64_t>(items.size());
             issued_worlds += count;
             issued_regions += regions;
+        }
+        } catch (...) {
+            w.fail(issued_regions, t0, now_ms());
+            throw;
         }
         w.close("FINISHED", issued_regions, t0, now_ms());
         if (out) {

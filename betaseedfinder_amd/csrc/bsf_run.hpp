@@ -65,6 +65,11 @@ class RunWriter {
     bool add(const bsf_run_attempt& a, const bsf_region_result& r);
     // Final leaderboard files, the manifest, the performance file and the latest-run copies.
     void close(const std::string& status, int64_t checked, int64_t start_ms, int64_t end_ms);
+    // A failed run: the candidate rows and leaderboards so far and an ERROR manifest, no
+    // latest-run copies (search(), :925-934).
+    void fail(int64_t checked, int64_t start_ms, int64_t end_ms);
+    // manifest.txt (writeManifest, :4309-4449); status RUNNING at the start (end_ms 0).
+    void write_manifest(const std::string& status, int64_t checked, int64_t start_ms, int64_t end_ms);
     const std::string& dir() const { return dir_; }
 
     int64_t matches = 0, duplicates = 0, top_updates = 0, attempts = 0;

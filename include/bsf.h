@@ -230,14 +230,18 @@ typedef struct bsf_run_attempt {
 } bsf_run_attempt;
 
 typedef struct bsf_run bsf_run;
-/* stamp: the folder's "yyyy-MM-dd_HH-mm-ss" (NULL: local time now); start_ms <= 0: now. */
+/* Creates the run folder, its initial files and a RUNNING manifest (initializeRunOutput,
+ * :4230-4299, and search(), :761).  stamp: the folder's "yyyy-MM-dd_HH-mm-ss" (NULL: local
+ * time now); start_ms <= 0: now. */
 int bsf_run_open(const char* output_root, const char* run_label, const char* stamp, int64_t start_ms,
                  const bsf_run_settings* settings, bsf_run** out);
 /* Region attempts in attempt order; *unique_match (optional) = 1 for a new match. */
 int bsf_run_add(bsf_run* run, const bsf_run_attempt* attempt, const bsf_region_result* result,
                 int32_t* unique_match);
 int bsf_run_dir(const bsf_run* run, char* buf, int64_t cap);
-/* Writes the final files (status "FINISHED", "STOPPED" ...; end_ms <= 0: now) and frees run. */
+/* Writes the final files (status "FINISHED" or "STOPPED"; end_ms <= 0: now) and frees run.
+ * Status "ERROR" writes the rows, leaderboards and manifest only, without the latest-run
+ * copies in the output root (search(), :925-934). */
 int bsf_run_close(bsf_run* run, const char* status, int64_t checked, int64_t end_ms);
 
 typedef struct bsf_hunt_summary {
@@ -249,7 +253,8 @@ typedef struct bsf_hunt_summary {
  * processGpuRegionAttempt, :1878-1971): world seeds of attempts 1, 2, ... of the sequence
  * (deterministicSeedMode), stage0stream batches of 32,768 worlds x 8 centres, the region
  * attempts of every record with reject stage 0, in attempt order, into a run folder.
- * run_dir (optional) receives the folder's path. */
+ * run_dir (optional) receives the folder's path.  When a batch fails, the folder keeps the
+ * batches before it and an ERROR manifest, and the call returns the batch's error. */
 int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, const bsf_run_settings* settings,
              bsf_hunt_summary* out, char* run_dir, int64_t run_dir_cap);
 

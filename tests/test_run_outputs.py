@@ -143,3 +143,26 @@ def test_run_writer_rejects_bad_settings(tmp_path):
         bsf.RunWriter(str(tmp_path), bsf.run_settings(top_keep=0))
     with pytest.raises(bsf.BsfError):
         bsf.RunWriter(str(tmp_path), bsf.run_settings(chunk_radius=25))
+
+
+def test_manifest_status_through_the_run(tmp_path):
+    """RUNNING from the start (no end time, no elapsed line), FINISHED with the copies in the
+    output root; an ERROR close keeps the rows and boards but publishes no copies."""
+    import betaseedfinder_amd as bsf
+    attempts, results = _synthetic_run(30, seed=5)
+    w = bsf.RunWriter(str(tmp_path / "a"), bsf.run_settings(regions=240), stamp="2026-03-04_05-06-07", start_ms=7000)
+    m = open(os.path.join(w.dir, "manifest.txt")).read()
+    assert "status=RUNNING\n" in m and "startEpochMs=7000\nendEpochMs=0\nchecked=0\n" in m
+    assert "elapsedSeconds" not in m
+    for a, r in zip(attempts, results):
+        w.add(a, r)
+    w.close("ERROR", 120, 9500)
+    got = ro.read_run(w.dir)
+    want = ro.run_files(attempts, results, w.dir, 50)
+    del want["performance_profile.txt"]  # written by a finished run only
+    assert set(got) == set(want) | {"manifest.txt"}
+    for name in want:
+        assert got[name] == want[name], name
+    assert "status=ERROR\n" in got["manifest.txt"] and "checked=120\n" in got["manifest.txt"]
+    assert "elapsedSeconds=2.500\n" in got["manifest.txt"]
+    assert sorted(os.listdir(tmp_path / "a")) == ["latest_run.txt", "runs"]
