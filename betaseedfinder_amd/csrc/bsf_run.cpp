@@ -723,52 +723,69 @@ int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, 
         int64_t issued_regions = 0, issued_worlds = 0, survivors = 0;
         static const int kCx[8] = {0, 15, 0, 15, -15, -15, 0, 15};
         static const int kCz[8] = {0, 0, 15, 15, 0, 15, -15, -15};
-        // a failed batch leaves the files of the batches before it and an ERROR manifest (:925-934)
-        try {
-        while (issued_regions < s->regions) {
-            const int64_t regions = std::min<int64_t>(int64_t(batch) * centers, s->regions - issued_regions);
-            const int count = static_cast<int>((regions + centers - 1) / centers);
-            // world attempts are 1-based (runGpuStage0Pipeline, :1321-1325)
-            checked_call(bsf_deterministic_seeds(s->sequence_seed, issued_worlds + 1, count, seeds.data()));
-            bsf_stage0_counters ctr{};
-            checked_call(bsf_stage0_records(seeds.data(), count, s->profile, s->research, centers, recs.data(), &ctr));
-            std::vector<bsf_region_item> items;
-            std::vector<bsf_run_attempt> atts;
-            for (int64_t off = 0; off < regions; ++off) {
-                const uint8_t* r = recs.data() + static_cast<size_t>(off) * rb;
-                bsf_run_attempt a{};
-                a.attempt = issued_regions + off + 1;
-                a.seed = seeds[static_cast<size_t>(off / centers)];
-                a.center_chunk_x = kCx[off % centers];
-                a.center_chunk_z = kCz[off % centers];
-                a.p20 = r[0];
-                a.upper = r[1];
-                a.high = r[2];
-                a.p19_pass = r[3] != 0;
-                a.coarse = r[4] | (r[5] << 8);
-                a.p19_extreme = r[6] != 0;
-                const bool mega = r[7] != 0;
-                uint64_t sb = 0;
-                for (int k = 0; k < 8; ++k) sb |= static_cast<uint64_t>(r[8 + k]) << (8 * k);
-                std::memcpy(&a.p19_score, &sb, 8);
-                // rejectStage 0 (:1366-1376)
-                const bool pass = a.p20 >= g.p20 && a.upper >= g.upper && a.high >= kStage0MinReentry &&
-                                  a.high >= g.high && a.p19_pass && !mega && a.coarse >= coarse_min;
-                if (!pass) continue;
-                atts.push_back(a);
-                items.push_back(bsf_region_item{a.seed, a.center_chunk_x, a.center_chunk_z, a.coarse, 0});
-            }
+        // Survivors wait for a region call until kFlushItems are pending or kFlushBatches
+        // batches have passed (a region call over a few windows costs about what one over
+        // hundreds costs); they reach the writer in attempt order either way.
+        constexpr size_t kFlushItems = 1024;
+        constexpr int kFlushBatches = 8;
+        std::vector<bsf_region_item> items;
+        std::vector<bsf_run_attempt> atts;
+        int64_t written_regions = 0, pending_regions = 0;
+        int pending_batches = 0;
+        auto drain = [&] {
             std::vector<bsf_region_result> res(items.size());
             if (!items.empty())
                 checked_call(bsf_region_attempts(items.data(), static_cast<int64_t>(items.size()), &s->thresholds,
                                                  s->chunk_radius, res.data()));
             for (size_t k = 0; k < items.size(); ++k) w.add(atts[k], res[k]);
             survivors += static_cast<int64_t>(items.size());
-            issued_worlds += count;
-            issued_regions += regions;
-        }
+            items.clear();
+            atts.clear();
+            written_regions += pending_regions;
+            pending_regions = 0;
+            pending_batches = 0;
+        };
+        // a failed call leaves the files of the regions before it and an ERROR manifest (:925-934)
+        try {
+            while (issued_regions < s->regions) {
+                const int64_t regions = std::min<int64_t>(int64_t(batch) * centers, s->regions - issued_regions);
+                const int count = static_cast<int>((regions + centers - 1) / centers);
+                // world attempts are 1-based (runGpuStage0Pipeline, :1321-1325)
+                checked_call(bsf_deterministic_seeds(s->sequence_seed, issued_worlds + 1, count, seeds.data()));
+                bsf_stage0_counters ctr{};
+                checked_call(bsf_stage0_records(seeds.data(), count, s->profile, s->research, centers, recs.data(), &ctr));
+                for (int64_t off = 0; off < regions; ++off) {
+                    const uint8_t* r = recs.data() + static_cast<size_t>(off) * rb;
+                    bsf_run_attempt a{};
+                    a.attempt = issued_regions + off + 1;
+                    a.seed = seeds[static_cast<size_t>(off / centers)];
+                    a.center_chunk_x = kCx[off % centers];
+                    a.center_chunk_z = kCz[off % centers];
+                    a.p20 = r[0];
+                    a.upper = r[1];
+                    a.high = r[2];
+                    a.p19_pass = r[3] != 0;
+                    a.coarse = r[4] | (r[5] << 8);
+                    a.p19_extreme = r[6] != 0;
+                    const bool mega = r[7] != 0;
+                    uint64_t sb = 0;
+                    for (int k = 0; k < 8; ++k) sb |= static_cast<uint64_t>(r[8 + k]) << (8 * k);
+                    std::memcpy(&a.p19_score, &sb, 8);
+                    // rejectStage 0 (:1366-1376)
+                    const bool pass = a.p20 >= g.p20 && a.upper >= g.upper && a.high >= kStage0MinReentry &&
+                                      a.high >= g.high && a.p19_pass && !mega && a.coarse >= coarse_min;
+                    if (!pass) continue;
+                    atts.push_back(a);
+                    items.push_back(bsf_region_item{a.seed, a.center_chunk_x, a.center_chunk_z, a.coarse, 0});
+                }
+                issued_worlds += count;
+                issued_regions += regions;
+                pending_regions += regions;
+                if (items.size() >= kFlushItems || ++pending_batches >= kFlushBatches) drain();
+            }
+            drain();
         } catch (...) {
-            w.fail(issued_regions, t0, now_ms());
+            w.fail(written_regions, t0, now_ms());
             throw;
         }
         w.close("FINISHED", issued_regions, t0, now_ms());

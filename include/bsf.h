@@ -253,8 +253,9 @@ typedef struct bsf_hunt_summary {
  * processGpuRegionAttempt, :1878-1971): world seeds of attempts 1, 2, ... of the sequence
  * (deterministicSeedMode), stage0stream batches of 32,768 worlds x 8 centres, the region
  * attempts of every record with reject stage 0, in attempt order, into a run folder.
- * run_dir (optional) receives the folder's path.  When a batch fails, the folder keeps the
- * batches before it and an ERROR manifest, and the call returns the batch's error. */
+ * Region calls gather the survivors of up to 8 batches (1,024 at most per call).  run_dir
+ * (optional) receives the folder's path.  When a call fails, the folder keeps the attempts
+ * written before it and an ERROR manifest, and bsf_hunt returns the call's error. */
 int bsf_hunt(const char* output_root, const char* run_label, const char* stamp, const bsf_run_settings* settings,
              bsf_hunt_summary* out, char* run_dir, int64_t run_dir_cap);
 

@@ -68,3 +68,45 @@ def test_worker_hunt_mode_writes_the_same_run(bsf, tmp_path):
         if name != "manifest.txt":
             assert a[name].replace(cli_dir, lib_dir) == b[name], name
     assert summary["survivors"] == len(a["hunter_candidates.csv"].splitlines()) - 1 > 0
+
+
+def test_hunt_over_several_batches_equals_batch_by_batch(bsf, tmp_path):
+    """Three stage0stream batches (the last cut short) whose survivors the hunt gathers into
+    region calls of up to 1024 items across batches: the run folder equals the one written
+    from region attempts made batch by batch through the public entry points."""
+    seq, keep = 0x5EED, 9
+    regions = 8 * 32768 * 2 + 8 * 100 + 3
+    s = bsf.island_params(0, 0, 1, 1, **LOW)
+    settings = bsf.run_settings("general", research=False, top_keep=keep, coarse_min=1, regions=regions,
+                                sequence_seed=seq, thresholds=s)
+    summary, run_dir = bsf.hunt(str(tmp_path / "h"), settings, label="batches", stamp="2026-05-06_07-08-09")
+    print("hunt:", summary, flush=True)
+
+    w = bsf.RunWriter(str(tmp_path / "m"), settings, label="batches", stamp="2026-05-06_07-08-09", start_ms=1)
+    done, world, survivors = 0, 1, 0
+    while done < regions:
+        n = min(8 * 32768, regions - done)
+        worlds = (n + 7) // 8
+        seeds = bsf.deterministic_seeds(seq, world, worlds)
+        recs, _ = bsf.stage0_records(seeds, "general", research=False, centers=8)
+        r = recs.reshape(worlds * 8, 16)[:n]
+        coarse = r[:, 4].astype(np.int64) | (r[:, 5].astype(np.int64) << 8)
+        score = r[:, 8:16].copy().view("<f8")[:, 0]
+        ok = (r[:, 3] == 1) & (r[:, 7] == 0) & (coarse >= 1) & (r[:, 0] >= 1) & (r[:, 1] >= 5) & (r[:, 2] >= 5)
+        idx = np.nonzero(ok)[0]
+        atts = [(done + int(o) + 1, int(seeds[o // 8]), CX[o % 8], CZ[o % 8], int(r[o, 0]), int(r[o, 1]),
+                 int(r[o, 2]), int(coarse[o]), 1, int(r[o, 6]), float(score[o])) for o in idx]
+        res = bsf.region_attempts([(a[1], a[2], a[3], a[7]) for a in atts], s, 7)
+        for a, x in zip(atts, res):
+            w.add(a, x)
+        survivors += len(atts)
+        done += n
+        world += worlds
+    w.close("FINISHED", regions, 2)
+    assert summary["regions"] == regions and summary["worlds"] == world - 1
+    assert summary["survivors"] == survivors > 1024  # at least one call split at 1024 items
+    a, b = ro.read_run(run_dir), ro.read_run(w.dir)
+    assert set(a) == set(b)
+    for name in a:
+        if name != "manifest.txt":
+            assert a[name] == b[name].replace(w.dir, run_dir), name
