@@ -344,6 +344,15 @@ def run_regions(bsf, args, worlds, batches):
            "regions": len(items), "regions_per_s": round(len(items) / gpu_s, 1), "ms": round(gpu_s * 1e3, 2),
            "matches": sum(r["hit"] is not None for r in res), "hot_scans": sum(r["hot_scan_ran"] for r in res),
            "border_verified": sum(r["radius_used"] > 7 for r in res)}
+    if items:
+        # the same windows 16 times over in one call: the rate of a call as large as the
+        # hunt's gathered region calls (a call over a few windows is latency-bound)
+        many = items * 16
+        t = time.perf_counter()
+        res16 = bsf.region_attempts(many, settings, 7)
+        many_s = time.perf_counter() - t
+        out["batch_x16"] = {"regions": len(many), "regions_per_s": round(len(many) / many_s, 1),
+                            "ms": round(many_s * 1e3, 2), "results_equal": res16 == res * 16}
     if not args.no_cpu and items:
         from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle
