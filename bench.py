@@ -411,9 +411,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one MI355X: every box in this setup has a single GPU, so --gpus %d is "
-              "not supported (seeds are independent; see the sharding note in DESIGN.md)" % args.gpus,
-              file=sys.stderr)
+        print("bench.py measures one MI355X: --gpus %d is not supported (seeds are independent, so each "
+              "GPU would take its own seed range in a process of its own)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import ctypes

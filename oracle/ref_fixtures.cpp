@@ -1,7 +1,7 @@
 // ref_fixtures.cpp -- writes golden fixtures from the reference's own C++ host code.
 //
 // Built by `make -C oracle ref` against the reference sources where they lie
-// (/root/reference/native/src: p20_exact_math.hpp, coarse_exact_core.hpp,
+// (native/src of the reference: p20_exact_math.hpp, coarse_exact_core.hpp,
 // p19_monster_gate.hpp -- all host-callable inline code); the binary goes to oracle/_ref/
 // and is run only in the development container.  Its outputs (small JSON files under
 // tests/golden/) are data: inputs and the reference's outputs for them.  Nothing here
