@@ -110,3 +110,27 @@ def test_hunt_over_several_batches_equals_batch_by_batch(bsf, tmp_path):
     for name in a:
         if name != "manifest.txt":
             assert a[name] == b[name].replace(w.dir, run_dir), name
+
+
+def test_hunt_research_and_lean_records_give_the_same_run(bsf, tmp_path):
+    """Research records (34 bytes) carry the lean record's 16 bytes first: a hunt over either
+    makes the same decisions, so only the manifest's response-mode lines differ."""
+    seq, regions = 0xC0FFEE, 8 * 3000 - 3
+    dirs, sums = [], []
+    for research in (False, True):
+        settings = bsf.run_settings("general", research=research, top_keep=6, coarse_min=1, regions=regions,
+                                    sequence_seed=seq, thresholds=bsf.island_params(0, 0, 1, 1, **LOW))
+        summary, d = bsf.hunt(str(tmp_path / str(research)), settings, label="mode", stamp="2026-01-02_03-04-05")
+        dirs.append(d)
+        sums.append({k: v for k, v in summary.items() if k != "seconds"})
+    a, b = ro.read_run(dirs[0]), ro.read_run(dirs[1])
+    assert sums[0] == sums[1] and sums[0]["survivors"] > 0
+    for name in a:
+        if name != "manifest.txt":
+            assert a[name].replace(dirs[0], dirs[1]) == b[name], name
+    ma = [l for l in a["manifest.txt"].splitlines() if "esponse" not in l and "Research" not in l
+          and not l.startswith(("runFolder=", "startEpochMs=", "endEpochMs=", "elapsedSeconds="))]
+    mb = [l for l in b["manifest.txt"].splitlines() if "esponse" not in l and "Research" not in l
+          and not l.startswith(("runFolder=", "startEpochMs=", "endEpochMs=", "elapsedSeconds="))]
+    assert ma == mb
+    assert "nativeResponseMode=lean\n" in a["manifest.txt"] and "nativeResponseMode=research\n" in b["manifest.txt"]
