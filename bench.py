@@ -481,6 +481,7 @@ def main():
     avg_launch_s = ks[dom][0] / 1e3 / launches
     units_per_launch = d["units"] / launches
     achieved_tflops = d["ops"] * units_per_launch / avg_launch_s / 1e12
+    dev_cu, dev_clk, _ = bsf.device_info()
 
     result = {
         "metric": METRIC,
@@ -520,6 +521,9 @@ def main():
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
             "traffic": int(round(HBM_BYTES_PER_UNIT[dom] * units_per_launch)),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x calibration + WRITE_SIZE)",
+            # the published peak against the device's own figures: CUs x 128 FP64 flops per clock
+            "device": {"compute_units": dev_cu, "peak_clock_mhz": round(dev_clk / 1e3, 1),
+                       "fp64_peak_tflops": round(dev_cu * 128 * dev_clk * 1e3 / 1e12, 2)},
         },
     }
 

@@ -86,6 +86,14 @@ def device_banner():
     return buf.value.decode()
 
 
+def device_info():
+    """(compute units, peak shader clock in kHz, global memory bytes) of the library's device."""
+    init()
+    cu, clk, mem = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int64(0)
+    _check(lib().bsf_device_info(ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(mem)))
+    return cu.value, clk.value, mem.value
+
+
 def _seeds(seeds):
     return np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
 

@@ -210,6 +210,7 @@ int bsf_init(int device) {
         check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         g_ctx.device = device;
         g_ctx.cu_count = prop.multiProcessorCount;
+        g_ctx.clock_khz = prop.clockRate;
         g_ctx.name = prop.name;
         g_ctx.arch = prop.gcnArchName;
         g_ctx.global_mem = prop.totalGlobalMem;
@@ -278,6 +279,15 @@ int bsf_device_banner(char* buf, size_t cap) {
         std::snprintf(buf, cap, "GPU: %s\nArchitecture: %s\nCompute units: %d\nGlobal memory: %g GiB\n",
                       g_ctx.name.c_str(), g_ctx.arch.c_str(), g_ctx.cu_count,
                       static_cast<double>(g_ctx.global_mem) / (1024.0 * 1024.0 * 1024.0));
+    });
+}
+
+int bsf_device_info(int32_t* compute_units, int32_t* peak_clock_khz, int64_t* global_mem_bytes) {
+    return guarded_call([&] {
+        require_init();
+        if (compute_units) *compute_units = g_ctx.cu_count;
+        if (peak_clock_khz) *peak_clock_khz = g_ctx.clock_khz;
+        if (global_mem_bytes) *global_mem_bytes = static_cast<int64_t>(g_ctx.global_mem);
     });
 }
 

@@ -44,6 +44,7 @@ struct Context {
     bool initialised = false;
     int device = 0;
     int cu_count = 0;
+    int clock_khz = 0;  // peak shader clock (hipDeviceProp_t::clockRate)
     std::string name, arch;
     size_t global_mem = 0;
     hipStream_t stream = nullptr;

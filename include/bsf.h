@@ -34,6 +34,10 @@ const char* bsf_last_error(void);
 /* Four-line device banner: "GPU:", "Architecture:", "Compute units:", "Global memory:"
  * (printDevice, gpu_p20_benchmark.cpp:979-992). */
 int         bsf_device_banner(char* buf, size_t cap);
+/* The banner's figures as numbers, with the peak shader clock (kHz), from which a vector FP64
+ * peak is compute_units x 128 flops per clock (4 SIMDs x 16 lanes x FMA) x clock; any
+ * pointer may be NULL.  No counterpart in the reference beyond printDevice. */
+int         bsf_device_info(int32_t* compute_units, int32_t* peak_clock_khz, int64_t* global_mem_bytes);
 
 /* ---- launch timing (HIP events on the launching stream; for benchmarks) ---------------
  * Kernel names: "density", "scout", "island_cache", "island_stage1" (one world per wave),

@@ -76,3 +76,11 @@ def test_redraw_seeds_match_oracle(bsf, oracle):
     p = bsf.island_params(-2, -2, 4, 4, **LOW)
     ref = [oracle.island_scan(s, oracle.island_params(-2, -2, 4, 4, **LOW)) for s in REDRAW_SEEDS]
     assert bsf.island_seeds(REDRAW_SEEDS, p) == sorted((r for r in ref if r is not None), key=lambda h: h[0])
+
+
+def test_device_info_matches_the_banner(bsf):
+    cu, clk, mem = bsf.device_info()
+    banner = dict(line.split(": ", 1) for line in bsf.device_banner().splitlines())
+    assert int(banner["Compute units"]) == cu > 0
+    assert abs(float(banner["Global memory"].split()[0]) - mem / 2**30) < 0.01 * mem / 2**30
+    assert 500_000 < clk < 5_000_000  # kHz
