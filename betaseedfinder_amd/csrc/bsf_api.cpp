@@ -145,17 +145,58 @@ void HostColumns::upload(DeviceBuffer& nbuf, DeviceBuffer& dbuf, hipStream_t s) 
     dev.dens = dd;
 }
 
-// generateTerrainNoise's 5x17x5 chunk lattice (BetaTerrain173.java:667-834): noise column
-// (cx*4 + i, cz*4 + l) with the per-chunk climate sample at block (3i+1, 3l+1)
-// (temperatures[(j2*3 + 1)*16 + l2*3 + 1], :741-749).
+// generateTerrainNoise's 5x17x5 chunk lattice (chunk_lattice_column).
 HostColumns chunk_columns(int cx, int cz) {
     HostColumns h;
-    for (int i = 0; i < 5; ++i)
-        for (int l = 0; l < 5; ++l) {
-            h.dens.push_back(make_int4(static_cast<int>(h.noise.size()), cx * 16 + 3 * i + 1, cz * 16 + 3 * l + 1, 0));
-            h.noise.push_back(make_int2(cx * 4 + i, cz * 4 + l));
-        }
+    h.noise.resize(kChunkColumns);
+    h.dens.resize(kChunkColumns);
+    for (int k = 0; k < kChunkColumns; ++k) chunk_lattice_column(k, cx, cz, h.noise[k], h.dens[k]);
     return h;
+}
+
+// ---------------------------------------------------------------------------
+// fixed device scratch of bsf_density_grid_device_noalloc
+// ---------------------------------------------------------------------------
+// The density kernel keeps one accumulator slot per resident workgroup (lane-interleaved
+// octave sums of the 25 columns: scratch_vals(17, 25, 25) doubles per lane).  The allocation-
+// free entry point runs at most kNoAllocSlots workgroups (a quarter of the 2,048 that fill the
+// chip at two waves per SIMD) out of this array, after the chunk's column table; the workgroups
+// walk the 64-seed groups of any count, so no count needs more than this fixed 350 MiB.
+constexpr int kNoAllocSlots = 512;
+constexpr size_t kNoAllocColumnBytes = 1024;
+constexpr size_t kNoAllocSlotBytes = size_t(scratch_vals(17, kChunkColumns, kChunkColumns)) * 64 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocColumnBytes + size_t(kNoAllocSlots) * kNoAllocSlotBytes;
+constexpr size_t kNoAllocDensOffset = 256;  // int2 noise[25] at 0, int4 dens[25] at 256
+static_assert(kChunkColumns * sizeof(int2) <= kNoAllocDensOffset, "column table");
+static_assert(kNoAllocDensOffset + kChunkColumns * sizeof(int4) <= kNoAllocColumnBytes, "column table");
+static_assert(kNoAllocColumnBytes % 256 == 0, "slot alignment");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device (hipGetSymbolAddress at bsf_init with that device
+// current); host code uses these, never the array's name.
+constexpr int kMaxDevices = 64;
+void* g_noalloc_scratch[kMaxDevices] = {};
+
+void run_density_noalloc(const int64_t* d_seeds, int64_t count, int cx, int cz, double* d_out, hipStream_t s) {
+    unsigned char* base = static_cast<unsigned char*>(g_noalloc_scratch[g_ctx.device]);
+    if (!base) throw std::runtime_error("scratch address not set up (bsf_init)");
+    int2* noise = reinterpret_cast<int2*>(base);
+    int4* dens = reinterpret_cast<int4*>(base + kNoAllocDensOffset);
+    hipLaunchKernelGGL(chunk_columns_kernel, dim3(1), dim3(64), 0, s, cx, cz, noise, dens);
+    check(hipGetLastError(), "chunk_columns_kernel launch");
+    auto kern = lattice_kernel<0, 17, MODE_DENSITY>;
+    const int64_t groups = (count + 63) / 64;
+    const int grid = static_cast<int>(
+            std::min<int64_t>(std::min<int64_t>(groups, g_ctx.density_resident), kNoAllocSlots));
+    LatticeArgs a{};
+    a.seeds = d_seeds;
+    a.count = count;
+    a.cols = ColumnSet{kChunkColumns, kChunkColumns, noise, dens};
+    a.scratch = reinterpret_cast<double*>(base + kNoAllocColumnBytes);
+    a.out = d_out;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, s, a);
+    check(hipGetLastError(), "lattice_kernel launch");
 }
 
 // ---------------------------------------------------------------------------
@@ -204,6 +245,7 @@ int bsf_init(int device) {
         int n = 0;
         check(hipGetDeviceCount(&n), "hipGetDeviceCount");
         if (device < 0 || device >= n) throw ArgError("no such HIP device");
+        if (device >= kMaxDevices) throw ArgError("device index beyond the library's 64-device limit");
         if (g_ctx.initialised && g_ctx.device == device) return;
         check(hipSetDevice(device), "hipSetDevice");
         hipDeviceProp_t prop{};
@@ -217,6 +259,12 @@ int bsf_init(int device) {
         check(upload_tables(), "upload tables");
         check(upload_stage0_tables(), "upload stage0 tables");
         check(upload_island_tables(), "upload island tables");
+        if (!g_noalloc_scratch[device]) {
+            void* p = nullptr;
+            check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+            g_noalloc_scratch[device] = p;
+        }
+        g_ctx.density_resident = resident_blocks(lattice_kernel<0, 17, MODE_DENSITY>, 64, 0);
         // the island floods keep two window bitsets in LDS (up to 123 KiB)
         const int flood_lds = static_cast<int>(flood_words(128, 1) * 8);
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(pass_flood_kernel<1, false>),
@@ -317,6 +365,17 @@ int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t cx, i
         if (count < 0 || (count > 0 && (!d_seeds || !d_out))) throw ArgError("bad density arguments");
         if (!chunk_span_ok(cx, 1) || !chunk_span_ok(cz, 1)) throw ArgError("chunk outside coordinates -2^19 .. 2^19");
         run_density(d_seeds, count, cx, cz, d_out, stream ? static_cast<hipStream_t>(stream) : g_ctx.stream);
+    });
+}
+
+int bsf_density_grid_device_noalloc(const int64_t* d_seeds, int64_t count, int32_t cx, int32_t cz, double* d_out,
+                                    void* stream) {
+    return guarded_call([&] {
+        require_init();
+        if (count < 0 || (count > 0 && (!d_seeds || !d_out))) throw ArgError("bad density arguments");
+        if (!chunk_span_ok(cx, 1) || !chunk_span_ok(cz, 1)) throw ArgError("chunk outside coordinates -2^19 .. 2^19");
+        if (count == 0) return;
+        run_density_noalloc(d_seeds, count, cx, cz, d_out, stream ? static_cast<hipStream_t>(stream) : g_ctx.stream);
     });
 }
 

@@ -45,6 +45,7 @@ struct Context {
     int device = 0;
     int cu_count = 0;
     int clock_khz = 0;  // peak shader clock (hipDeviceProp_t::clockRate)
+    int density_resident = 0;  // resident 64-thread workgroups of the density kernel
     std::string name, arch;
     size_t global_mem = 0;
     hipStream_t stream = nullptr;

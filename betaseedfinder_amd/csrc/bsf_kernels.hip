@@ -201,6 +201,11 @@ __global__ __launch_bounds__(64) void lattice_kernel(LatticeArgs a) {
 
 template __global__ void lattice_kernel<0, 17, MODE_DENSITY>(LatticeArgs);
 
+__global__ __launch_bounds__(64) void chunk_columns_kernel(int cx, int cz, int2* noise, int4* dens) {
+    const int k = threadIdx.x;
+    if (k < kChunkColumns) chunk_lattice_column(k, cx, cz, noise[k], dens[k]);
+}
+
 __global__ __launch_bounds__(256) void splitmix_seeds_kernel(uint64_t sequence_seed, int64_t first_attempt,
                                                              int64_t count, int64_t* out) {
     const int64_t step = static_cast<int64_t>(gridDim.x) * blockDim.x;

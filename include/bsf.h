@@ -65,6 +65,17 @@ int bsf_density_grid(const int64_t* seeds, int64_t count, int32_t chunk_x, int32
 /* Same with device-resident seeds/output on a caller stream (hipStream_t as void*). */
 int bsf_density_grid_device(const int64_t* d_seeds, int64_t count, int32_t chunk_x, int32_t chunk_z,
                             double* d_out, void* stream);
+/* Allocation-free form of bsf_density_grid_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call launches kernels only; their scratch (the chunk's
+ * column table and 512 workgroup accumulator slots, a fixed 350 MiB) is an array in the
+ * library's code object on each device, located at bsf_init.  Any count is served (the 512
+ * workgroups walk the seeds 64 at a time, a quarter of the chip's resident waves, so it is
+ * slower than bsf_density_grid_device).  Every call on a device shares that array: calls that
+ * overlap on one device (other streams, other threads) race and must be serialised by the
+ * caller. */
+int bsf_density_grid_device_noalloc(const int64_t* d_seeds, int64_t count, int32_t chunk_x, int32_t chunk_z,
+                                    double* d_out, void* stream);
 
 /* ---- Stage-0 scout over a contiguous seed range ------------------------------------
  * Per seed: setSeed + the first terrain Perlin build (NoiseGeneratorPerlin173.java:63-78)
