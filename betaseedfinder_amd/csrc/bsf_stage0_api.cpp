@@ -233,6 +233,20 @@ int bsf_p20_counts(const int64_t* seeds, int32_t count, int32_t* out) {
     });
 }
 
+int bsf_stage0_profile(int32_t profile, int32_t* p20_min, int32_t* upper_min, int32_t* high_min, double* p19_min,
+                       int32_t* coarse_min, int32_t* mega_topology) {
+    return guarded_call([&] {
+        if (profile < 0 || profile > 3) throw ArgError("hunt profile must be general, mega, record60, or record80");
+        const Profile& pr = kProfiles[profile];
+        if (p20_min) *p20_min = pr.p20_min;
+        if (upper_min) *upper_min = pr.upper_min;
+        if (high_min) *high_min = pr.high_min;
+        if (p19_min) *p19_min = pr.p19_min;
+        if (coarse_min) *coarse_min = pr.coarse_min;
+        if (mega_topology) *mega_topology = pr.mega_topology ? 1 : 0;
+    });
+}
+
 int bsf_stage0_records(const int64_t* seeds, int32_t count, int32_t profile, int32_t research, int32_t centers,
                        uint8_t* out, bsf_stage0_counters* counters) {
     return guarded_call([&] {

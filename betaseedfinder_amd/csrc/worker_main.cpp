@@ -116,19 +116,17 @@ int stream_mode(int capacity) {
     return 0;
 }
 
+// The profiles' argv names and display names (HuntProfileConfig, gpu_p20_benchmark.cpp:7270-7290);
+// their thresholds come from the library (bsf_stage0_profile).
 struct ProfileInfo {
     const char* arg;
     const char* display;
-    int p20, upper, high;
-    double p19;
-    int coarse;
-    bool mega;
 };
 const ProfileInfo kProfiles[4] = {
-    {"general", "GENERAL", 1, 5, 5, 4.02, 85, false},
-    {"mega", "MEGA 30k+", 1, 8, 6, 4.02, 85, true},
-    {"record60", "RECORD 60k+ (empirical)", 3, 19, 12, 6.70, 95, false},
-    {"record80", "WORLD RECORD COARSE 700 (compact-thick safe)", 3, 19, 20, 6.70, 700, false},
+    {"general", "GENERAL"},
+    {"mega", "MEGA 30k+"},
+    {"record60", "RECORD 60k+ (empirical)"},
+    {"record80", "WORLD RECORD COARSE 700 (compact-thick safe)"},
 };
 
 // streamStage0Filter (gpu_p20_benchmark.cpp:7310-7612).
@@ -141,14 +139,19 @@ int stage0_mode(int capacity, const std::string& profile, const std::string& tel
     if (capacity < 1) throw WorkerError("Stage0 stream capacity must be >= 1");
     if (centers != 1 && centers != 8) throw WorkerError("Stage0 stream center count must be 1 or 8");
     const bool research = telemetry == "research";
-    const ProfileInfo& pr = kProfiles[pid];
+    int p20 = 0, upper = 0, high = 0, coarse = 0, mega = 0;
+    double p19 = 0.0;
+    check(bsf_stage0_profile(pid, &p20, &upper, &high, &p19, &coarse, &mega), "profile");
     print_device();
     std::fprintf(stderr, "Stage0+P19+coarse stream capacity: %d worlds / %lld regions\n", capacity,
                  static_cast<long long>(capacity) * centers);
-    std::fprintf(stderr, "Multi-center: %d exact radius-7 regions/world | 15-chunk spacing\n", centers);
-    std::fprintf(stderr, "Profile: %s | P20>=%d | Upper>=%d | High>=%d | P19>=%g/extreme | Coarse>=%d%s\n",
-                 pr.display, pr.p20, pr.upper, pr.high, pr.p19, pr.coarse,
-                 pr.mega ? " | Mega topology reject ON" : " | topology reject OFF");
+    std::fprintf(stderr,
+                 "P38 production multi-center: %d exact radius-7 regions/world | 15-chunk spacing | region budget "
+                 "semantics\n",
+                 centers);
+    std::fprintf(stderr, "P38 profile: %s | P20>=%d | Upper>=%d | High>=%d | P19>=%g/extreme | Coarse>=%d%s\n",
+                 kProfiles[pid].display, p20, upper, high, p19, coarse,
+                 mega ? " | Mega topology reject ON" : " | topology reject OFF");
     if (pid >= 2) std::fprintf(stderr, "WARNING: record profile is empirically validated, not mathematically lossless.\n");
     std::fprintf(stderr, "Stage0+P19+coarse pipeline: exact P20 -> full upper survivors -> lower reentry -> exact "
                          "cached Stage0.75 gate -> optional Mega topology reject -> exact radius-7 coarse score\n");

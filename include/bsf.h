@@ -338,6 +338,13 @@ int bsf_p20_counts(const int64_t* seeds, int32_t count, int32_t* out_counts);
 #define BSF_PROFILE_RECORD60 2
 #define BSF_PROFILE_RECORD80 3
 
+/* The gate thresholds of a hunt profile (HuntProfileConfig, huntProfileConfig,
+ * gpu_p20_benchmark.cpp:7251-7290): P20, full-upper and high re-entry minimum counts, the P19
+ * minimum score, the coarse minimum cells, and whether the mega topology reject runs.  Any
+ * pointer may be NULL.  Host only: needs neither bsf_init nor a GPU. */
+int bsf_stage0_profile(int32_t profile, int32_t* p20_min, int32_t* upper_min, int32_t* high_min, double* p19_min,
+                       int32_t* coarse_min, int32_t* mega_topology);
+
 typedef struct bsf_stage0_counters {
     uint64_t worlds, regions, p20_passed, stage1_passed, stage05_passed;
     uint64_t stage075_evaluated, stage075_passed, stage075_rejected;

@@ -855,12 +855,13 @@ void stage0Lattice(const Terrain& t, int offX, int offZ, double* out /* 256*17 *
 
 bool isP20Axis(int i) { return i == 0 || i == 2 || i == 5 || i == 7 || i == 8 || i == 10 || i == 13 || i == 15; }
 
-// highestSparseReentryYIndex, BetaTerrain173.java:2486-2504 (positive -> gap -> positive)
-int highestReentry(const double* col) {
+// highestSparseReentryYIndex, BetaTerrain173.java:2486-2504 (positive -> gap -> positive), on
+// a column's 17-level positive mask (bit y = density > 0 at level y).
+int highestReentryMask(uint32_t mask) {
     bool sawPositive = false, sawGap = false;
     int highest = -1;
     for (int y = 0; y < 17; ++y) {
-        bool pos = col[y] > 0.0;
+        bool pos = (mask >> y) & 1u;
         if (pos) {
             if (sawPositive && sawGap) highest = y;
             sawPositive = true;
@@ -869,6 +870,13 @@ int highestReentry(const double* col) {
         }
     }
     return highest;
+}
+
+int highestReentry(const double* col) {
+    uint32_t mask = 0;
+    for (int y = 0; y < 17; ++y)
+        if (col[y] > 0.0) mask |= 1u << y;
+    return highestReentryMask(mask);
 }
 
 // ---------------------------------------------------------------------------
@@ -1403,6 +1411,19 @@ static const Stage0Profile kStage0Profiles[4] = {
     {1, 5, 5, 4.02, 85, 0}, {1, 8, 6, 4.02, 85, 1}, {3, 19, 12, 6.70, 95, 0}, {3, 19, 20, 6.70, 700, 0}};
 static const int kStage0CenterX[8] = {0, 15, 0, 15, -15, -15, 0, 15};
 static const int kStage0CenterZ[8] = {0, 0, 15, 15, 0, 15, -15, -15};
+
+int bsfo_highest_reentry_mask(uint32_t mask17) { return highestReentryMask(mask17 & 0x1FFFFu); }
+
+void bsfo_stage0_profile(int32_t profile, int32_t* p20_min, int32_t* upper_min, int32_t* high_min, double* p19_min,
+                         int32_t* coarse_min, int32_t* mega) {
+    const Stage0Profile& pr = kStage0Profiles[profile & 3];
+    *p20_min = pr.p20_min;
+    *upper_min = pr.upper_min;
+    *high_min = pr.high_min;
+    *p19_min = pr.p19_min;
+    *coarse_min = pr.coarse_min;
+    *mega = pr.mega;
+}
 
 void bsfo_stage0_records(const int64_t* seeds, int64_t count, int32_t profile, int32_t research, int32_t centers,
                          int threads, uint8_t* out, uint64_t* counters) {

@@ -158,6 +158,11 @@ typedef struct bsfo_p19_features {
     int32_t y96_largest, y96_width, y96_depth, y96_touches;
 } bsfo_p19_features;
 
+/* highest re-entry Y index (-1 = none) of a 17-level positive mask, bit y = density > 0 at y */
+int      bsfo_highest_reentry_mask(uint32_t mask17);
+/* the hunt profile table (HuntProfileConfig, gpu_p20_benchmark.cpp:7270-7290), profile 0..3 */
+void     bsfo_stage0_profile(int32_t profile, int32_t* p20_min, int32_t* upper_min, int32_t* high_min,
+                             double* p19_min, int32_t* coarse_min, int32_t* mega);
 void     bsfo_p19_features_from_highest(const uint8_t highest[256], bsfo_p19_features* f);
 double   bsfo_p19_score(int32_t scout, const bsfo_p19_features* f);
 int      bsfo_p19_extreme(const bsfo_p19_features* f);

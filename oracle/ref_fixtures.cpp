@@ -177,8 +177,48 @@ std::string chunk00_lattice(int64_t seed, const coarsecore::HostScratch& s) {
            ", \"n4\": " + hexlist(n4) + ", \"n5\": " + hexlist(n5) + "}";
 }
 
+// The Stage0 lattice at coarse offset 0 is every fourth column of the coarse grid (coarse
+// -28 + 4i, stage0_exact_gpu.hpp:48-63, and columnCoordinates, coarse_exact_core.hpp:74-81, with
+// the same coarse*4+2 climate), so the grid's signs give each Stage0 column's 17-level positive
+// mask.  The dense per-chunk climate (getBiomeNoise over a chunk's 16x16 blocks) meets the grid's
+// climate samples at the blocks whose world coordinate is 2 mod 4: chunk-local blocks
+// {2, 6, 10, 14}^2 of chunk (cx, cz) are the grid's columns cx*4 + {0..3}.
+const int kClimateChunks[][2] = {{0, 0}, {-7, -7}, {7, 7}, {-7, 7}, {3, -2}, {-1, 4}};
+
+std::string stage0_climate_row(int64_t seed, const coarsecore::HostScratch& s) {
+    std::string out;
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "{\"seed\": %" PRId64 ", \"masks\": [", seed);
+    out += buf;
+    for (int ix = 0; ix < 16; ++ix)
+        for (int iz = 0; iz < 16; ++iz) {
+            const int col = (ix * 4) * coarsecore::SIZE + iz * 4;
+            unsigned mask = 0;
+            for (int y = 0; y < 17; ++y)
+                if (s.signs[col * 17 + y]) mask |= 1u << y;
+            std::snprintf(buf, sizeof(buf), "%s%u", ix || iz ? "," : "", mask);
+            out += buf;
+        }
+    out += "], \"climate\": [";
+    bool first = true;
+    for (const auto& ch : kClimateChunks) {
+        std::vector<double> temp, rain;
+        for (int i = 0; i < 4; ++i)
+            for (int l = 0; l < 4; ++l) {
+                const int col = (ch[0] * 4 + i - coarsecore::FROM_COARSE) * coarsecore::SIZE +
+                                (ch[1] * 4 + l - coarsecore::FROM_COARSE);
+                temp.push_back(s.temp[col]);
+                rain.push_back(s.rain[col]);
+            }
+        std::snprintf(buf, sizeof(buf), "%s{\"chunk\": [%d, %d], \"temp\": ", first ? "" : ", ", ch[0], ch[1]);
+        out += buf + hexlist(temp) + ", \"rain\": " + hexlist(rain) + "}";
+        first = false;
+    }
+    return out + "]}";
+}
+
 void coarse_fixture(const std::string& dir, int n_seeds) {
-    std::vector<std::string> lattice_rows;
+    std::vector<std::string> lattice_rows, stage0_rows;
     FILE* f = open_out(dir, "ref_coarse.json");
     std::fprintf(f, "{\"cases\": [\n");
     coarsecore::HostScratch s;
@@ -208,6 +248,7 @@ void coarse_fixture(const std::string& dir, int n_seeds) {
                      dig(s.noise4), dig(s.noise5), dig(s.temp), dig(s.rain));
         std::fflush(f);
         if (i < 4) lattice_rows.push_back(chunk00_lattice(seed, s));
+        if (i < 10) stage0_rows.push_back(stage0_climate_row(seed, s));
     }
     std::fprintf(f, "]}\n");
     std::fclose(f);
@@ -217,6 +258,11 @@ void coarse_fixture(const std::string& dir, int n_seeds) {
     for (size_t i = 0; i < lattice_rows.size(); ++i) std::fprintf(g, "%s%s\n", i ? "," : "", lattice_rows[i].c_str());
     std::fprintf(g, "]}\n");
     std::fclose(g);
+    FILE* h = open_out(dir, "ref_stage0_climate.json");
+    std::fprintf(h, "{\"cases\": [\n");
+    for (size_t i = 0; i < stage0_rows.size(); ++i) std::fprintf(h, "%s%s\n", i ? "," : "", stage0_rows[i].c_str());
+    std::fprintf(h, "]}\n");
+    std::fclose(h);
 }
 
 // P19 monster gate (p19_monster_gate.hpp:48-79) on random and structured feature vectors.
