@@ -316,8 +316,9 @@ class RunWriter:
         return bool(u.value)
 
     def close(self, status="FINISHED", checked=0, end_ms=0):
-        _check(lib().bsf_run_close(self._h, status.encode(), ctypes.c_int64(checked), ctypes.c_int64(end_ms)))
-        self._h = None
+        # bsf_run_close frees the handle even when its writes fail
+        h, self._h = self._h, None
+        _check(lib().bsf_run_close(h, status.encode(), ctypes.c_int64(checked), ctypes.c_int64(end_ms)))
 
 
 def hunt(root, settings, label="", stamp=None):

@@ -567,6 +567,11 @@ void RunWriter::write_manifest(const std::string& status, int64_t checked, int64
     kv("gpuProtocol", "ST0R3708");
     kv("gpuTerrainPerlinCacheStates", "66");
     kv("gpuClimatePerlinCacheStates", "10");
+    kv("gpuCachedCoarseInitialization", "true");
+    kv("gpuCompactLowerThreads", "32");
+    kv("gpuCompactUpperThreads", "192");
+    kv("gpuSharedUpperYAxis", "true");
+    kv("gpuDirectWriteNoise23", "true");
     m += "\n";
     kv("chunkRadius", std::to_string(s_.chunk_radius));
     kv("seedsToCheck", std::to_string(s_.regions));

@@ -103,6 +103,16 @@ def test_run_folder_matches_restatement(tmp_path, n, keep):
     for name in want:
         assert got[name] == want[name], name
     assert "status=FINISHED\n" in got["manifest.txt"] and "elapsedSeconds=60.000\n" in got["manifest.txt"]
+    # the GPU coverage block as writeManifest writes it with the stage0stream scout active
+    # (IslandSearchEngine.java:4335-4344)
+    gpu_block = ("gpuCoverageCentersPerWorld=8\ngpuCoverageCenterSpacingChunks=15\ngpuProtocol=ST0R3708\n"
+                 "gpuTerrainPerlinCacheStates=66\ngpuClimatePerlinCacheStates=10\n"
+                 "gpuCachedCoarseInitialization=true\ngpuCompactLowerThreads=32\ngpuCompactUpperThreads=192\n"
+                 "gpuSharedUpperYAxis=true\ngpuDirectWriteNoise23=true\n\nchunkRadius=7\n")
+    assert gpu_block in got["manifest.txt"]
+    # a closed writer holds no handle: closing or adding again is an error, not a use of freed memory
+    with pytest.raises(bsf.BsfError):
+        w.close("FINISHED", 8 * n, 61000)
     # the latest-run copies and pointer in the output root
     root = {n: open(os.path.join(tmp_path, n)).read() for n in os.listdir(tmp_path) if n != "runs"}
     assert root["latest_run.txt"] == w.dir + "\n"
