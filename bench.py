@@ -7,14 +7,18 @@ Workload (the full-pipeline configuration): a 2^36-seed contiguous range startin
 splitmix64(0xB5F00004) & (2^48 - 1); a 4x4 chunk window {-2..1}^2 around the origin;
 minimum island volume V = 200 + floor(1801 u), u = splitmix64(0xB5F00005) as a 53-bit
 double; the other island gates at the reference's SearchSettings defaults.  One step is one
-batch of 2^B consecutive seeds of that range (inputs are a seed range: nothing is copied to
-the device).  The timed region covers K steps; seeds/s = K * 2^B / elapsed.
+batch of consecutive seeds of that range (96 passes of 2^18 seeds by default; inputs are a seed
+range: nothing is copied to the device).  The timed region covers K steps on the production
+path (every pass one captured hipGraph, no timers); seeds/s = K * seeds per step / elapsed.
 
 The JSON line also carries:
-  roofline      FP64 VALU roofline of the dominant FP64 kernel (island Stage 2 here), timed with HIP
-                events on its own stream over the timed region;
-  cpu_baseline  the CPU oracle on a bounded sample of the same range, all host threads
-                (rank 0, N = 1 only), with the sample's hit list compared to the GPU's.
+  roofline      FP64 VALU roofline of the dominant FP64 kernel (island Stage 2 here): its
+                launches timed with HIP events on the engine's stream over the next steps of the
+                same range (timers need the kernels launched one by one, so not inside the graphs
+                of the timed region); HBM traffic per unit from the rocprofv3 counter runs named
+                in traffic_source;
+  cpu_baseline  the CPU oracle on a bounded sample of the same range, on the host's CPUs
+                (at most 16), with the sample's hit list compared to the GPU's.
 """
 import argparse
 import json
@@ -33,6 +37,14 @@ FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no
 # a 256 MiB device copy under-reads) + WRITE_SIZE, from the rocprofv3 --pmc runs summarised in
 # profiles/island_pipeline_4x4_v1134.md.
 HBM_BYTES_PER_UNIT = {"island_stage1": 52145, "island_stage2": 196216}
+
+
+T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line per finished leg on stderr (the JSON result line alone goes to stdout)."""
+    print("bench %7.1f s: %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
 
 
 def splitmix64(x):
@@ -135,13 +147,16 @@ def job_params(bsf, w, h, profile, u_blocks, u_thick):
                              min_avg_thickness=pr["thick"][0] + (pr["thick"][1] - pr["thick"][0]) * u_thick)
 
 
-def batched_jobs(bsf, n=64, lo=10, span=10):
-    """Heterogeneous jobs k = 0..n-1 from splitmix64(0xB5F00006 + k): size 2^(lo + span*u),
-    window w x h with w, h uniform in 1..8, one of the four profiles."""
+def batched_jobs(bsf, n=64, lo=20, span=14, cap_log2=None):
+    """The batched workload: jobs k = 0..n-1 from splitmix64(0xB5F00006 + k), size 2^(lo + span*u)
+    (2^20 .. 2^34) seeds, cut at 2^cap_log2 when given, window w x h with w, h uniform in 1..8,
+    one of the four profiles."""
     jobs = []
     for k in range(n):
         r = [unit53(splitmix64(0xB5F00006 + k) + 977 * i) for i in range(7)]
         size = int(round(2 ** (lo + span * r[0])))
+        if cap_log2 is not None:
+            size = min(size, 1 << cap_log2)
         w, h = 1 + int(8 * r[1]), 1 + int(8 * r[2])
         first = splitmix64(0xB5F00006 + k) & MASK48
         jobs.append((first, size, job_params(bsf, w, h, int(4 * r[3]), r[4], r[5])))
@@ -154,7 +169,7 @@ def pct(v, q):
 
 
 def run_batched(bsf, args):
-    jobs = batched_jobs(bsf, lo=args.batch_job_log2_lo, span=args.batch_job_log2_span)
+    jobs = batched_jobs(bsf, cap_log2=args.batch_job_cap_log2)
     shapes = sorted({(p.chunks_x, p.chunks_z) for _, _, p in jobs})
     t = time.perf_counter()
     tuned = {}
@@ -163,18 +178,23 @@ def run_batched(bsf, args):
             tt = bsf.autotune(bsf.island_params(-(w // 2), -(h // 2), w, h), 0x5EED, args.tune_seeds)
             tuned["%dx%d" % (w, h)] = [tt.worlds_per_wave, tt.gate]
     tune_s = time.perf_counter() - t
+    progress("autotuner done: %d shapes" % len(shapes))
     bsf.island_range(jobs[0][0], 4096, jobs[0][2])  # warm the pass graphs
     t = time.perf_counter()
     plain = [bsf.island_range(first, count, p, cap=1 << 20) for first, count, p in jobs]
     plain_s = time.perf_counter() - t
+    progress("plain runs of the batched jobs done: %.1f s" % plain_s)
     t = time.perf_counter()
     hits, per_job = bsf.island_batch(jobs)
     batch_s = time.perf_counter() - t
+    progress("batched run done: %.1f s" % batch_s)
     same = all([h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in plain[i]] for i in range(len(jobs)))
     seeds = sum(c for _, c, _ in jobs)
     return {
-        "workload": "64 jobs k from splitmix64(0xB5F00006+k): size 2^(%d+%d u) seeds, window w x h uniform in "
-                    "1..8 chunks, 4 threshold profiles" % (args.batch_job_log2_lo, args.batch_job_log2_span),
+        "workload": "64 jobs k from splitmix64(0xB5F00006+k): size min(2^(20+14 u), 2^%d) seeds, window w x h "
+                    "uniform in 1..8 chunks, 4 threshold profiles; Stage-1 settings from the autotuner"
+                    % args.batch_job_cap_log2,
+        "jobs_at_cap": sum(1 for _, c, _ in jobs if c == 1 << args.batch_job_cap_log2),
         "seeds": seeds,
         "hits": int(sum(per_job)),
         "batched_seeds_per_s": round(seeds / batch_s, 1),
@@ -202,7 +222,11 @@ def served_trace(bsf, seconds, rate, median_log2, seed=0xB5F00008):
         out.append((t, first, size, job_params(bsf, w, h, prof, float(rng.random()), float(rng.random()))))
 
 
-def run_trace(bsf, trace, fifo, horizon_s, pass_log2=17):
+def run_trace(bsf, trace, fifo, horizon_s, pass_log2=17, stop_at_horizon=False):
+    """Replays a trace of (arrival s, first seed, seeds, params) against a server.  With
+    stop_at_horizon the server is stopped at horizon_s whatever is left (the stream's own law
+    offers far more work than one GPU does); otherwise the run ends when every job is done or at
+    horizon_s.  Latencies are over the jobs that reached them (first hit, completion)."""
     srv = bsf.Server(fifo=fifo, hit_buffer_bytes=(1 << pass_log2) * 128)
     ids, hits_seen = [], 0
     t0 = time.perf_counter()
@@ -215,26 +239,74 @@ def run_trace(bsf, trace, fifo, horizon_s, pass_log2=17):
                 ids.append(srv.submit(first, size, p))
                 i += 1
             hits_seen += sum(1 for e in srv.poll(timeout_ms=2) if e.kind == bsf.api.EVENT_HIT)
-            if i == len(trace):
+            if now > horizon_s:
+                break
+            if not stop_at_horizon and i == len(trace):
                 st = srv.stats()
-                if st.completed + st.rejected >= len(ids) or now > horizon_s:
+                if st.completed + st.rejected >= len(ids):
                     break
         wall = time.perf_counter() - t0
         infos = [srv.job(j) for j in ids]
         st = srv.stats()
     finally:
         srv.stop(drain=False)
+    progress("%s trace of %d jobs done" % ("plain FIFO" if fifo else "served", len(ids)))
     done = [x for x in infos if x.state == 2]
-    ttfh = [x.first_hit_ms - x.submit_ms for x in done if x.first_hit_ms >= 0]
+    ttfh = [x.first_hit_ms - x.submit_ms for x in infos if x.first_hit_ms >= 0]
     comp = [x.done_ms - x.submit_ms for x in done]
     seeds_done = sum(x.seeds_done for x in infos)
     return {
-        "jobs": len(ids), "completed": len(done), "rejected": int(st.rejected),
-        "unfinished_at_horizon": len(ids) - len(done) - int(st.rejected),
+        "jobs": len(ids), "admitted": int(st.admitted), "queued_at_end": int(st.queued),
+        "rejected": int(st.rejected), "completed": len(done),
+        "unfinished_at_end": int(st.admitted) - len(done),
+        "jobs_with_a_hit": len(ttfh),
         "ttfh_ms_p50": pct(ttfh, 50), "ttfh_ms_p99": pct(ttfh, 99),
         "completion_ms_p50": pct(comp, 50), "completion_ms_p99": pct(comp, 99),
         "seeds_per_s": round(seeds_done / wall, 1), "wall_s": round(wall, 2), "passes": int(st.passes),
         "hits_streamed": hits_seen,
+    }
+
+
+def host_info():
+    """The CPU the baselines run on: logical CPUs, the ones this process may use, the model
+    name, and the oracle threads used (the process's CPUs, at most 16: the GPU host grants each
+    GPU 16 CPUs)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "cpus_allowed": allowed, "model": model, "threads_used": min(16, allowed)}
+
+
+def run_cpu_configs(bsf, args, grid, scout_start, scout_thr):
+    """CPU oracle timings of the density grid (all 65,536 seeds) and of a prefix of the scout
+    range, with their results compared to the GPU's."""
+    import numpy as np
+    from oracle import oracle
+    threads = host_info()["threads_used"]
+    t = time.perf_counter()
+    _, digests = oracle.run_density(0, 65536, threads=threads)
+    dens_s = time.perf_counter() - t
+    gpu_digests = np.array([oracle.density_digest(grid[i]) for i in range(65536)], dtype=np.uint64)
+    n = 1 << args.cpu_scout_log2
+    t = time.perf_counter()
+    cpu_surv = oracle.run_scout(scout_start, n, scout_thr, threads=threads, cap=n)
+    scout_s = time.perf_counter() - t
+    gpu_surv = bsf.scout_range(scout_start, n, scout_thr, cap=n)
+    return {
+        "density_grid": {"seeds_per_s": round(65536 / dens_s, 1), "seconds": round(dens_s, 2), "cores": threads,
+                         "kind": "port", "sample": "all 65,536 seeds; per-seed digests equal the GPU's: %s"
+                                                   % bool(np.array_equal(digests, gpu_digests))},
+        "scout": {"seeds_per_s": round(n / scout_s, 1), "seconds": round(scout_s, 2), "cores": threads,
+                  "kind": "port", "sample": "the first 2^%d seeds of the range (%d survivors); survivor sets equal: "
+                                            "%s" % (args.cpu_scout_log2, len(cpu_surv),
+                                                    sorted(cpu_surv.tolist()) == gpu_surv.tolist())},
     }
 
 
@@ -255,6 +327,7 @@ def run_density_scout(bsf, args):
     t = time.perf_counter()
     surv = bsf.scout_range(start, n, thr)
     scout_s = time.perf_counter() - t
+    progress("density grid and scout done")
     digest = 0
     for v in surv.tolist():
         digest = (digest + splitmix64(v & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
@@ -269,9 +342,15 @@ def run_density_scout(bsf, args):
         totals += np.array([getattr(c, n) for n, _ in c._fields_], dtype=np.int64)
         batches.append(recs)
     s0_s = time.perf_counter() - t
+    progress("stage0stream done")
     n_worlds = 32768 * args.stage0_batches
     regions = run_regions(bsf, args, worlds, batches)
+    progress("region attempts done")
     chain = run_mega_hunt(bsf, args)
+    progress("MEGA hunt done")
+    cpu = None if args.no_cpu else run_cpu_configs(bsf, args, grid, start, thr)
+    if cpu:
+        progress("CPU density grid and scout prefix done")
     return {
         "density_grid": {"workload": "seeds 0..65535, chunk (0,0), 5x17x5 FP64 grids (host in/out, PCIe included)",
                          "seeds_per_s": round(65536 / dens_s, 1), "ms": round(dens_s * 1e3, 2),
@@ -288,6 +367,7 @@ def run_density_scout(bsf, args):
                          "counters": dict(zip([n for n, _ in bsf.Stage0Counters._fields_], totals.tolist()))},
         "region_attempts": regions,
         "mega_hunt": chain,
+        "cpu_oracle": cpu,
     }
 
 
@@ -370,42 +450,64 @@ def run_regions(bsf, args, worlds, batches):
 
 
 def run_serving(bsf, args):
+    """The served stream at its own law (Poisson 20 jobs/s, sizes min(2^36, 2^26 e^(2Z)), windows
+    1..8 x 1..8, four profiles) for a fixed horizon, continuous batching against the plain
+    one-job-at-a-time worker on the same engine; then a scaled-down stream (median 2^13 seeds)
+    that both run to completion."""
     trace = served_trace(bsf, args.serve_seconds, args.serve_rate, args.serve_median_log2)
-    served = run_trace(bsf, trace, False, args.serve_seconds + args.serve_drain, args.serve_pass_log2)
-    plain = run_trace(bsf, trace, True, args.serve_seconds + args.serve_drain, args.serve_pass_log2)
-    return {
-        "workload": "Poisson arrivals %.0f jobs/s for %.0f s; size min(2^36, round(2^%d e^(2Z))); windows w x h "
-                    "uniform in 1..8 chunks; 4 threshold profiles" % (args.serve_rate, args.serve_seconds,
-                                                                      args.serve_median_log2),
+    out = {
+        "workload": "Poisson arrivals %.0f jobs/s; size min(2^36, round(2^%d e^(2Z))); windows w x h uniform in "
+                    "1..8 chunks; 4 threshold profiles; arrivals and work stop at %.0f s (the law's 600 s offers "
+                    "about 10^10 seeds/s, far beyond one GPU)" % (args.serve_rate, args.serve_median_log2,
+                                                                   args.serve_seconds),
         "seeds_per_pass": 1 << args.serve_pass_log2,
-        "served": served,
-        "plain_fifo": plain,
+        "offered_seeds": int(sum(c for _, _, c, _ in trace)),
+        "served": run_trace(bsf, trace, False, args.serve_seconds, args.serve_pass_log2, stop_at_horizon=True),
+        "plain_fifo": run_trace(bsf, trace, True, args.serve_seconds, args.serve_pass_log2, stop_at_horizon=True),
     }
+    if args.small_serve_seconds > 0:
+        small = served_trace(bsf, args.small_serve_seconds, args.serve_rate, 13)
+        horizon = args.small_serve_seconds + args.small_serve_drain
+        out["scaled"] = {
+            "workload": "the same stream with sizes min(2^36, round(2^13 e^(2Z))) for %.0f s, run to completion"
+                        % args.small_serve_seconds,
+            "served": run_trace(bsf, small, False, horizon, args.serve_pass_log2),
+            "plain_fifo": run_trace(bsf, small, True, horizon, args.serve_pass_log2),
+        }
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch-log2", type=int, default=23)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=96 << 18,
+                    help="seeds per step (default 96 passes of 2^18 seeds: 20 steps take about a minute)")
+    ap.add_argument("--batch-log2", type=int, default=None, help="seeds per step as a power of two (overrides --batch)")
+    ap.add_argument("--timing-steps", type=int, default=2,
+                    help="steps of the same workload after the timed region with per-kernel HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--scout-log2", type=int, default=30)
+    ap.add_argument("--scout-log2", type=int, default=32)
+    ap.add_argument("--cpu-scout-log2", type=int, default=20)
     ap.add_argument("--stage0-batches", type=int, default=4)
     ap.add_argument("--region-cpu-items", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-seeds", type=int, default=1 << 15)
-    ap.add_argument("--batch-job-log2-lo", type=int, default=10)
-    ap.add_argument("--batch-job-log2-span", type=int, default=10)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-drain", type=float, default=45.0)
+    ap.add_argument("--batch-job-cap-log2", type=int, default=21,
+                    help="batched jobs keep their 2^(20+14u) sizes up to this cap")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="horizon of the served stream at its own law (arrivals and work stop there)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-median-log2", type=int, default=13)
+    ap.add_argument("--serve-median-log2", type=int, default=26)
     ap.add_argument("--serve-pass-log2", type=int, default=17)
+    ap.add_argument("--small-serve-seconds", type=float, default=8.0,
+                    help="arrival window of the secondary, scaled-down stream (median 2^13 seeds), run to completion")
+    ap.add_argument("--small-serve-drain", type=float, default=30.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (traffic-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -435,16 +537,17 @@ def main():
         del src, dst
     start, min_blocks = island_workload()
     params = bsf.island_params(-2, -2, 4, 4, min_blocks=min_blocks)
-    batch = 1 << args.batch_log2
+    batch = (1 << args.batch_log2) if args.batch_log2 is not None else args.batch
 
     def step(k):
         return bsf.island_range(start + k * batch, batch, params, cap=1 << 20)
 
     for k in range(args.warmup):
         step(k)
+    progress("%d warm-up steps done" % args.warmup)
+    # the timed region: the production path (each pass one captured hipGraph, no timers)
+    L.bsf_timing_enable(0)
     stats0 = bsf.engine_stats()
-    L.bsf_timing_reset()
-    L.bsf_timing_enable(1)
     sync()
     t0 = time.perf_counter()
     hits = 0
@@ -452,6 +555,20 @@ def main():
         hits += len(step(k))
     sync()
     elapsed = time.perf_counter() - t0
+    stats1 = bsf.engine_stats()
+    progress("timed region done: %d steps in %.1f s" % (args.steps, elapsed))
+    # per-kernel HIP events need the kernels launched one by one: the next steps of the same
+    # range run that way, untimed as a whole, and give the roofline's launch durations
+    L.bsf_timing_reset()
+    L.bsf_timing_enable(1)
+    t1 = time.perf_counter()
+    k0 = args.warmup + args.steps
+    for k in range(k0, k0 + args.timing_steps):
+        step(k)
+    sync()
+    eager_elapsed = time.perf_counter() - t1
+    stats2 = bsf.engine_stats()
+    progress("kernel-timing steps done")
     L.bsf_timing_enable(0)
 
     def kstats(name):
@@ -459,13 +576,12 @@ def main():
         L.bsf_kernel_stats(name.encode(), ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(units))
         return ms.value, launches.value, units.value
 
-    stats1 = bsf.engine_stats()
     seeds = args.steps * batch
     value = seeds / elapsed
     names = ["island_cache", "island_stage1", "island_stage1_multi", "island_stage1b", "island_stage2", "island_flood"]
     ks = {n: kstats(n) for n in names}
     passes = stats1[0] - stats0[0]
-    survivors = stats1[2] - stats0[2]
+    survivors = stats2[2] - stats1[2]  # Stage-1 survivors of the HIP-event steps
     # the FP64 kernels and their algorithmic work per launch
     fp64 = {
         "island_stage1": dict(kernel="pass_stage1_kernel<0> (Stage 1a: 64 sample columns per seed, low-frequency "
@@ -505,7 +621,12 @@ def main():
             "hits_in_timed_region": hits,
             "stage1_survivor_fraction": round((stats1[2] - stats0[2]) / max(1, stats1[1] - stats0[1]), 5),
             "passes": passes,
-            "kernel_ms": {n: round(ks[n][0], 3) for n in names},
+            "timed_path": "captured pass graphs, no timers",
+            "kernel_timing": {"steps": args.timing_steps, "ms_per_step": round(eager_elapsed * 1e3 /
+                                                                             max(1, args.timing_steps), 3),
+                              "path": "kernels launched one by one between HIP events, same workload after "
+                                      "the timed region",
+                              "kernel_ms": {n: round(ks[n][0], 3) for n in names}},
         },
         "roofline": {
             "bound": "valu",
@@ -521,6 +642,8 @@ def main():
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
             "traffic": int(round(HBM_BYTES_PER_UNIT[dom] * units_per_launch)),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x calibration + WRITE_SIZE)",
+            "traffic_source": "per-unit HBM bytes of the --pmc FETCH_SIZE / WRITE_SIZE runs in "
+                              "profiles/island_pipeline_4x4_v1134.md x this run's units per launch",
             # the published peak against the device's own figures: CUs x 128 FP64 flops per clock
             "device": {"compute_units": dev_cu, "peak_clock_mhz": round(dev_clk / 1e3, 1),
                        "fp64_peak_tflops": round(dev_cu * 128 * dev_clk * 1e3 / 1e12, 2)},
@@ -529,7 +652,8 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle
-        threads = min(16, os.cpu_count() or 1)
+        host = host_info()
+        threads = host["threads_used"]
         # bounded sample: grow until ~cpu_seconds of oracle work
         n = 256
         t = time.perf_counter()
@@ -551,7 +675,9 @@ def main():
             "kind": "port",
             "sample": "first %d seeds of the same range through the CPU oracle (%d threads, %.1f s); "
                       "hit lists equal: %s" % (n, threads, dt, gpu_sample == cpu_hits),
+            "host": host,
         }
+        progress("CPU baseline done: %d seeds in %.1f s" % (n, dt))
     if not args.no_configs:
         result["other_workloads"] = run_density_scout(bsf, args)
     if not args.no_batched:

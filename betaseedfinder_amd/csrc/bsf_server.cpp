@@ -59,6 +59,7 @@ void bsf_server::run() {
         bsf::check(hipSetDevice(bsf::g_ctx.device), "hipSetDevice");
         bsf::IslandEngine::PassResult r;
         for (;;) {
+            bool halt = false;  // stop without drain: finish the passes in flight, launch no more
             {
                 std::unique_lock<std::mutex> lk(mu);
                 // admission: free job-table entries (state memory); fifo admits one job at a time
@@ -86,9 +87,10 @@ void bsf_server::run() {
                     cv_work.wait_for(lk, std::chrono::milliseconds(20));
                     continue;
                 }
-                if (stopping && !drain && eng.inflight() == 0) break;
+                halt = stopping && !drain;
+                if (halt && eng.inflight() == 0) break;
             }
-            while (eng.inflight() < 2 && eng.has_work()) eng.launch(false);
+            while (!halt && eng.inflight() < 2 && eng.has_work()) eng.launch(false);
             if (eng.inflight() == 0) continue;
             eng.complete_oldest(r);
             const double t = now_ms();

@@ -79,13 +79,22 @@ def test_batch_matches_plain_runs(bsf):
     assert sum(per_job) > 0
 
 
-def test_batch_matches_oracle(bsf, oracle):
-    jobs = batch_jobs(bsf)[:4]
-    hits, _ = bsf.island_batch(jobs)
-    for i, (first, count, p) in enumerate(jobs):
+@pytest.fixture(scope="module")
+def oracle_job_hits(bsf, oracle):
+    """The oracle's hit list of every seed of every batch_jobs job (about 22,000 seeds)."""
+    out = []
+    for first, count, p in batch_jobs(bsf):
         po = oracle.island_params(**{f: getattr(p, f) for f, _ in p._fields_})
-        ref = oracle.run_island(first, count, po, threads=16)
-        assert [h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in ref], i
+        out.append([h[:-1] for h in oracle.run_island(first, count, po, threads=16)] if count else [])
+    return out
+
+
+def test_batch_matches_oracle(bsf, oracle_job_hits):
+    jobs = batch_jobs(bsf)
+    hits, _ = bsf.island_batch(jobs)
+    for i in range(len(jobs)):
+        assert [h[:-1] for h in hits if h[-1] == i] == oracle_job_hits[i], i
+    assert sum(len(r) for r in oracle_job_hits) > 0
 
 
 def test_small_passes_and_tiny_shares(bsf):
@@ -122,13 +131,17 @@ def serve(bsf, jobs, fifo, gap_s=0.0):
 
 
 @pytest.mark.parametrize("fifo", [False, True])
-def test_served_results_equal_alone_and_in_mix(bsf, fifo):
+def test_served_results_equal_alone_and_in_mix(bsf, fifo, oracle_job_hits):
+    """Every served job's hits, in the mix (continuous batching) and one at a time (FIFO), equal
+    its plain run alone and the oracle's hit list over all its seeds."""
     jobs = [j for j in batch_jobs(bsf) if j[1] > 0]
+    ref = [r for j, r in zip(batch_jobs(bsf), oracle_job_hits) if j[1] > 0]
     ids, events, infos, st = serve(bsf, jobs, fifo, gap_s=0.01)
-    for jid, (first, count, p) in zip(ids, jobs):
+    for k, (jid, (first, count, p)) in enumerate(zip(ids, jobs)):
         alone = sorted(h[:-1] for h in bsf.island_range(first, count, p))
         mixed = sorted(hit[:-1] for job, kind, hit in events if job == jid and kind == bsf.api.EVENT_HIT)
         assert mixed == alone, jid
+        assert mixed == ref[k], jid
         info = infos[ids.index(jid)]
         assert info.state == 2 and info.seeds_done == count and info.hits == len(alone)
         assert info.submit_ms <= info.admit_ms <= info.done_ms
