@@ -9,6 +9,7 @@
 #include "bsf_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -1530,6 +1531,362 @@ int bsfo_coarse_score(int64_t seed, int32_t ox, int32_t oz, uint8_t* signs_out) 
                        rain.data(), signs.data());
     if (signs_out) std::memcpy(signs_out, signs.data(), signs.size());
     return coarseScoreSigns(signs.data());
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP64 operation count of island Stage 2: the work the staged pipeline does for one Stage-1
+// survivor of a window, written once over a number type T.  With T = double it computes the
+// window's lattice densities at levels 8..15 (checked bit for bit against bsfo_chunk_density);
+// with T = CountedDouble it tallies the FP64 operations that computation executes:
+//  - per 3D octave (noise2, noise3: 16 each; noise1: 8): the octave's scales and weight, the
+//    y axis of levels 8..15 once (with the walk back to the start of level 8's cell), then per
+//    lattice column the x and z axes, the corner gradients and x-lerps of every Y cell
+//    (generateNoise's legacy bulk method, NoiseGeneratorPerlin173.java:224-283, which recomputes
+//    them only when the masked y index changes), and per level the y / z lerps, the weight and
+//    the accumulation;
+//  - per 2D octave (noise4: 10, noise5: 16) and lattice column the yLen == 1 branch (:188-223);
+//  - per density column (25 per chunk, climate at block (3i+1, 3l+1), BetaTerrain173.java:741-749)
+//    the 10 simplex octaves (NoiseGenerator2173.java:117-172), WorldChunkManager173's climate
+//    post-processing (:100-127), the column shape and the densities of levels 8..15 (:741-831).
+// A Java gradient is one add (the sign flips are negations, exact and free); comparisons,
+// int <-> double conversions, adds, subtractions, multiplications and divisions are counted.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct OpTally {
+    uint64_t add = 0, mul = 0, div = 0, cmp = 0, cvt = 0;
+};
+thread_local OpTally* t_tally = nullptr;
+
+struct CountedDouble {
+    double v = 0.0;
+    CountedDouble() = default;
+    CountedDouble(double x) : v(x) {}  // constants: no operation
+};
+using CD = CountedDouble;
+inline CD operator+(CD a, CD b) { ++t_tally->add; return CD(a.v + b.v); }
+inline CD operator-(CD a, CD b) { ++t_tally->add; return CD(a.v - b.v); }
+inline CD operator-(CD a) { return CD(-a.v); }
+inline CD operator*(CD a, CD b) { ++t_tally->mul; return CD(a.v * b.v); }
+inline CD operator/(CD a, CD b) { ++t_tally->div; return CD(a.v / b.v); }
+inline bool operator<(CD a, CD b) { ++t_tally->cmp; return a.v < b.v; }
+inline bool operator>(CD a, CD b) { ++t_tally->cmp; return a.v > b.v; }
+
+template <class T> T i2d(int i);
+template <> inline double i2d<double>(int i) { return static_cast<double>(i); }
+template <> inline CD i2d<CD>(int i) { ++t_tally->cvt; return CD(static_cast<double>(i)); }
+inline int d2i(double v) { return jd2i(v); }
+inline int d2i(CD v) { ++t_tally->cvt; return jd2i(v.v); }
+inline double raw(double v) { return v; }
+inline double raw(CD v) { return v.v; }
+
+template <class T> T fadeT(T v) { return v * v * v * (v * (v * T(6.0) - T(15.0)) + T(10.0)); }
+template <class T> T lerpT(T t, T a, T b) { return a + t * (b - a); }
+template <class T> int floorT(T v) {
+    int i = d2i(v);
+    if (v < i2d<T>(i)) --i;
+    return i;
+}
+template <class T> struct AxisT {
+    int i;
+    T f, fm1, fd;
+};
+template <class T> AxisT<T> axisT(T v) {
+    AxisT<T> a;
+    const int fl = floorT(v);
+    a.i = fl & 255;
+    a.f = v - i2d<T>(fl);
+    a.fm1 = a.f - T(1.0);
+    a.fd = fadeT(a.f);
+    return a;
+}
+template <class T> T grad3T(int h, T x, T y, T z) {  // :145-165
+    switch (h & 15) {
+        case 0: return x + y;
+        case 1: return -x + y;
+        case 2: return x + -y;
+        case 3: return -x + -y;
+        case 4: return x + z;
+        case 5: return -x + z;
+        case 6: return x + -z;
+        case 7: return -x + -z;
+        case 8: return y + z;
+        case 9: return -y + z;
+        case 10: return y + -z;
+        case 11: return -y + -z;
+        case 12: return y + x;
+        case 13: return -y + z;
+        case 14: return y + -x;
+        default: return -y + -z;
+    }
+}
+template <class T> T grad2T(int h, T x, T z) {  // :123-143
+    const T zeroX = T(0.0) * x;
+    switch (h & 15) {
+        case 0: return x + T(0.0);
+        case 1: return -x + T(0.0);
+        case 2: return x + T(-0.0);
+        case 3: return -x + T(-0.0);
+        case 4: return x + z;
+        case 5: return -x + z;
+        case 6: return x + -z;
+        case 7: return -x + -z;
+        case 8: return zeroX + z;
+        case 9: return -zeroX + z;
+        case 10: return zeroX + -z;
+        case 11: return -zeroX + -z;
+        case 12: return zeroX + x;
+        case 13: return -zeroX + z;
+        case 14: return zeroX + -x;
+        default: return -zeroX + -z;
+    }
+}
+
+constexpr int kS2Y0 = 8, kS2Levels = 8;
+
+// One 3D octave group over the window's lattice columns, levels 8..15; acc[col][k].
+template <class T>
+void stage2Octaves3d(const Octaves& oct, double sxz, double sy0, int x0, int z0, int nx, int nz,
+                     std::vector<std::array<T, kS2Levels>>& acc) {
+    T amp = T(1.0);
+    for (size_t o = 0; o < oct.g.size(); ++o) {
+        const Perlin& p = oct.g[o];
+        const T sx = T(sxz) * amp, sy = T(sy0) * amp, w = T(1.0) / amp;
+        // the y axis of levels 8..15, once per octave; a level starts a cell when its masked
+        // index differs from the level below (level 8's cell may start below level 8: its
+        // corners use that level's fraction)
+        int yi[kS2Levels];
+        bool fresh[kS2Levels];
+        T cell_f[kS2Levels], fd[kS2Levels];
+        for (int k = 0; k < kS2Levels; ++k) {
+            const T v = i2d<T>(kS2Y0 + k) * sy + T(p.b);
+            const int fl = floorT(v);
+            yi[k] = fl & 255;
+            const T f = v - i2d<T>(fl);
+            fd[k] = fadeT(f);
+            fresh[k] = k == 0 || yi[k] != yi[k - 1];
+            cell_f[k] = fresh[k] ? f : cell_f[k - 1];
+        }
+        for (int y = kS2Y0 - 1; y >= 0; --y) {
+            const T v = i2d<T>(y) * sy + T(p.b);
+            const int fl = floorT(v);
+            if ((fl & 255) != yi[0]) break;
+            cell_f[0] = v - i2d<T>(fl);
+        }
+        T cell_fm1[kS2Levels];
+        for (int k = 0; k < kS2Levels; ++k)
+            if (fresh[k]) cell_fm1[k] = cell_f[k] - T(1.0);
+            else cell_fm1[k] = cell_fm1[k - 1];
+        for (int u = 0; u < nx; ++u) {
+            const AxisT<T> ax = axisT(i2d<T>(x0 + u) * sx + T(p.a));
+            for (int v = 0; v < nz; ++v) {
+                const AxisT<T> az = axisT(i2d<T>(z0 + v) * sx + T(p.c));
+                auto& out = acc[u * nz + v];
+                T q00 = T(0.0), q01 = T(0.0), q10 = T(0.0), q11 = T(0.0);
+                for (int k = 0; k < kS2Levels; ++k) {
+                    if (fresh[k]) {
+                        const int p60 = p.d[ax.i] + yi[k], p61 = p.d[p60] + az.i, p62 = p.d[p60 + 1] + az.i;
+                        const int p63 = p.d[ax.i + 1] + yi[k], p33 = p.d[p63] + az.i, p64 = p.d[p63 + 1] + az.i;
+                        const T yf = cell_f[k], yfm1 = cell_fm1[k];
+                        q00 = lerpT(ax.fd, grad3T(p.d[p61], ax.f, yf, az.f), grad3T(p.d[p33], ax.fm1, yf, az.f));
+                        q01 = lerpT(ax.fd, grad3T(p.d[p62], ax.f, yfm1, az.f), grad3T(p.d[p64], ax.fm1, yfm1, az.f));
+                        q10 = lerpT(ax.fd, grad3T(p.d[p61 + 1], ax.f, yf, az.fm1),
+                                    grad3T(p.d[p33 + 1], ax.fm1, yf, az.fm1));
+                        q11 = lerpT(ax.fd, grad3T(p.d[p62 + 1], ax.f, yfm1, az.fm1),
+                                    grad3T(p.d[p64 + 1], ax.fm1, yfm1, az.fm1));
+                    }
+                    const T r0 = lerpT(fd[k], q00, q01), r1 = lerpT(fd[k], q10, q11);
+                    const T val = lerpT(az.fd, r0, r1) * w;
+                    out[k] = o == 0 ? T(0.0) + val : out[k] + val;
+                }
+            }
+        }
+        amp = amp / T(2.0);
+    }
+}
+
+// One 2D octave group (generateNoise's yLen == 1 overload) over the window's lattice columns.
+template <class T>
+void stage2Octaves2d(const Octaves& oct, double sc, int x0, int z0, int nx, int nz, std::vector<T>& acc) {
+    T amp = T(1.0);
+    for (size_t o = 0; o < oct.g.size(); ++o) {
+        const Perlin& p = oct.g[o];
+        const T sx = T(sc) * amp, w = T(1.0) / amp;
+        for (int u = 0; u < nx; ++u) {
+            const AxisT<T> ax = axisT(i2d<T>(x0 + u) * sx + T(p.a));
+            for (int v = 0; v < nz; ++v) {
+                const AxisT<T> az = axisT(i2d<T>(z0 + v) * sx + T(p.c));
+                const int p47 = p.d[p.d[ax.i]] + az.i, p20 = p.d[p.d[ax.i + 1]] + az.i;
+                const T q0 = lerpT(ax.fd, grad2T(p.d[p47], ax.f, az.f), grad3T(p.d[p20], ax.fm1, T(0.0), az.f));
+                const T q1 = lerpT(ax.fd, grad3T(p.d[p47 + 1], ax.f, T(0.0), az.fm1),
+                                   grad3T(p.d[p20 + 1], ax.fm1, T(0.0), az.fm1));
+                const T val = lerpT(az.fd, q0, q1) * w;
+                T& out = acc[u * nz + v];
+                out = o == 0 ? T(0.0) + val : out + val;
+            }
+        }
+        amp = amp / T(2.0);
+    }
+}
+
+template <class T> T simplexGradT(int g, T x, T z) {  // gradient2, NoiseGenerator2173.java:58-72
+    switch (g) {
+        case 0: return T(1.0) * x + T(1.0) * z;
+        case 1: return T(-1.0) * x + T(1.0) * z;
+        case 2: return T(1.0) * x + T(-1.0) * z;
+        case 3: return T(-1.0) * x + T(-1.0) * z;
+        case 4:
+        case 6: return T(1.0) * x + T(0.0) * z;
+        case 5:
+        case 7: return T(-1.0) * x + T(0.0) * z;
+        case 8:
+        case 10: return T(0.0) * x + T(1.0) * z;
+        default: return T(0.0) * x + T(-1.0) * z;
+    }
+}
+template <class T> int simplexFloorT(T v) { return v > T(0.0) ? d2i(v) : d2i(v) - 1; }
+template <class T> T simplexSampleT(const Simplex& s, T d5, T d6) {  // :117-172
+    const T f = T(kSimplexF), g = T(kSimplexG);
+    const T d7 = (d5 + d6) * f;
+    const int j1 = simplexFloorT(d5 + d7), k1 = simplexFloorT(d6 + d7);
+    const T d8 = i2d<T>(j1 + k1) * g;
+    const T d9 = i2d<T>(j1) - d8, d10 = i2d<T>(k1) - d8;
+    const T d11 = d5 - d9, d12 = d6 - d10;
+    const int b0 = d11 > d12 ? 1 : 0, b1 = 1 - b0;
+    const T d13 = d11 - i2d<T>(b0) + g, d14 = d12 - i2d<T>(b1) + g;
+    const T d15 = d11 - T(1.0) + T(2.0) * g, d16 = d12 - T(1.0) + T(2.0) * g;
+    const int l1 = j1 & 255, i2 = k1 & 255;
+    const int j2 = s.e[l1 + s.e[i2]] % 12, k2 = s.e[l1 + b0 + s.e[i2 + b1]] % 12, l2 = s.e[l1 + 1 + s.e[i2 + 1]] % 12;
+    T d17 = T(0.5) - d11 * d11 - d12 * d12, d18, d19 = T(0.5) - d13 * d13 - d14 * d14, d20;
+    T d21 = T(0.5) - d15 * d15 - d16 * d16, d22;
+    if (d17 < T(0.0)) d18 = T(0.0);
+    else { d17 = d17 * d17; d18 = d17 * d17 * simplexGradT(j2, d11, d12); }
+    if (d19 < T(0.0)) d20 = T(0.0);
+    else { d19 = d19 * d19; d20 = d19 * d19 * simplexGradT(k2, d13, d14); }
+    if (d21 < T(0.0)) d22 = T(0.0);
+    else { d21 = d21 * d21; d22 = d21 * d21 * simplexGradT(l2, d15, d16); }
+    return T(70.0) * (d18 + d20 + d22);
+}
+template <class T>
+T stage2ClimateGroup(const SimplexOctaves& so, double start, double octave_scale, int bx, int bz) {
+    const T d2 = T(start) / T(1.5);
+    T d6 = T(1.0), d7 = T(1.0), acc = T(0.0);
+    for (size_t o = 0; o < so.g.size(); ++o) {
+        const Simplex& s = so.g[o];
+        const T sc = d2 * d7, w = T(0.55) / d6;
+        const T v = simplexSampleT(s, i2d<T>(bx) * sc + T(s.a), i2d<T>(bz) * sc + T(s.b)) * w;
+        acc = o == 0 ? T(0.0) + v : acc + v;
+        d7 = d7 * T(octave_scale);
+        d6 = d6 * T(0.5);
+    }
+    return acc;
+}
+
+// The survivor's Stage-2 work over window p; out[(chunk*25 + i*5 + l)*8 + k] = density of level
+// 8 + k of density column (i, l) of chunk (chunk / chunks_z, chunk % chunks_z).
+template <class T>
+void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
+    const int nx = 4 * p.chunks_x + 1, nz = 4 * p.chunks_z + 1, x0 = 4 * p.chunk_x0, z0 = 4 * p.chunk_z0;
+    const int nc = nx * nz;
+    std::vector<std::array<T, kS2Levels>> a2(nc), a3(nc), a1(nc);
+    std::vector<T> a4(nc), a5(nc);
+    stage2Octaves3d<T>(t.n2, 684.412, 684.412, x0, z0, nx, nz, a2);
+    stage2Octaves3d<T>(t.n3, 684.412, 684.412, x0, z0, nx, nz, a3);
+    stage2Octaves3d<T>(t.n1, 684.412 / 80.0, 684.412 / 160.0, x0, z0, nx, nz, a1);
+    stage2Octaves2d<T>(t.n4, 1.121, x0, z0, nx, nz, a4);
+    stage2Octaves2d<T>(t.n5, 200.0, x0, z0, nx, nz, a5);
+    for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
+        const int cx = p.chunk_x0 + ch / p.chunks_z, cz = p.chunk_z0 + ch % p.chunks_z;
+        for (int i = 0; i < 5; ++i)
+            for (int l = 0; l < 5; ++l) {
+                const int bx = cx * 16 + 3 * i + 1, bz = cz * 16 + 3 * l + 1;
+                const T tr = stage2ClimateGroup<T>(t.climate.e, 0.02500000037252903, 0.25, bx, bz);
+                const T rr = stage2ClimateGroup<T>(t.climate.f, 0.05000000074505806, 0.3333333333333333, bx, bz);
+                const T cr = stage2ClimateGroup<T>(t.climate.g, 0.25, 0.5882352941176471, bx, bz);
+                // WorldChunkManager173.java:100-127
+                const T c0 = cr * T(1.1) + T(0.5);
+                T d3 = (tr * T(0.15) + T(0.7)) * (T(1.0) - T(0.01)) + c0 * T(0.01);
+                T d4 = (rr * T(0.15) + T(0.5)) * (T(1.0) - T(0.0020)) + c0 * T(0.0020);
+                d3 = T(1.0) - (T(1.0) - d3) * (T(1.0) - d3);
+                if (d3 < T(0.0)) d3 = T(0.0);
+                if (d4 < T(0.0)) d4 = T(0.0);
+                if (d3 > T(1.0)) d3 = T(1.0);
+                if (d4 > T(1.0)) d4 = T(1.0);
+                // column shape, BetaTerrain173.java:751-790
+                const int col = (ch / p.chunks_z * 4 + i) * nz + (ch % p.chunks_z * 4 + l);
+                const T rain = d4 * d3;
+                T e4 = T(1.0) - rain;
+                e4 = e4 * e4;
+                e4 = e4 * e4;
+                e4 = T(1.0) - e4;
+                T d5 = (a4[col] + T(256.0)) / T(512.0);
+                d5 = d5 * e4;
+                if (d5 > T(1.0)) d5 = T(1.0);
+                T d6 = a5[col] / T(8000.0);
+                if (d6 < T(0.0)) d6 = -d6 * T(0.3);
+                d6 = d6 * T(3.0) - T(2.0);
+                if (d6 < T(0.0)) {
+                    d6 = d6 / T(2.0);
+                    if (d6 < T(-1.0)) d6 = T(-1.0);
+                    d6 = d6 / T(1.4);
+                    d6 = d6 / T(2.0);
+                    d5 = T(0.0);
+                } else {
+                    if (d6 > T(1.0)) d6 = T(1.0);
+                    d6 = d6 / T(8.0);
+                }
+                if (d5 < T(0.0)) d5 = T(0.0);
+                d5 = d5 + T(0.5);
+                d6 = d6 * T(17.0) / T(16.0);
+                const T d7 = T(17.0) / T(2.0) + d6 * T(4.0);
+                for (int k = 0; k < kS2Levels; ++k) {
+                    const int y = kS2Y0 + k;
+                    T d9 = (i2d<T>(y) - d7) * T(12.0) / d5;
+                    if (d9 < T(0.0)) d9 = d9 * T(4.0);
+                    const T d10 = a2[col][k] / T(512.0), d11 = a3[col][k] / T(512.0);
+                    const T d12 = (a1[col][k] / T(10.0) + T(1.0)) / T(2.0);
+                    T d8;
+                    if (d12 < T(0.0)) d8 = d10;
+                    else if (d12 > T(1.0)) d8 = d11;
+                    else d8 = d10 + (d11 - d10) * d12;
+                    d8 = d8 - d9;
+                    if (y > 13) {
+                        const T d13 = T(static_cast<double>(static_cast<float>(y - 13) / 3.0F));
+                        d8 = d8 * (T(1.0) - d13) + T(-10.0) * d13;
+                    }
+                    out[(ch * 25 + i * 5 + l) * kS2Levels + k] = d8;
+                }
+            }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void bsfo_stage2_op_count(int64_t seed, const bsfo_island_params* p, uint64_t* counts5, double* lattice_out) {
+    Terrain t(seed);
+    const int n = p->chunks_x * p->chunks_z * 25 * kS2Levels;
+    OpTally tally;
+    t_tally = &tally;
+    std::vector<CD> counted(n);
+    stage2Window<CD>(t, *p, counted.data());
+    t_tally = nullptr;
+    counts5[0] = tally.add;
+    counts5[1] = tally.mul;
+    counts5[2] = tally.div;
+    counts5[3] = tally.cmp;
+    counts5[4] = tally.cvt;
+    if (lattice_out) {
+        std::vector<double> plain(n);
+        stage2Window<double>(t, *p, plain.data());
+        for (int i = 0; i < n; ++i) lattice_out[i] = plain[i];
+        // the counted evaluation computes the same values
+        for (int i = 0; i < n; ++i)
+            if (std::memcmp(&plain[i], &counted[i].v, 8) != 0) lattice_out[i] = std::numeric_limits<double>::quiet_NaN();
+    }
 }
 
 }  // extern "C"

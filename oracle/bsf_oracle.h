@@ -182,6 +182,14 @@ void     bsfo_coarse_arrays(int64_t seed, int32_t coarse_offset_x, int32_t coars
                             double* n1, double* n2, double* n3, double* n4, double* n5,
                             double* temp, double* rain, uint8_t* signs);
 
+/* FP64 operations of island Stage 2 for one seed over window p (the 40 3D octaves at levels
+ * 8..15 and the 26 2D octaves of the (4w+1)(4h+1) lattice columns, climate, column shape and
+ * densities of the 25wh density columns): counts5 = {add/sub, mul, div, compare, int<->double
+ * conversion}.  lattice_out (optional, 25wh*8 doubles) receives the densities the same code
+ * computes with plain doubles, [(chunk*25 + i*5 + l)*8 + level - 8]; an entry where the counted
+ * evaluation differed is NaN. */
+void     bsfo_stage2_op_count(int64_t seed, const bsfo_island_params* p, uint64_t* counts5, double* lattice_out);
+
 #ifdef __cplusplus
 }
 #endif
