@@ -255,7 +255,7 @@ EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
     return a;
 }
 
-void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
+void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int buckets) {
     const EngineArgs a = args_for(ab, stage1_only);
     hipStream_t s = stream_;
     const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
@@ -291,11 +291,15 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int bucket) {
     }
     if (stage1_only) return;
     {
+        // one Stage-2 kernel per lattice layout present in the pass (bit b of buckets)
         LaunchTimer t("island_stage2", s, timers_);
-        auto s2 = bucket == 0 ? pass_stage2_kernel<true> : pass_stage2_kernel<false>;
-        hipLaunchKernelGGL(s2, dim3(grid_s2_[bucket]), dim3(64),
-                           static_cast<size_t>(2 * kNdBucket[bucket]) * sizeof(double), s, a);
-        check(hipGetLastError(), "stage 2 launch");
+        for (int b = 0; b < 2; ++b) {
+            if (!((buckets >> b) & 1)) continue;
+            auto s2 = b == 0 ? pass_stage2_kernel<true> : pass_stage2_kernel<false>;
+            hipLaunchKernelGGL(s2, dim3(grid_s2_[b]), dim3(64), static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double),
+                               s, a);
+            check(hipGetLastError(), "stage 2 launch");
+        }
         t.stop(0);
     }
     {
@@ -333,7 +337,7 @@ void IslandEngine::launch(bool stage1_only) {
     f.seq = ++seq_;
     f.stage1_only = stage1_only;
     // upload admitted jobs, then plan the pass exactly as pass_build will
-    int nd_max = 0;
+    int buckets = 0;
     for (int i = 0; i < cfg_jobs_; ++i) {
         HostJob& h = host_jobs_[i];
         if (h.used && h.upload) {
@@ -342,16 +346,16 @@ void IslandEngine::launch(bool stage1_only) {
                   "H2D job entry");
             h.upload = false;
         }
-        if (h.used && h.cursor < h.count) nd_max = std::max(nd_max, h.nd);
+        if (h.used && h.cursor < h.count) buckets |= 1 << bucket_of(h.nd);
     }
     f.seeds = plan_pass(f.cursors, f.finished, f.seeds_kind);
-    const int bucket = bucket_of(nd_max);
+    if (buckets == 0) buckets = 1;
     if (timers_ && timing_enabled()) {
-        enqueue_kernels(ab, stage1_only, bucket);
+        enqueue_kernels(ab, stage1_only, buckets);
     } else if (!use_graphs_) {
-        enqueue_kernels(ab, stage1_only, bucket);
+        enqueue_kernels(ab, stage1_only, buckets);
     } else {
-        const int key = ab | (stage1_only ? 2 : 0) | (bucket << 2);
+        const int key = ab | (stage1_only ? 2 : 0) | (buckets << 2);
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
             const bool saved = timers_;
@@ -359,7 +363,7 @@ void IslandEngine::launch(bool stage1_only) {
             hipGraph_t g = nullptr;
             check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_kernels(ab, stage1_only, bucket);
+                enqueue_kernels(ab, stage1_only, buckets);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &g);
                 if (g) (void)hipGraphDestroy(g);

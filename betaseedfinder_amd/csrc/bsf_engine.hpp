@@ -97,7 +97,7 @@ class IslandEngine {
         std::vector<std::pair<int, int64_t>> cursors;  // (slot, cursor after this pass)
         std::vector<int> finished;
     };
-    void enqueue_kernels(int ab, bool stage1_only, int bucket);
+    void enqueue_kernels(int ab, bool stage1_only, int buckets);
     void drop_graphs();
     EngineArgs args_for(int ab, bool stage1_only) const;
     int64_t plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
@@ -127,12 +127,9 @@ class IslandEngine {
     int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[2] = {0, 0}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
     int64_t flood_scratch_words_ = 0;
     int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | bucket << 2
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | buckets << 2
     static constexpr int64_t kHitPrefix = 4096;  // hits copied back with every pass
 };
-
-// Density columns per survivor of the two lattice layouts (4x4 and 8x8 windows).
-constexpr int kNdBucket[2] = {400, 1600};
 
 // The library's own engine (plain and batched entry points, under the library lock).
 IslandEngine& engine();

@@ -72,8 +72,13 @@ enum {
     kCtrStage1Multi = 13,
     kCtrOpen = 14,       // open columns left by Stage 1a
     kCtrStage1b = 15,
-    kCounters = 16
+    kCtrStage2Wide = 16, // Stage-2 queue head of the survivors of windows above 4x4 chunks
+    kCounters = 17
 };
+
+// Density columns per survivor of the two Stage-2 lattice layouts (windows up to 4x4 chunks, up
+// to 8x8 chunks); each layout has its own Stage-2 kernel and queue.
+constexpr int kNdBucket[2] = {400, 1600};
 
 // Flood classes by window size (block bitsets of one window: rows x 64-bit z words).
 enum {
