@@ -174,7 +174,7 @@ def run_batched(bsf, args):
     if not args.no_tune:
         for (w, h) in shapes:
             tt = bsf.autotune(bsf.island_params(-(w // 2), -(h // 2), w, h), 0x5EED, args.tune_seeds)
-            tuned["%dx%d" % (w, h)] = [tt.worlds_per_wave, tt.gate]
+            tuned["%dx%d" % (w, h)] = [tt.worlds_per_wave, tt.gate, tt.stage2_fma]
     tune_s = time.perf_counter() - t
     progress("autotuner done: %d shapes" % len(shapes))
     bsf.island_range(jobs[0][0], 4096, jobs[0][2])  # warm the pass graphs

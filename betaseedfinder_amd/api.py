@@ -340,7 +340,7 @@ class IslandJob(ctypes.Structure):
 
 class IslandTuning(ctypes.Structure):
     _fields_ = [("worlds_per_wave", ctypes.c_int32), ("gate", ctypes.c_int32), ("seeds_per_sec", ctypes.c_double),
-                ("candidates", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("candidates", ctypes.c_int32), ("stage2_fma", ctypes.c_int32)]
 
 
 def island_batch(jobs, cap=1 << 20):
@@ -389,10 +389,19 @@ def get_tuning(params):
     return t
 
 
-def set_tuning(params, worlds_per_wave, gate=1):
+def set_tuning(params, worlds_per_wave, gate=1, stage2_fma=-1):
     init()
-    t = IslandTuning(worlds_per_wave, gate, 0.0, 0, 0)
+    t = IslandTuning(worlds_per_wave, gate, 0.0, 0, stage2_fma)
     _check(lib().bsf_island_set_tuning(ctypes.byref(params), ctypes.byref(t)))
+
+
+def autotune_engine(params, first_seed, trial_seeds):
+    """Times engine passes of 2^16 .. 2^19 seeds and configures the fastest: (pass slots, seeds/s)."""
+    init()
+    slots, rate = ctypes.c_int64(0), ctypes.c_double(0.0)
+    _check(lib().bsf_island_autotune_engine(ctypes.byref(params), ctypes.c_int64(first_seed),
+                                            ctypes.c_int64(trial_seeds), ctypes.byref(slots), ctypes.byref(rate)))
+    return slots.value, rate.value
 
 
 # ---------------------------------------------------------------------------

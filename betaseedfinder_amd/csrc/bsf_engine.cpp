@@ -42,7 +42,11 @@ T* pinned(size_t n) {
     return static_cast<T*>(p);
 }
 
-int bucket_of(int nd) { return nd <= kNdBucket[0] ? 0 : 1; }
+// The Stage-2 kernel and dynamic LDS (temperature and rain of the layout's density columns) of a
+// launch variant.
+using Stage2Kernel = void (*)(EngineArgs);
+Stage2Kernel stage2_kernel(int v) { return v == kS2FmaSmall ? pass_stage2_kernel<true> : pass_stage2_kernel<false>; }
+size_t stage2_lds(int v) { return static_cast<size_t>(2 * kNdBucket[v == kS2SelectLarge ? 1 : 0]) * sizeof(double); }
 
 }  // namespace
 
@@ -127,9 +131,7 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
-    for (int b = 0; b < 2; ++b)
-        grid_s2_[b] = b == 0 ? occupancy_grid(pass_stage2_kernel<true>, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0)
-                             : occupancy_grid(pass_stage2_kernel<false>, 64, static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double), 0);
+    for (int v = 0; v < kS2Variants; ++v) grid_s2_[v] = occupancy_grid(stage2_kernel(v), 64, stage2_lds(v), 0);
     grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(0), 0);
     grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(1), 0);
     grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, kFloodThreads, flood_lds_bytes(2), 0);
@@ -174,6 +176,8 @@ int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64
     j.worlds_per_wave = tuning.gate ? std::max(1, std::min(tuning.worlds_per_wave, kMaxWorldsPerWave)) : 64;
     if (tuning.gate && j.worlds_per_wave * 4 * p.chunks_x * p.chunks_z > 64) j.worlds_per_wave = 1;
     j.flood_class = flood_class_of(p);
+    const int nd = 25 * p.chunks_x * p.chunks_z;
+    j.stage2_fma = nd <= kNdBucket[0] && tuning.stage2_fma != 0 ? 1 : 0;
     h_jobs_[slot] = j;
     HostJob& h = host_jobs_[slot];
     h = HostJob{};
@@ -183,7 +187,8 @@ int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64
     h.count = count;
     h.worlds_per_wave = j.worlds_per_wave;
     h.upload = true;
-    h.nd = 25 * p.chunks_x * p.chunks_z;
+    h.nd = nd;
+    h.s2 = j.stage2_fma ? kS2FmaSmall : (nd <= kNdBucket[0] ? kS2SelectSmall : kS2SelectLarge);
     return slot;
 }
 
@@ -255,7 +260,7 @@ EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
     return a;
 }
 
-void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int buckets) {
+void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int s2_variants) {
     const EngineArgs a = args_for(ab, stage1_only);
     hipStream_t s = stream_;
     const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
@@ -291,13 +296,11 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int buckets) {
     }
     if (stage1_only) return;
     {
-        // one Stage-2 kernel per lattice layout present in the pass (bit b of buckets)
+        // one Stage-2 kernel per form present in the pass (bit v of s2_variants)
         LaunchTimer t("island_stage2", s, timers_);
-        for (int b = 0; b < 2; ++b) {
-            if (!((buckets >> b) & 1)) continue;
-            auto s2 = b == 0 ? pass_stage2_kernel<true> : pass_stage2_kernel<false>;
-            hipLaunchKernelGGL(s2, dim3(grid_s2_[b]), dim3(64), static_cast<size_t>(2 * kNdBucket[b]) * sizeof(double),
-                               s, a);
+        for (int v = 0; v < kS2Variants; ++v) {
+            if (!((s2_variants >> v) & 1)) continue;
+            hipLaunchKernelGGL(stage2_kernel(v), dim3(grid_s2_[v]), dim3(64), stage2_lds(v), s, a);
             check(hipGetLastError(), "stage 2 launch");
         }
         t.stop(0);
@@ -337,7 +340,7 @@ void IslandEngine::launch(bool stage1_only) {
     f.seq = ++seq_;
     f.stage1_only = stage1_only;
     // upload admitted jobs, then plan the pass exactly as pass_build will
-    int buckets = 0;
+    int variants = 0;
     for (int i = 0; i < cfg_jobs_; ++i) {
         HostJob& h = host_jobs_[i];
         if (h.used && h.upload) {
@@ -346,16 +349,18 @@ void IslandEngine::launch(bool stage1_only) {
                   "H2D job entry");
             h.upload = false;
         }
-        if (h.used && h.cursor < h.count) buckets |= 1 << bucket_of(h.nd);
+        if (h.used && h.cursor < h.count) variants |= 1 << h.s2;
     }
     f.seeds = plan_pass(f.cursors, f.finished, f.seeds_kind);
-    if (buckets == 0) buckets = 1;
+    // the case-selection kernel with the large layout's LDS serves small windows too
+    if (variants & (1 << kS2SelectLarge)) variants &= ~(1 << kS2SelectSmall);
+    if (variants == 0) variants = 1 << kS2FmaSmall;
     if (timers_ && timing_enabled()) {
-        enqueue_kernels(ab, stage1_only, buckets);
+        enqueue_kernels(ab, stage1_only, variants);
     } else if (!use_graphs_) {
-        enqueue_kernels(ab, stage1_only, buckets);
+        enqueue_kernels(ab, stage1_only, variants);
     } else {
-        const int key = ab | (stage1_only ? 2 : 0) | (buckets << 2);
+        const int key = ab | (stage1_only ? 2 : 0) | (variants << 2);
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
             const bool saved = timers_;
@@ -363,7 +368,7 @@ void IslandEngine::launch(bool stage1_only) {
             hipGraph_t g = nullptr;
             check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_kernels(ab, stage1_only, buckets);
+                enqueue_kernels(ab, stage1_only, variants);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &g);
                 if (g) (void)hipGraphDestroy(g);

@@ -31,6 +31,7 @@ struct HostJob {
     unsigned long long hits = 0;     // device hit counter at the last completed pass
     bool upload = false;             // entry still to be copied to the device
     int nd = 0;                      // density columns of the window
+    int s2 = kS2FmaSmall;            // its Stage-2 launch variant (kS2*)
 };
 
 struct EngineStats {
@@ -41,6 +42,7 @@ struct EngineStats {
 struct Tuning {
     int worlds_per_wave = 1;
     int gate = 1;
+    int stage2_fma = -1;  // 1: FMA gradients, 0: case selection, -1: FMA for windows up to 4x4 chunks
 };
 
 class IslandEngine {
@@ -52,6 +54,7 @@ class IslandEngine {
 
     int64_t pass_slots() const { return cfg_slots_; }
     int max_jobs() const { return cfg_jobs_; }
+    int min_quantum() const { return cfg_quantum_; }
     int free_slots() const;
 
     // Puts a job into a free job-table slot (returns the slot, -1 when the table is full).
@@ -97,7 +100,7 @@ class IslandEngine {
         std::vector<std::pair<int, int64_t>> cursors;  // (slot, cursor after this pass)
         std::vector<int> finished;
     };
-    void enqueue_kernels(int ab, bool stage1_only, int buckets);
+    void enqueue_kernels(int ab, bool stage1_only, int s2_variants);
     void drop_graphs();
     EngineArgs args_for(int ab, bool stage1_only) const;
     int64_t plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
@@ -124,10 +127,10 @@ class IslandEngine {
     std::vector<HostJob> host_jobs_;
     int host_rr_ = 0;
     uint64_t seq_ = 0;
-    int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[2] = {0, 0}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
+    int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[kS2Variants] = {0, 0, 0}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
     int64_t flood_scratch_words_ = 0;
     int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | buckets << 2
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | Stage-2 variants << 2
     static constexpr int64_t kHitPrefix = 4096;  // hits copied back with every pass
 };
 

@@ -21,6 +21,10 @@ struct HipError : std::runtime_error {
 struct CapacityError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// Two exact forms of the same computation gave different results: a defect, never expected.
+struct ParityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 void check(hipError_t e, const char* what);
 void set_error(const std::string& m);
@@ -105,6 +109,9 @@ int guarded_call(F&& f) {
     } catch (const CapacityError& e) {
         set_error(e.what());
         return BSF_ERR_CAPACITY;
+    } catch (const ParityError& e) {
+        set_error(e.what());
+        return BSF_ERR_PARITY;
     } catch (const HipError& e) {
         set_error(e.what());
         return BSF_ERR_HIP;

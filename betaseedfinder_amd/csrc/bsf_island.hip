@@ -502,9 +502,9 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
     if constexpr (FMA) init_gcoef(st.gcoef);
     unsigned long long nsurv = a.ctr[kCtrSurv];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
-    // the FMA form serves the small layout (windows up to 4x4 chunks), the other the large one;
-    // when a pass holds both, both kernels run and each takes the survivors of its layout
-    const int queue = FMA ? kCtrStage2 : kCtrStage2Wide;
+    // each job names its form (the FMA form only for windows up to 4x4 chunks); when a pass holds
+    // both, both kernels run and each takes the survivors of its own form
+    const int queue = FMA ? kCtrStage2 : kCtrStage2Select;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&a.ctr[queue], 1ULL);
         __syncthreads();
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         const bsf_island_params p = a.jobs[jix].p;
         const int cx_n = p.chunks_x, cz_n = p.chunks_z;
         const int nd = 25 * cx_n * cz_n;
-        if ((nd <= kNdBucket[0]) != FMA) continue;
+        if ((a.jobs[jix].stage2_fma != 0) != FMA) continue;
         double* s_temp = s_clim;
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;

@@ -36,7 +36,8 @@ struct IslandJob {
     int32_t gate;              // 1: Stage-1 candidate gate; 0: every seed goes to Stage 2
     int32_t flood_class;       // kFloodClass* below
     int32_t seed_source;       // kSeed*
-    int32_t pad;
+    int32_t stage2_fma;        // 1: Stage-2 corner gradients from the coefficient table (windows up to 4x4
+                               // chunks); 0: Java's case selection
     unsigned long long sequence_seed;  // kSeedSplitMix: splitmix_seed(sequence_seed, first_seed + i)
 };
 static_assert(sizeof(IslandJob) == 120, "job-table entry size (server state memory is counted in entries)");
@@ -72,13 +73,17 @@ enum {
     kCtrStage1Multi = 13,
     kCtrOpen = 14,       // open columns left by Stage 1a
     kCtrStage1b = 15,
-    kCtrStage2Wide = 16, // Stage-2 queue head of the survivors of windows above 4x4 chunks
+    kCtrStage2Select = 16, // Stage-2 queue head of the case-selection kernel (kCtrStage2: FMA form)
     kCounters = 17
 };
 
 // Density columns per survivor of the two Stage-2 lattice layouts (windows up to 4x4 chunks, up
-// to 8x8 chunks); each layout has its own Stage-2 kernel and queue.
+// to 8x8 chunks).
 constexpr int kNdBucket[2] = {400, 1600};
+// Stage-2 launches: the FMA-gradient kernel (small layout only) and the case-selection kernel
+// with the small or the large layout's LDS.  A pass launches one per form present among its
+// jobs; each takes the survivors of its form from its own queue.
+enum { kS2FmaSmall = 0, kS2SelectSmall = 1, kS2SelectLarge = 2, kS2Variants = 3 };
 
 // Flood classes by window size (block bitsets of one window: rows x 64-bit z words).
 enum {

@@ -180,43 +180,109 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
         if (trial_seeds < 64 || !out) throw ArgError("bad autotune arguments");
         ensure_engine();
         const int ncols = 4 * p->chunks_x * p->chunks_z;
+        const bool small = 25 * p->chunks_x * p->chunks_z <= kNdBucket[0];
         std::vector<Tuning> cands;
         for (int k : {1, 2, 3, 4, 5, 8, 16})
-            if (k == 1 || (k <= kMaxWorldsPerWave && k * ncols <= 64)) cands.push_back(Tuning{k, 1});
-        cands.push_back(Tuning{64, 0});
+            if (k == 1 || (k <= kMaxWorldsPerWave && k * ncols <= 64)) cands.push_back(Tuning{k, 1, -1});
+        cands.push_back(Tuning{64, 0, -1});
         std::vector<bsf_island_hit> ref;
+        bool have_ref = false;
+        int timed = 0;
+        // the faster of two timed runs (after one untimed run that warms the graphs and caches);
+        // every candidate's hits must equal the first's
+        auto rate_of = [&](const Tuning& t) {
+            run_one(nullptr, first_seed, std::min<int64_t>(trial_seeds, 4096), *p, t, nullptr, nullptr);
+            double best_dt = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                std::vector<bsf_island_hit> hits;
+                check(hipDeviceSynchronize(), "autotune sync");
+                const auto t0 = std::chrono::steady_clock::now();
+                run_one(nullptr, first_seed, trial_seeds, *p, t, &hits, nullptr);
+                best_dt = std::min(best_dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+                std::sort(hits.begin(), hits.end(), hit_less);
+                if (!have_ref) {
+                    ref = hits;
+                    have_ref = true;
+                } else if (hits.size() != ref.size() ||
+                           std::memcmp(hits.data(), ref.data(), hits.size() * sizeof(bsf_island_hit)) != 0) {
+                    throw ParityError("autotune: settings (" + std::to_string(t.worlds_per_wave) + " worlds per wave, gate " +
+                                      std::to_string(t.gate) + ", Stage-2 form " + std::to_string(t.stage2_fma) +
+                                      ") changed the hit list");
+                }
+            }
+            ++timed;
+            return static_cast<double>(trial_seeds) / std::max(best_dt, 1e-9);
+        };
         double best_rate = -1.0;
         Tuning best{};
-        bool first = true;
         for (const Tuning& t : cands) {
-            std::vector<bsf_island_hit> hits;
-            // one untimed run warms the graphs and caches, the second is timed
-            run_one(nullptr, first_seed, std::min<int64_t>(trial_seeds, 4096), *p, t, nullptr, nullptr);
-            check(hipDeviceSynchronize(), "autotune sync");
-            const auto t0 = std::chrono::steady_clock::now();
-            run_one(nullptr, first_seed, trial_seeds, *p, t, &hits, nullptr);
-            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            std::sort(hits.begin(), hits.end(), hit_less);
-            if (first) {
-                ref = hits;
-                first = false;
-            } else if (hits.size() != ref.size() ||
-                       std::memcmp(hits.data(), ref.data(), hits.size() * sizeof(bsf_island_hit)) != 0) {
-                throw HipError("autotune: settings (" + std::to_string(t.worlds_per_wave) + ", gate " +
-                               std::to_string(t.gate) + ") changed the hit list");
-            }
-            const double rate = static_cast<double>(trial_seeds) / std::max(dt, 1e-9);
-            if (rate > best_rate) {
-                best_rate = rate;
+            const double r = rate_of(t);
+            if (r > best_rate) {
+                best_rate = r;
                 best = t;
+            }
+        }
+        best.stage2_fma = small ? 1 : 0;
+        if (small) {
+            Tuning alt = best;
+            alt.stage2_fma = 0;
+            const double r = rate_of(alt);
+            if (r > best_rate) {
+                best_rate = r;
+                best = alt;
             }
         }
         set_tuning(*p, best);
         out->worlds_per_wave = best.worlds_per_wave;
         out->gate = best.gate;
         out->seeds_per_sec = best_rate;
-        out->candidates = static_cast<int32_t>(cands.size());
-        out->reserved = 0;
+        out->candidates = timed;
+        out->stage2_fma = best.stage2_fma;
+    });
+}
+
+int bsf_island_autotune_engine(const bsf_island_params* p, int64_t first_seed, int64_t trial_seeds, int64_t* pass_slots,
+                               double* seeds_per_sec) {
+    return guarded_call([&] {
+        require_init();
+        validate_island_params(p);
+        if (trial_seeds < 64) throw ArgError("bad autotune arguments");
+        ensure_engine();
+        IslandEngine& e = engine();
+        const int jobs = e.max_jobs(), quantum = e.min_quantum();
+        const Tuning t = tuning_for(*p);
+        std::vector<bsf_island_hit> ref;
+        bool have_ref = false;
+        double best_rate = -1.0;
+        int64_t best_slots = e.pass_slots();
+        for (int lg = 16; lg <= 19; ++lg) {
+            e.configure(int64_t(1) << lg, jobs, quantum, true);
+            run_one(nullptr, first_seed, std::min<int64_t>(trial_seeds, 4096), *p, t, nullptr, nullptr);
+            double best_dt = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                std::vector<bsf_island_hit> hits;
+                check(hipDeviceSynchronize(), "autotune sync");
+                const auto t0 = std::chrono::steady_clock::now();
+                run_one(nullptr, first_seed, trial_seeds, *p, t, &hits, nullptr);
+                best_dt = std::min(best_dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+                std::sort(hits.begin(), hits.end(), hit_less);
+                if (!have_ref) {
+                    ref = hits;
+                    have_ref = true;
+                } else if (hits.size() != ref.size() ||
+                           std::memcmp(hits.data(), ref.data(), hits.size() * sizeof(bsf_island_hit)) != 0) {
+                    throw ParityError("autotune: passes of 2^" + std::to_string(lg) + " seeds changed the hit list");
+                }
+            }
+            const double rate = static_cast<double>(trial_seeds) / std::max(best_dt, 1e-9);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best_slots = int64_t(1) << lg;
+            }
+        }
+        e.configure(best_slots, jobs, quantum, true);
+        if (pass_slots) *pass_slots = best_slots;
+        if (seeds_per_sec) *seeds_per_sec = best_rate;
     });
 }
 
@@ -228,6 +294,7 @@ int bsf_island_get_tuning(const bsf_island_params* p, bsf_island_tuning* out) {
         std::memset(out, 0, sizeof(*out));
         out->worlds_per_wave = t.worlds_per_wave;
         out->gate = t.gate;
+        out->stage2_fma = 25 * p->chunks_x * p->chunks_z <= kNdBucket[0] && t.stage2_fma != 0 ? 1 : 0;
     });
 }
 
@@ -239,7 +306,8 @@ int bsf_island_set_tuning(const bsf_island_params* p, const bsf_island_tuning* t
         if (t->gate && (t->worlds_per_wave < 1 || t->worlds_per_wave > kMaxWorldsPerWave ||
                         (t->worlds_per_wave > 1 && t->worlds_per_wave * 4 * p->chunks_x * p->chunks_z > 64)))
             throw ArgError("worlds per wave must fit the window's sample columns into 64 lanes");
-        set_tuning(*p, Tuning{t->gate ? t->worlds_per_wave : 64, t->gate});
+        if (t->stage2_fma < -1 || t->stage2_fma > 1) throw ArgError("stage2_fma must be -1, 0 or 1");
+        set_tuning(*p, Tuning{t->gate ? t->worlds_per_wave : 64, t->gate, t->stage2_fma});
     });
 }
 

@@ -24,6 +24,7 @@ extern "C" {
 #define BSF_ERR_HIP 2     /* a HIP runtime call failed */
 #define BSF_ERR_STATE 3   /* library not initialised / wrong device */
 #define BSF_ERR_CAPACITY 4 /* output capacity exceeded (results truncated) */
+#define BSF_ERR_PARITY 5   /* two exact forms of one computation disagreed (autotuner): a defect */
 
 /* ---- lifecycle ---------------------------------------------------------------- */
 /* Select the HIP device and upload constant tables.  Replaces the implicit
@@ -153,19 +154,31 @@ int bsf_island_engine_configure(int64_t pass_slots, int32_t max_jobs, int32_t mi
  * Stage-1 survivors, hits. */
 int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits);
 
-/* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave and
- * whether the Stage-1 gate runs (gate = 0 sends every seed to Stage 2).  bsf_island_autotune
- * times every candidate on trial_seeds seeds from first_seed, checks that all candidates give
- * the same hits, and keeps the fastest for later jobs of that shape. */
+/* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave, whether
+ * the Stage-1 gate runs (gate = 0 sends every seed to Stage 2), and for windows up to 4x4 chunks
+ * Stage 2's corner-gradient form.  bsf_island_autotune times the Stage-1 candidates, then the
+ * other Stage-2 form with the best of them, each over two runs of trial_seeds seeds from
+ * first_seed (the faster run counts), checks that every candidate gives the same hits
+ * (BSF_ERR_PARITY otherwise), and keeps the fastest for later jobs of that shape.  The reference
+ * sizes its batches for throughput against stop latency (SearchSettings.java:25-37); here the
+ * engine's pass size is that knob (bsf_island_autotune_engine). */
 typedef struct bsf_island_tuning {
     int32_t worlds_per_wave;
     int32_t gate;
     double seeds_per_sec;             /* measured rate of the chosen setting (autotune only) */
     int32_t candidates;               /* settings timed (autotune only) */
-    int32_t reserved;
+    int32_t stage2_fma;               /* 1: corner gradients from the coefficient table, 0: Java's case
+                                         selection; windows above 4x4 chunks always select; -1 in
+                                         set_tuning: the default (table form up to 4x4) */
 } bsf_island_tuning;
 int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t trial_seeds,
                         bsf_island_tuning* out);
+/* Engine autotuner: times passes of 2^16, 2^17, 2^18 and 2^19 seeds on trial_seeds seeds of window
+ * p (the tuned settings of its shape; trial_seeds should hold several passes of the largest),
+ * checks that the hits agree, and configures the engine with the fastest (job-table size and
+ * minimum share kept).  *pass_slots and *seeds_per_sec (optional) receive the choice. */
+int bsf_island_autotune_engine(const bsf_island_params* p, int64_t first_seed, int64_t trial_seeds,
+                               int64_t* pass_slots, double* seeds_per_sec);
 int bsf_island_get_tuning(const bsf_island_params* p, bsf_island_tuning* out);
 int bsf_island_set_tuning(const bsf_island_params* p, const bsf_island_tuning* t);
 

@@ -48,12 +48,47 @@ def test_worlds_per_wave_and_gate_split_keep_hits(bsf, oracle):
         bsf.set_tuning(p, 8, 1)
 
 
+def test_stage2_forms_keep_hits(bsf, oracle):
+    """Stage 2's corner gradients from the coefficient table or by Java's case selection, for a
+    window alone and for two windows of one shape sharing passes with different forms."""
+    p = bsf.island_params(**kw(-1, 0, 3, 2))
+    first, count = 61000, 3000
+    ref = oracle.run_island(first, count, oracle.island_params(**kw(-1, 0, 3, 2)), threads=16)
+    assert len(ref) > 0
+    q = bsf.island_params(**kw(2, 2, 2, 2))
+    try:
+        for form in (1, 0):
+            bsf.set_tuning(p, 1, 1, stage2_fma=form)
+            assert bsf.get_tuning(p).stage2_fma == form
+            assert bsf.island_range(first, count, p) == ref, form
+        # one job of each form in the same passes
+        bsf.set_tuning(p, 1, 1, stage2_fma=0)
+        bsf.set_tuning(q, 1, 1, stage2_fma=1)
+        jobs = [(first, count, p), (first + 7, count, q)]
+        hits, _ = bsf.island_batch(jobs)
+        for i, (f0, c0, pp) in enumerate(jobs):
+            assert [h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in bsf.island_range(f0, c0, pp)], i
+    finally:
+        bsf.set_tuning(p, 1, 1, stage2_fma=-1)
+        bsf.set_tuning(q, 1, 1, stage2_fma=-1)
+
+
 def test_autotune_agrees_and_records(bsf):
     p = bsf.island_params(**kw(0, 0, 2, 2))
     t = bsf.autotune(p, 7000, 2048)
-    assert t.candidates >= 3 and t.seeds_per_sec > 0
+    assert t.candidates >= 4 and t.seeds_per_sec > 0 and t.stage2_fma in (0, 1)
     got = bsf.get_tuning(p)
-    assert (got.worlds_per_wave, got.gate) == (t.worlds_per_wave, t.gate)
+    assert (got.worlds_per_wave, got.gate, got.stage2_fma) == (t.worlds_per_wave, t.gate, t.stage2_fma)
+
+
+def test_autotune_engine_picks_a_pass_size(bsf):
+    p = bsf.island_params(**kw(0, 0, 2, 2))
+    try:
+        slots, rate = bsf.autotune_engine(p, 9000, 1 << 20)
+        assert slots in (1 << 16, 1 << 17, 1 << 18, 1 << 19) and rate > 0
+        assert bsf.island_range(9000, 2048, p) == bsf.island_range(9000, 2048, p)
+    finally:
+        bsf.engine_configure()
 
 
 def batch_jobs(bsf):
