@@ -490,7 +490,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--scout-log2", type=int, default=32)
-    ap.add_argument("--cpu-scout-log2", type=int, default=20)
+    ap.add_argument("--cpu-scout-log2", type=int, default=24)
     ap.add_argument("--stage0-batches", type=int, default=4)
     ap.add_argument("--region-cpu-items", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
@@ -623,6 +623,9 @@ def main():
             "stage1_survivor_fraction": round((stats1[2] - stats0[2]) / max(1, stats1[1] - stats0[1]), 5),
             "passes": passes,
             "timed_path": "captured pass graphs, no timers",
+            # the island engine's totals over every pass of this process so far (warm-up, timed and
+            # kernel-timing steps): the units behind a profiler's per-dispatch counter sums
+            "engine_totals": dict(zip(("passes", "seeds", "survivors", "hits"), [int(v) for v in stats2])),
             "kernel_timing": {"steps": args.timing_steps, "ms_per_step": round(eager_elapsed * 1e3 /
                                                                              max(1, args.timing_steps), 3),
                               "path": "kernels launched one by one between HIP events, same workload after "

@@ -45,7 +45,10 @@ T* pinned(size_t n) {
 // The Stage-2 kernel and dynamic LDS (temperature and rain of the layout's density columns) of a
 // launch variant.
 using Stage2Kernel = void (*)(EngineArgs);
-Stage2Kernel stage2_kernel(int v) { return v == kS2FmaSmall ? pass_stage2_kernel<true> : pass_stage2_kernel<false>; }
+Stage2Kernel stage2_kernel(int v, bool filter) {
+    if (v == kS2FmaSmall) return filter ? pass_stage2_kernel<true, true> : pass_stage2_kernel<true, false>;
+    return filter ? pass_stage2_kernel<false, true> : pass_stage2_kernel<false, false>;
+}
 size_t stage2_lds(int v) { return static_cast<size_t>(2 * kNdBucket[v == kS2SelectLarge ? 1 : 0]) * sizeof(double); }
 
 }  // namespace
@@ -131,7 +134,8 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
-    for (int v = 0; v < kS2Variants; ++v) grid_s2_[v] = occupancy_grid(stage2_kernel(v), 64, stage2_lds(v), 0);
+    for (int v = 0; v < kS2Variants; ++v)
+        for (int f = 0; f < 2; ++f) grid_s2_[v][f] = occupancy_grid(stage2_kernel(v, f != 0), 64, stage2_lds(v), 0);
     grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(0), 0);
     grid_flood_[kFloodClassWideX] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(1), 0);
     grid_flood_[kFloodClassWideZ] = occupancy_grid(pass_flood_kernel<2, false>, kFloodThreads, flood_lds_bytes(2), 0);
@@ -296,11 +300,13 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int s2_variants) {
     }
     if (stage1_only) return;
     {
-        // one Stage-2 kernel per form present in the pass (bit v of s2_variants)
+        // one Stage-2 kernel per form present in the pass (bit v of s2_variants), filtering by form
+        // only when there are two
         LaunchTimer t("island_stage2", s, timers_);
+        const bool filter = (s2_variants & (s2_variants - 1)) != 0;
         for (int v = 0; v < kS2Variants; ++v) {
             if (!((s2_variants >> v) & 1)) continue;
-            hipLaunchKernelGGL(stage2_kernel(v), dim3(grid_s2_[v]), dim3(64), stage2_lds(v), s, a);
+            hipLaunchKernelGGL(stage2_kernel(v, filter), dim3(grid_s2_[v][filter]), dim3(64), stage2_lds(v), s, a);
             check(hipGetLastError(), "stage 2 launch");
         }
         t.stop(0);

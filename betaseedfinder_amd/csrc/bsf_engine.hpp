@@ -127,7 +127,7 @@ class IslandEngine {
     std::vector<HostJob> host_jobs_;
     int host_rr_ = 0;
     uint64_t seq_ = 0;
-    int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[kS2Variants] = {0, 0, 0}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
+    int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[kS2Variants][2] = {}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
     int64_t flood_scratch_words_ = 0;
     int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
     std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | Stage-2 variants << 2

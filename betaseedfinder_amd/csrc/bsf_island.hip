@@ -492,7 +492,7 @@ constexpr int kS2Threads = 64;
 
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
-template <bool FMA>
+template <bool FMA, bool FILTER>
 __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a) {
     __shared__ typename OneWorldT<FMA>::Stage st;
     __shared__ unsigned long long s_item;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
     unsigned long long nsurv = a.ctr[kCtrSurv];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
     // each job names its form (the FMA form only for windows up to 4x4 chunks); when a pass holds
-    // both, both kernels run and each takes the survivors of its own form
+    // both, both kernels run with FILTER and each takes the survivors of its own form
     const int queue = FMA ? kCtrStage2 : kCtrStage2Select;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&a.ctr[queue], 1ULL);
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         const bsf_island_params p = a.jobs[jix].p;
         const int cx_n = p.chunks_x, cz_n = p.chunks_z;
         const int nd = 25 * cx_n * cz_n;
-        if ((a.jobs[jix].stage2_fma != 0) != FMA) continue;
+        if (FILTER && (a.jobs[jix].stage2_fma != 0) != FMA) continue;
         double* s_temp = s_clim;
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;
@@ -560,7 +560,9 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
     }
 }
 
-template __global__ void pass_stage2_kernel<true>(EngineArgs);
-template __global__ void pass_stage2_kernel<false>(EngineArgs);
+template __global__ void pass_stage2_kernel<true, false>(EngineArgs);
+template __global__ void pass_stage2_kernel<false, false>(EngineArgs);
+template __global__ void pass_stage2_kernel<true, true>(EngineArgs);
+template __global__ void pass_stage2_kernel<false, true>(EngineArgs);
 
 }  // namespace bsf

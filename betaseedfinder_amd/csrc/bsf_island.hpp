@@ -183,7 +183,8 @@ __global__ void pass_stage1_kernel(EngineArgs a);
 __global__ void pass_stage1b_kernel(EngineArgs a);
 // FMA: corner gradients from an LDS coefficient table (2 KB more LDS per workgroup; used where
 // LDS does not bound the occupancy, the 4 x 4 lattice bucket).
-template <bool FMA>
+// FILTER: the pass holds both Stage-2 forms, so the kernel skips the survivors of the other form.
+template <bool FMA, bool FILTER>
 __global__ void pass_stage2_kernel(EngineArgs a);
 template <int W, bool GLOBAL>
 __global__ void pass_flood_kernel(EngineArgs a, int cls);
