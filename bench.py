@@ -576,10 +576,11 @@ def main():
 
     seeds = args.steps * batch
     value = seeds / elapsed
-    names = ["island_cache", "island_stage1", "island_stage1_multi", "island_stage1b", "island_stage2", "island_flood"]
+    names = ["island_cache", "island_stage1", "island_stage1_multi", "island_stage1b", "island_ground", "island_stage2",
+             "island_flood"]
     ks = {n: kstats(n) for n in names}
     passes = stats1[0] - stats0[0]
-    survivors = stats2[2] - stats1[2]  # Stage-1 survivors of the HIP-event steps
+    stage2_units = stats2[4] - stats1[4]  # ground-gate survivors of the HIP-event steps: Stage 2's items
     # the FP64 kernels and their algorithmic work per launch
     fp64 = {
         "island_stage1": dict(kernel="pass_stage1_kernel<0> (Stage 1a: 64 sample columns per seed, low-frequency "
@@ -588,7 +589,7 @@ def main():
                               source="per-construct model (OPS)"),
         "island_stage2": dict(kernel="pass_stage2_kernel (Stage 2: 17x17 noise lattice + 400 density columns per "
                                      "survivor)",
-                              unit="survivor", ops=STAGE2_FP64_OPS, units=survivors,
+                              unit="survivor", ops=STAGE2_FP64_OPS, units=stage2_units,
                               source="instrumented oracle count (bsfo_stage2_op_count), mean over the first "
                                      "2^16 seeds of the range"),
     }
@@ -621,11 +622,12 @@ def main():
             "min_island_blocks": min_blocks,
             "hits_in_timed_region": hits,
             "stage1_survivor_fraction": round((stats1[2] - stats0[2]) / max(1, stats1[1] - stats0[1]), 5),
+            "ground_gate_pass_fraction": round((stats1[4] - stats0[4]) / max(1, stats1[2] - stats0[2]), 5),
             "passes": passes,
             "timed_path": "captured pass graphs, no timers",
             # the island engine's totals over every pass of this process so far (warm-up, timed and
             # kernel-timing steps): the units behind a profiler's per-dispatch counter sums
-            "engine_totals": dict(zip(("passes", "seeds", "survivors", "hits"), [int(v) for v in stats2])),
+            "engine_totals": dict(zip(("passes", "seeds", "survivors", "hits", "stage2"), [int(v) for v in stats2])),
             "kernel_timing": {"steps": args.timing_steps, "ms_per_step": round(eager_elapsed * 1e3 /
                                                                              max(1, args.timing_steps), 3),
                               "path": "kernels launched one by one between HIP events, same workload after "

@@ -170,6 +170,17 @@ def island_stage1_range(first_seed, count, params, cap=1 << 24):
     return out[:n.value].copy()
 
 
+def island_ground_range(first_seed, count, params, cap=1 << 24):
+    """Seeds that pass the ground gate (Stage-1 survivors with a start not grounded within its
+    own chunk), sorted."""
+    init()
+    out = np.empty(cap, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    _check(lib().bsf_island_ground_range(ctypes.c_int64(first_seed), ctypes.c_int64(count), ctypes.byref(params),
+                                         out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(cap), ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
 def p20_counts(seeds):
     init()
     s = _seeds(seeds)
@@ -367,9 +378,10 @@ def engine_configure(pass_slots=1 << 18, max_jobs=1024, min_quantum=1024):
 
 
 def engine_stats():
-    """(passes, seeds, survivors, hits) of the library's island engine."""
+    """(passes, seeds, survivors, hits, stage2) of the library's island engine: Stage-1 survivors,
+    and the survivors that the ground gate passed to Stage 2."""
     init()
-    v = [ctypes.c_uint64(0) for _ in range(4)]
+    v = [ctypes.c_uint64(0) for _ in range(5)]
     _check(lib().bsf_island_engine_stats(*[ctypes.byref(x) for x in v]))
     return tuple(x.value for x in v)
 
@@ -389,7 +401,7 @@ def get_tuning(params):
     return t
 
 
-def set_tuning(params, worlds_per_wave, gate=1, stage2_fma=-1):
+def set_tuning(params, worlds_per_wave, gate=2, stage2_fma=-1):
     init()
     t = IslandTuning(worlds_per_wave, gate, 0.0, 0, stage2_fma)
     _check(lib().bsf_island_set_tuning(ctypes.byref(params), ctypes.byref(t)))

@@ -1,7 +1,7 @@
 // bsf_engine.cpp -- the island search engine's host side (see bsf_engine.hpp).
 //
 // A pass is launched as one captured hipGraph (or eagerly, when launch timing is on):
-//   reset -> build -> cache -> Stage 1 -> Stage 2 -> flood x 4
+//   reset -> build -> cache -> Stage 1 -> ground gate -> Stage 2 -> flood x 4
 // followed by asynchronous copies of its counters, the job table and the first hits into
 // pinned memory.  Two result buffer sets alternate, so pass k+1 runs while the host reads
 // pass k.
@@ -57,7 +57,7 @@ void IslandEngine::release() {
     drop_graphs();
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &perms_, &offs_, &surv_slot_, &surv_seed_, &surv_job_, &lattice_,
-                            &flood_list_, &flood_scratch_, &open_, &surv_flag_, &ctr_[0], &ctr_[1], &hits_[0],
+                            &flood_list_, &flood_scratch_, &open_, &surv_mask_, &s2_list_, &ctr_[0], &ctr_[1], &hits_[0],
                             &hits_[1]})
         b->release();
     auto free_pinned = [](void* p) {
@@ -70,10 +70,12 @@ void IslandEngine::release() {
         free_pinned(h_ctr_[i]);
         free_pinned(h_hits_[i]);
         free_pinned(h_stage1_[i]);
+        free_pinned(h_list_[i]);
         h_snap_[i] = nullptr;
         h_ctr_[i] = nullptr;
         h_hits_[i] = nullptr;
         h_stage1_[i] = nullptr;
+        h_list_[i] = nullptr;
         if (done_[i]) (void)hipEventDestroy(done_[i]);
         done_[i] = nullptr;
     }
@@ -113,7 +115,8 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     // every sample column of every one-world seed can be open: 4 columns per chunk, 64 chunks
     open_cap_ = static_cast<int64_t>(P) * 256;
     open_.ensure(static_cast<size_t>(open_cap_) * sizeof(OpenColumn));
-    surv_flag_.ensure(P * sizeof(int32_t));
+    surv_mask_.ensure(P * sizeof(unsigned long long));
+    s2_list_.ensure(P * sizeof(int32_t));
     for (int i = 0; i < 2; ++i) {
         ctr_[i].ensure(kCounters * sizeof(unsigned long long));
         hits_[i].ensure(P * sizeof(bsf_island_hit));
@@ -122,6 +125,7 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
         h_ctr_[i] = pinned<unsigned long long>(kCounters);
         h_hits_[i] = pinned<bsf_island_hit>(P);
         h_stage1_[i] = pinned<int64_t>(P);
+        h_list_[i] = pinned<int32_t>(P);
         check(hipEventCreateWithFlags(&done_[i], hipEventDisableTiming), "hipEventCreate");
     }
     h_jobs_ = pinned<IslandJob>(static_cast<size_t>(max_jobs));
@@ -134,6 +138,7 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
+    grid_ground_ = occupancy_grid(pass_ground_kernel, 64, 0, 0);
     for (int v = 0; v < kS2Variants; ++v)
         for (int f = 0; f < 2; ++f) grid_s2_[v][f] = occupancy_grid(stage2_kernel(v, f != 0), 64, stage2_lds(v), 0);
     grid_flood_[kFloodClassSmall] = occupancy_grid(pass_flood_kernel<1, false>, kFloodThreads, flood_lds_bytes(0), 0);
@@ -237,7 +242,7 @@ int64_t IslandEngine::plan_pass(std::vector<std::pair<int, int64_t>>& cursors, s
     return seeds;
 }
 
-EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
+EngineArgs IslandEngine::args_for(int ab, int audit) const {
     EngineArgs a{};
     a.jobs = static_cast<IslandJob*>(jobs_.ptr);
     a.max_jobs = cfg_jobs_;
@@ -259,13 +264,14 @@ EngineArgs IslandEngine::args_for(int ab, bool stage1_only) const {
     a.flood_scratch_words = flood_scratch_words_;
     a.open = static_cast<OpenColumn*>(open_.ptr);
     a.open_cap = open_cap_;
-    a.surv_flag = static_cast<int32_t*>(surv_flag_.ptr);
-    a.stage1_only = stage1_only ? 1 : 0;
+    a.surv_mask = static_cast<unsigned long long*>(surv_mask_.ptr);
+    a.s2_list = static_cast<int32_t*>(s2_list_.ptr);
+    a.audit = audit;
     return a;
 }
 
-void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int s2_variants) {
-    const EngineArgs a = args_for(ab, stage1_only);
+void IslandEngine::enqueue_kernels(int ab, int audit, int s2_variants) {
+    const EngineArgs a = args_for(ab, audit);
     hipStream_t s = stream_;
     const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
     hipLaunchKernelGGL(pass_reset_kernel, dim3(256), dim3(256), 0, s, a);
@@ -298,7 +304,15 @@ void IslandEngine::enqueue_kernels(int ab, bool stage1_only, int s2_variants) {
         check(hipGetLastError(), "stage 1b launch");
         t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[0]));
     }
-    if (stage1_only) return;
+    if (audit == kAuditStage1) return;
+    {
+        // units: the pass's Stage-1 survivors
+        LaunchTimer t("island_ground", s, timers_);
+        hipLaunchKernelGGL(pass_ground_kernel, dim3(grid_ground_), dim3(64), 0, s, a);
+        check(hipGetLastError(), "ground gate launch");
+        t.stop(0);
+    }
+    if (audit == kAuditGround) return;
     {
         // one Stage-2 kernel per form present in the pass (bit v of s2_variants), filtering by form
         // only when there are two
@@ -336,7 +350,7 @@ void IslandEngine::drop_graphs() {
     graphs_.clear();
 }
 
-void IslandEngine::launch(bool stage1_only) {
+void IslandEngine::launch(int audit) {
     if (!configured()) throw ArgError("island engine not configured");
     if (inflight_n_ >= 2) throw ArgError("two passes already in flight");
     const int ab = next_ab_;
@@ -344,7 +358,7 @@ void IslandEngine::launch(bool stage1_only) {
     Flight& f = flight_[ab];
     f = Flight{};
     f.seq = ++seq_;
-    f.stage1_only = stage1_only;
+    f.audit = audit;
     // upload admitted jobs, then plan the pass exactly as pass_build will
     int variants = 0;
     for (int i = 0; i < cfg_jobs_; ++i) {
@@ -362,11 +376,11 @@ void IslandEngine::launch(bool stage1_only) {
     if (variants & (1 << kS2SelectLarge)) variants &= ~(1 << kS2SelectSmall);
     if (variants == 0) variants = 1 << kS2FmaSmall;
     if (timers_ && timing_enabled()) {
-        enqueue_kernels(ab, stage1_only, variants);
+        enqueue_kernels(ab, audit, variants);
     } else if (!use_graphs_) {
-        enqueue_kernels(ab, stage1_only, variants);
+        enqueue_kernels(ab, audit, variants);
     } else {
-        const int key = ab | (stage1_only ? 2 : 0) | (variants << 2);
+        const int key = ab | (audit << 1) | (variants << 3);
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
             const bool saved = timers_;
@@ -374,7 +388,7 @@ void IslandEngine::launch(bool stage1_only) {
             hipGraph_t g = nullptr;
             check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_kernels(ab, stage1_only, variants);
+                enqueue_kernels(ab, audit, variants);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -398,10 +412,14 @@ void IslandEngine::launch(bool stage1_only) {
     check(hipMemcpyAsync(h_hits_[ab], hits_[ab].ptr, static_cast<size_t>(prefix) * sizeof(bsf_island_hit),
                          hipMemcpyDeviceToHost, s),
           "D2H hits");
-    if (stage1_only)
+    if (audit != kAuditNone)
         check(hipMemcpyAsync(h_stage1_[ab], surv_seed_.ptr, static_cast<size_t>(cfg_slots_) * sizeof(int64_t),
                              hipMemcpyDeviceToHost, s),
               "D2H stage 1 survivors");
+    if (audit == kAuditGround)
+        check(hipMemcpyAsync(h_list_[ab], s2_list_.ptr, static_cast<size_t>(cfg_slots_) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s),
+              "D2H ground-gate survivors");
     check(hipEventRecord(done_[ab], s), "hipEventRecord");
     next_ab_ ^= 1;
     ++inflight_n_;
@@ -415,6 +433,7 @@ void IslandEngine::complete_oldest(PassResult& out) {
     const unsigned long long* c = h_ctr_[ab];
     const int64_t nh = std::min<int64_t>(static_cast<int64_t>(c[kCtrHits]), cfg_slots_);
     const int64_t ns = std::min<int64_t>(static_cast<int64_t>(c[kCtrSurv]), cfg_slots_);
+    const int64_t nk = f.audit == kAuditStage1 ? 0 : std::min<int64_t>(static_cast<int64_t>(c[kCtrKept]), cfg_slots_);
     const int64_t prefix = std::min(kHitPrefix, cfg_slots_);
     if (nh > prefix) {
         // the device buffer of this pass is not reused before the pass after next
@@ -428,8 +447,11 @@ void IslandEngine::complete_oldest(PassResult& out) {
     out.seq = f.seq;
     out.seeds = f.seeds;
     out.survivors = ns;
+    out.stage2 = nk;
     out.hits.assign(h_hits_[ab], h_hits_[ab] + nh);
-    if (f.stage1_only) out.stage1.assign(h_stage1_[ab], h_stage1_[ab] + ns);
+    if (f.audit == kAuditStage1) out.stage1.assign(h_stage1_[ab], h_stage1_[ab] + ns);
+    if (f.audit == kAuditGround)
+        for (int64_t q = 0; q < nk; ++q) out.stage1.push_back(h_stage1_[ab][h_list_[ab][q]]);
     // the device advanced the same cursors as the host's replica of pass_build
     for (const auto& sc : f.cursors) {
         const IslandJob& dj = h_snap_[ab][sc.first];
@@ -444,15 +466,16 @@ void IslandEngine::complete_oldest(PassResult& out) {
     stats.passes += 1;
     stats.seeds += static_cast<uint64_t>(f.seeds);
     stats.survivors += static_cast<uint64_t>(ns);
+    stats.stage2 += static_cast<uint64_t>(nk);
     stats.hits += static_cast<uint64_t>(nh);
     oldest_ab_ ^= 1;
     --inflight_n_;
 }
 
-void IslandEngine::drive(bool stage1_only, const std::function<void(PassResult&)>& on_pass) {
+void IslandEngine::drive(int audit, const std::function<void(PassResult&)>& on_pass) {
     PassResult r;
     for (;;) {
-        while (inflight_n_ < 2 && has_work()) launch(stage1_only);
+        while (inflight_n_ < 2 && has_work()) launch(audit);
         if (inflight_n_ == 0) break;
         complete_oldest(r);
         on_pass(r);
@@ -476,7 +499,7 @@ Tuning tuning_for(const bsf_island_params& p) {
     if (it != g_tuning.end()) return it->second;
     Tuning t;
     t.worlds_per_wave = default_worlds_per_wave(p);
-    t.gate = 1;
+    t.gate = kGateGround;
     return t;
 }
 

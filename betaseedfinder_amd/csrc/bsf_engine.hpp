@@ -36,12 +36,13 @@ struct HostJob {
 
 struct EngineStats {
     uint64_t passes = 0, seeds = 0, survivors = 0, hits = 0;
+    uint64_t stage2 = 0;  // survivors that the ground gate passed to Stage 2
 };
 
 // Tuned per-window settings (keyed by chunks_x, chunks_z).
 struct Tuning {
     int worlds_per_wave = 1;
-    int gate = 1;
+    int gate = kGateGround;  // kGate*
     int stage2_fma = -1;  // 1: FMA gradients, 0: case selection, -1: FMA for windows up to 4x4 chunks
 };
 
@@ -72,17 +73,18 @@ class IslandEngine {
     struct PassResult {
         uint64_t seq = 0;
         std::vector<bsf_island_hit> hits;        // hits of the pass (window_index = job tag)
-        std::vector<int64_t> stage1;             // Stage-1 survivor seeds (stage1_only passes)
+        std::vector<int64_t> stage1;             // audited survivor seeds (audit passes)
         std::vector<int> finished;               // slots whose job completed with this pass
-        int64_t seeds = 0, survivors = 0;
+        int64_t seeds = 0, survivors = 0, stage2 = 0;
     };
 
     // Runs passes until every resident job is done; `on_pass` sees each result in order.
-    void drive(bool stage1_only, const std::function<void(PassResult&)>& on_pass);
+    // audit (kAudit*): passes that stop after Stage 1 or the ground gate and return its survivors.
+    void drive(int audit, const std::function<void(PassResult&)>& on_pass);
 
     // Pipelined primitives used by drive() and by the server loop: at most two passes in
     // flight, each with its own result buffers.
-    void launch(bool stage1_only);
+    void launch(int audit);
     int inflight() const { return inflight_n_; }
     void complete_oldest(PassResult& out);
 
@@ -94,15 +96,15 @@ class IslandEngine {
    private:
     struct Flight {
         uint64_t seq = 0;
-        bool stage1_only = false;
+        int audit = kAuditNone;
         int64_t seeds = 0;
         int64_t seeds_kind[2] = {0, 0};                // seeds by Stage-1 kind
         std::vector<std::pair<int, int64_t>> cursors;  // (slot, cursor after this pass)
         std::vector<int> finished;
     };
-    void enqueue_kernels(int ab, bool stage1_only, int s2_variants);
+    void enqueue_kernels(int ab, int audit, int s2_variants);
     void drop_graphs();
-    EngineArgs args_for(int ab, bool stage1_only) const;
+    EngineArgs args_for(int ab, int audit) const;
     int64_t plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
                       int64_t seeds_kind[2]);
 
@@ -112,9 +114,9 @@ class IslandEngine {
     bool timers_ = false, use_graphs_ = true;
     hipStream_t stream_ = nullptr, copy_stream_ = nullptr;
     DeviceBuffer jobs_, segs_, desc_, perms_, offs_, surv_slot_, surv_seed_, surv_job_, lattice_, flood_list_,
-            flood_scratch_, open_, surv_flag_;
+            flood_scratch_, open_, surv_mask_, s2_list_;
     int64_t open_cap_ = 0;
-    int grid_s1b_ = 0;
+    int grid_s1b_ = 0, grid_ground_ = 0;
     DeviceBuffer ctr_[2], hits_[2];
     // pinned host memory
     IslandJob* h_jobs_ = nullptr;
@@ -122,6 +124,7 @@ class IslandEngine {
     unsigned long long* h_ctr_[2] = {nullptr, nullptr};
     bsf_island_hit* h_hits_[2] = {nullptr, nullptr};
     int64_t* h_stage1_[2] = {nullptr, nullptr};
+    int32_t* h_list_[2] = {nullptr, nullptr};    // s2_list of ground-gate audits
     hipEvent_t done_[2] = {nullptr, nullptr};
     Flight flight_[2];
     std::vector<HostJob> host_jobs_;
@@ -130,7 +133,7 @@ class IslandEngine {
     int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[kS2Variants][2] = {}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
     int64_t flood_scratch_words_ = 0;
     int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | stage1_only << 1 | Stage-2 variants << 2
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | audit << 1 | Stage-2 variants << 3
     static constexpr int64_t kHitPrefix = 4096;  // hits copied back with every pass
 };
 

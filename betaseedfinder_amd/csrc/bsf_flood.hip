@@ -101,14 +101,8 @@ __device__ __forceinline__ Z<W> zfill(const Z<W>& s, const Z<W>& m) {
     return r;
 }
 
-// Lattice values are below 2^11 in magnitude, so the upsampler's rounding stays below 1e-12.
-constexpr double kSignMargin = 1e-9;
-
 // Word wi of the eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks of chunks
-// 4 wi .. 4 wi + 3 in z) from one lattice y-cell, walking generateBareTerrain's incremental
-// chain (BetaTerrain173.java:599-664) once: d1..d4 step by d5..d8 per level, d10/d11 step i2
-// times by d12/d13 along x, d15 steps by d16 along z.  The i2 steps are all formed and the
-// right one selected, so lanes with different x stay converged.
+// 4 wi .. 4 wi + 3 in z) from one lattice y-cell per chunk (cell_stone_bits).
 __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, int wi,
                                 unsigned long long rows[8]) {
     const int cxi = x >> 4, lx = x & 15;
@@ -122,47 +116,13 @@ __device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, i
             auto L = [&](int ii, int jj, int kk) -> double {
                 return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
             };
-            double d1 = L(i1, j1, k1), d2 = L(i1, j1 + 1, k1), d3 = L(i1 + 1, j1, k1), d4 = L(i1 + 1, j1 + 1, k1);
-            const double e1 = L(i1, j1, k1 + 1), e2 = L(i1, j1 + 1, k1 + 1), e3 = L(i1 + 1, j1, k1 + 1),
-                         e4 = L(i1 + 1, j1 + 1, k1 + 1);
+            const uint32_t m = cell_stone_bits(L(i1, j1, k1), L(i1, j1 + 1, k1), L(i1 + 1, j1, k1), L(i1 + 1, j1 + 1, k1),
+                                               L(i1, j1, k1 + 1), L(i1, j1 + 1, k1 + 1), L(i1 + 1, j1, k1 + 1),
+                                               L(i1 + 1, j1 + 1, k1 + 1), i2);
+            if (m == 0) continue;
             const int sh = (czi - 4 * wi) * 16 + j1 * 4;
-            // every block of the cell is a convex combination of its 8 corners (up to a few ulps),
-            // so a cell whose corners are all clearly negative is air and all clearly positive
-            // is stone
-            const double mx = fmax(fmax(fmax(d1, d2), fmax(d3, d4)), fmax(fmax(e1, e2), fmax(e3, e4)));
-            const double mn = fmin(fmin(fmin(d1, d2), fmin(d3, d4)), fmin(fmin(e1, e2), fmin(e3, e4)));
-            if (mx <= -kSignMargin) continue;
-            if (mn > kSignMargin) {
 #pragma unroll
-                for (int l1 = 0; l1 < 8; ++l1) rows[l1] |= 0xFULL << sh;
-                continue;
-            }
-            const double d5 = (e1 - d1) * 0.125;
-            const double d6 = (e2 - d2) * 0.125;
-            const double d7 = (e3 - d3) * 0.125;
-            const double d8 = (e4 - d4) * 0.125;
-#pragma unroll
-            for (int l1 = 0; l1 < 8; ++l1) {
-                const double d12 = (d3 - d1) * 0.25;
-                const double d13 = (d4 - d2) * 0.25;
-                const double a1 = d1 + d12, a2 = a1 + d12, a3 = a2 + d12;
-                const double b1 = d2 + d13, b2 = b1 + d13, b3 = b2 + d13;
-                const double d10 = i2 == 0 ? d1 : (i2 == 1 ? a1 : (i2 == 2 ? a2 : a3));
-                const double d11 = i2 == 0 ? d2 : (i2 == 1 ? b1 : (i2 == 2 ? b2 : b3));
-                const double d16 = (d11 - d10) * 0.25;
-                double d15 = d10;
-                unsigned long long bits = 0;
-#pragma unroll
-                for (int k2 = 0; k2 < 4; ++k2) {
-                    if (d15 > 0.0) bits |= 1ULL << k2;
-                    d15 += d16;
-                }
-                rows[l1] |= bits << sh;
-                d1 += d5;
-                d2 += d6;
-                d3 += d7;
-                d4 += d8;
-            }
+            for (int l1 = 0; l1 < 8; ++l1) rows[l1] |= static_cast<unsigned long long>((m >> (4 * l1)) & 0xFu) << sh;
         }
     }
 }
@@ -600,7 +560,7 @@ __global__ __launch_bounds__(kFloodThreads) void pass_flood_kernel(EngineArgs a,
     __shared__ FloodShared<W> sh;
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
-    if (a.stage1_only) return;
+    if (a.audit != kAuditNone) return;
     unsigned long long* const base =
             GLOBAL ? a.flood_scratch + static_cast<size_t>(blockIdx.x) * a.flood_scratch_words : flood_lds;
     const unsigned long long n_items = a.ctr[kCtrFloodCount + cls];

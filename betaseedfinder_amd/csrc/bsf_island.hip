@@ -26,12 +26,12 @@ hipError_t upload_island_tables() {
     return hipMemcpyToSymbol(HIP_SYMBOL(isl_magic), t, sizeof(t));
 }
 
-// Clears the pass counters and the survivor flags (a fill kernel, so that the pass can be
-// captured as a graph).
+// Clears the pass counters and the survivors' start-chunk masks (a fill kernel, so that the pass
+// can be captured as a graph).
 __global__ void pass_reset_kernel(EngineArgs a) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < kCounters) a.ctr[t] = 0;
-    for (int64_t i = t; i < a.pass_slots; i += static_cast<int64_t>(gridDim.x) * blockDim.x) a.surv_flag[i] = 0;
+    for (int64_t i = t; i < a.pass_slots; i += static_cast<int64_t>(gridDim.x) * blockDim.x) a.surv_mask[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -389,9 +389,14 @@ __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
             const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
             const bool found = stage1_column(s, p.chunk_x0 + ch / cz_n, p.chunk_z0 + ch % cz_n, i, l, active);
             const unsigned long long m = __ballot(found);
-            bool surv = false;
-            if (tid < nk) surv = ((m >> (tid * ncols)) & ((1ULL << ncols) - 1ULL)) != 0;
+            // lane w < nk: world w's sample columns with a start (ncols <= 32), then its chunks
+            unsigned long long cols = 0, chunks = 0;
+            if (tid < nk) cols = (m >> (tid * ncols)) & ((1ULL << ncols) - 1ULL);
+            for (int q = 0; 4 * q < ncols; ++q)
+                if ((cols >> (4 * q)) & 0xFULL) chunks |= 1ULL << q;
+            const bool surv = cols != 0;
             const int64_t slot = slot_first + (tid < nk ? tid : 0);
+            if (surv) a.surv_mask[slot] = chunks;
             append_survivors(a, surv, slot, job_seed(job, g.start + (slot - g.slot0)), g.job);
         }
     }
@@ -474,13 +479,173 @@ __global__ __launch_bounds__(64) void pass_stage1b_kernel(EngineArgs a) {
                 d1 += d5;
             }
         }
-        if (active && found && atomicOr(&a.surv_flag[c.slot], 1) == 0) {
+        // the column's chunk in the window; the first column of a seed to set its mask appends it
+        unsigned long long old = 1;
+        if (active && found) {
+            const bsf_island_params& p = a.jobs[c.job].p;
+            const int ch = ((c.X >> 2) - p.chunk_x0) * p.chunks_z + ((c.Z >> 2) - p.chunk_z0);
+            old = atomicOr(&a.surv_mask[c.slot], 1ULL << ch);
+        }
+        if (old == 0) {
             const unsigned long long k = atomicAdd(&a.ctr[kCtrSurv], 1ULL);
             if (static_cast<int64_t>(k) < a.pass_slots) {
                 a.surv_slot[k] = c.slot;
                 a.surv_seed[k] = c.seed;
                 a.surv_job[k] = c.job;
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ground gate
+// ---------------------------------------------------------------------------
+// Java's scan floods the 6-connected stone component above y = 67 of every candidate start that
+// an earlier flood has not visited; a component with a block at y = 68 over stone at y = 67
+// touches the bottom, and isMatch rejects it (IslandSearchEngine.java:3972-4003, :4170-4175).
+// A stone path inside one chunk of the window is a path of the window, and a bottom contact
+// inside the chunk is a contact of the window, so a start whose component reaches a contact
+// without leaving its chunk is grounded in the window.  The gate rebuilds each chunk holding a
+// start (Stage 1's surv_mask) exactly as the flood will see it -- its 25 density columns with
+// the chunk's own climate samples, the same upsampler (cell_stone_bits) -- removes the stone
+// grounded within the chunk, and passes the survivor to Stage 2 iff some start is left.  A
+// survivor whose every start is grounded has no component that can match, and no hit.
+//
+// One 64-lane wave per survivor.  Chunks go in pairs: lanes 0..24 evaluate the density columns
+// of the first chunk, 25..49 of the second.  For the ground removal lane l owns the four block
+// columns x = l / 4, z = 4 (l % 4) .. 4 (l % 4) + 3 of a chunk as 64-bit masks of y = 64..127.
+constexpr int kGroundThreads = 64;
+
+// Bits of m in a run of consecutive ones of m that holds a bit of s (s within m): the carry of
+// m + s sweeps each run upward from its seed bits, the same sum on the reversed words downward.
+__device__ __forceinline__ unsigned long long run_fill(unsigned long long s, unsigned long long m) {
+    const unsigned long long up = (((m + s) ^ m) & m) | s;
+    const unsigned long long rm = __builtin_bitreverse64(m), rs = __builtin_bitreverse64(s);
+    return up | __builtin_bitreverse64((((rm + rs) ^ rm) & rm) | rs);
+}
+
+// Whether some candidate start of the chunk whose density columns are in dens ([25][8] doubles,
+// levels 8..15, column i * 5 + l) is not grounded within the chunk.  Every lane of the wave calls it.
+__device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
+    const int lane = threadIdx.x;
+    const int x = lane >> 2, j1 = lane & 3;
+    const int i1 = x >> 2, i2 = x & 3;
+    unsigned long long col[4] = {0, 0, 0, 0};  // z = 4 j1 + k2, bit y - 64
+    for (int k1 = 8; k1 < 16; ++k1) {
+        auto L = [&](int ii, int jj, int kk) -> double {
+            return kk >= 16 ? -10.0 : dens[(ii * 5 + jj) * 8 + (kk - 8)];
+        };
+        const uint32_t m = cell_stone_bits(L(i1, j1, k1), L(i1, j1 + 1, k1), L(i1 + 1, j1, k1), L(i1 + 1, j1 + 1, k1),
+                                           L(i1, j1, k1 + 1), L(i1, j1 + 1, k1 + 1), L(i1 + 1, j1, k1 + 1),
+                                           L(i1 + 1, j1 + 1, k1 + 1), i2);
+        if (m == 0) continue;
+        const int sh = (k1 - 8) * 8;
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            unsigned long long c = 0;
+#pragma unroll
+            for (int l1 = 0; l1 < 8; ++l1) c |= static_cast<unsigned long long>((m >> (4 * l1 + k2)) & 1u) << l1;
+            col[k2] |= c << sh;
+        }
+    }
+    // stone above y = 67 (bits 4..63); grounded: the runs over the contacts (y = 68 and 67 stone),
+    // then grown through x / z neighbours and along runs until nothing changes
+    unsigned long long R[4], g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        R[k] = col[k] & ~0xFULL;
+        g[k] = (col[k] & 0x18ULL) == 0x18ULL ? run_fill(0x10ULL, R[k]) : 0ULL;
+    }
+    for (;;) {
+        unsigned long long xm[4], xp[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xm[k] = __shfl(g[k], (lane + 60) & 63);
+            xp[k] = __shfl(g[k], (lane + 4) & 63);
+            if (x == 0) xm[k] = 0;
+            if (x == 15) xp[k] = 0;
+        }
+        unsigned long long zl = __shfl(g[3], (lane + 63) & 63), zr = __shfl(g[0], (lane + 1) & 63);
+        if (j1 == 0) zl = 0;
+        if (j1 == 3) zr = 0;
+        bool grew = false;
+        unsigned long long n[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long nb = g[k] | xm[k] | xp[k] | (k > 0 ? g[k - 1] : zl) | (k < 3 ? g[k + 1] : zr);
+            n[k] = run_fill(nb & R[k], R[k]);
+            grew |= n[k] != g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g[k] = n[k];
+        if (!__any(grew)) break;
+    }
+    // candidate starts: sample columns x, z in {4, 12} (k2 = 0 of lanes with x % 8 == 4, j1 odd),
+    // stone over non-stone at y >= 68
+    const bool sample = (x & 7) == 4 && (j1 & 1) == 1;
+    const unsigned long long starts = R[0] & ~(col[0] << 1);
+    return __any(sample && (starts & ~g[0]) != 0);
+}
+
+__global__ __launch_bounds__(kGroundThreads, 3) void pass_ground_kernel(EngineArgs a) {
+    __shared__ OneWorldFma::Stage st;
+    __shared__ double s_dens[2][25 * 8];
+    __shared__ unsigned long long s_item;
+    const int tid = threadIdx.x;
+    if (a.audit == kAuditStage1) return;
+    init_gcoef(st.gcoef);
+    unsigned long long nsurv = a.ctr[kCtrSurv];
+    if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.ctr[kCtrGround], 1ULL);
+        __syncthreads();
+        const unsigned long long it = s_item;
+        __syncthreads();
+        if (it >= nsurv) break;
+        const int64_t slot = a.surv_slot[it];
+        const IslandJob& job = a.jobs[a.surv_job[it]];
+        bool keep = true;
+        if (job.gate == kGateGround) {
+            const bsf_island_params p = job.p;
+            const int cz_n = p.chunks_z;
+            const OneWorldFma s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                                a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
+            unsigned long long mask = a.surv_mask[slot];
+            keep = false;
+            while (mask != 0 && !keep) {
+                int ch[2];
+                ch[0] = __ffsll(static_cast<long long>(mask)) - 1;
+                mask &= mask - 1;
+                ch[1] = mask != 0 ? __ffsll(static_cast<long long>(mask)) - 1 : -1;
+                if (ch[1] >= 0) mask &= mask - 1;
+                const int nch = ch[1] >= 0 ? 2 : 1;
+                // density column c of pair member k: lattice (i, l) = (c / 5, c % 5) of chunk ch[k]
+                const int k = tid < 25 ? 0 : 1;
+                const int c = tid - 25 * k;
+                const bool active = tid < 25 * nch;
+                const int chk = active ? ch[k] : ch[0];
+                const int cx = p.chunk_x0 + chk / cz_n, cz = p.chunk_z0 + chk % cz_n;
+                const int i = active ? c / 5 : 0, l = active ? c % 5 : 0;
+                const double BX = static_cast<double>(cx * 16 + 3 * i + 1);
+                const double BZ = static_cast<double>(cz * 16 + 3 * l + 1);
+                double t = 0.0, r = 0.0;
+                colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &t, &r);
+                ColAcc<8, 16> acc;
+                colpar_noise<8, 16, true>(s, static_cast<double>(cx * 4 + i), static_cast<double>(cz * 4 + l), active,
+                                          acc);
+                if (active) {
+                    const ColumnShape cs = column_shape(t, r, acc.n4, acc.n5);
+#pragma unroll
+                    for (int y = 0; y < 8; ++y) s_dens[k][c * 8 + y] = lattice_density(cs, 8 + y, acc.n1[y], acc.n2[y], acc.n3[y]);
+                }
+                __syncthreads();
+                for (int q = 0; q < nch && !keep; ++q) keep = chunk_has_floating_start(s_dens[q]);
+                // the next pair's first stage barrier orders these reads before its writes
+            }
+        }
+        if (keep && tid == 0) {
+            const unsigned long long pos = atomicAdd(&a.ctr[kCtrKept], 1ULL);
+            a.s2_list[pos] = static_cast<int32_t>(it);
         }
     }
 }
@@ -498,9 +663,10 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
-    if (a.stage1_only) return;
+    if (a.audit != kAuditNone) return;
     if constexpr (FMA) init_gcoef(st.gcoef);
-    unsigned long long nsurv = a.ctr[kCtrSurv];
+    // the survivors that the ground gate kept
+    unsigned long long nsurv = a.ctr[kCtrKept];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
     // each job names its form (the FMA form only for windows up to 4x4 chunks); when a pass holds
     // both, both kernels run with FILTER and each takes the survivors of its own form
@@ -508,9 +674,10 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&a.ctr[queue], 1ULL);
         __syncthreads();
-        const unsigned long long it = s_item;
+        const unsigned long long q = s_item;
         __syncthreads();
-        if (it >= nsurv) break;
+        if (q >= nsurv) break;
+        const int32_t it = a.s2_list[q];
         const int64_t slot = a.surv_slot[it];
         const int32_t jix = a.surv_job[it];
         const bsf_island_params p = a.jobs[jix].p;

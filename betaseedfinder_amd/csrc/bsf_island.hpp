@@ -4,7 +4,7 @@
 // from the jobs resident in a device job table:
 //
 //   pass_reset -> pass_build -> pass_cache -> pass_stage1 (1a, multi) -> pass_stage1b
-//     -> pass_stage2 -> pass_flood x 4
+//     -> pass_ground -> pass_stage2 -> pass_flood x 4
 //
 // pass_build hands every active job a share of the pass (its seed cursor lives in the job
 // table and is advanced on the device), so the same launch sequence -- captured once as a
@@ -33,7 +33,7 @@ struct IslandJob {
     bsf_island_params p;
     int32_t tag;               // written into bsf_island_hit::window_index
     int32_t worlds_per_wave;   // Stage-1 worlds per 64-lane wave (1 .. kMaxWorldsPerWave)
-    int32_t gate;              // 1: Stage-1 candidate gate; 0: every seed goes to Stage 2
+    int32_t gate;              // kGate* below
     int32_t flood_class;       // kFloodClass* below
     int32_t seed_source;       // kSeed*
     int32_t stage2_fma;        // 1: Stage-2 corner gradients from the coefficient table (windows up to 4x4
@@ -41,6 +41,14 @@ struct IslandJob {
     unsigned long long sequence_seed;  // kSeedSplitMix: splitmix_seed(sequence_seed, first_seed + i)
 };
 static_assert(sizeof(IslandJob) == 120, "job-table entry size (server state memory is counted in entries)");
+
+// Survivor audits: a pass that stops after Stage 1 or after the ground gate.
+enum : int32_t { kAuditNone = 0, kAuditStage1 = 1, kAuditGround = 2 };
+
+// Gates of a job.  kGateNone: every seed goes to Stage 2.  kGateStage1: the seeds with a
+// candidate start.  kGateGround: of those, the ones with a start that is not grounded within its
+// own chunk (pass_ground_kernel).
+enum : int32_t { kGateNone = 0, kGateStage1 = 1, kGateGround = 2 };
 
 // A contiguous run of one job's seeds inside a pass.
 struct PassSeg {
@@ -74,7 +82,9 @@ enum {
     kCtrOpen = 14,       // open columns left by Stage 1a
     kCtrStage1b = 15,
     kCtrStage2Select = 16, // Stage-2 queue head of the case-selection kernel (kCtrStage2: FMA form)
-    kCounters = 17
+    kCtrGround = 17,     // ground-gate queue head
+    kCtrKept = 18,       // survivors the ground gate passes to Stage 2 (s2_list entries)
+    kCounters = 19
 };
 
 // Density columns per survivor of the two Stage-2 lattice layouts (windows up to 4x4 chunks, up
@@ -116,6 +126,51 @@ struct OpenColumn {
 };
 
 constexpr int kFloorY = 68;
+
+// Lattice values are below 2^11 in magnitude, so the upsampler's rounding stays below 1e-12.
+constexpr double kSignMargin = 1e-9;
+
+// Stone blocks of one lattice y-cell of a chunk for block-column offset i2 along x (0..3):
+// bit 4 l1 + k2 is block (i2, l1, k2) of the cell's 4 x 8 x 4 blocks, from the corners d1..d4
+// (levels k1: (i, j), (i, j+1), (i+1, j), (i+1, j+1)) and e1..e4 (level k1 + 1), walking
+// generateBareTerrain's incremental chain (BetaTerrain173.java:599-664): d1..d4 step by d5..d8
+// per level, d10/d11 step i2 times by d12/d13 along x, d15 steps by d16 along z.  The i2 steps
+// are all formed and the right one selected, so lanes with different x stay converged.  Every
+// block of the cell is a convex combination of its 8 corners (up to a few ulps), so a cell whose
+// corners are all clearly negative is air and all clearly positive is stone.
+__device__ __forceinline__ uint32_t cell_stone_bits(double d1, double d2, double d3, double d4, double e1,
+                                                    double e2, double e3, double e4, int i2) {
+    const double mx = fmax(fmax(fmax(d1, d2), fmax(d3, d4)), fmax(fmax(e1, e2), fmax(e3, e4)));
+    const double mn = fmin(fmin(fmin(d1, d2), fmin(d3, d4)), fmin(fmin(e1, e2), fmin(e3, e4)));
+    if (mx <= -kSignMargin) return 0u;
+    if (mn > kSignMargin) return 0xFFFFFFFFu;
+    const double d5 = (e1 - d1) * 0.125;
+    const double d6 = (e2 - d2) * 0.125;
+    const double d7 = (e3 - d3) * 0.125;
+    const double d8 = (e4 - d4) * 0.125;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int l1 = 0; l1 < 8; ++l1) {
+        const double d12 = (d3 - d1) * 0.25;
+        const double d13 = (d4 - d2) * 0.25;
+        const double a1 = d1 + d12, a2 = a1 + d12, a3 = a2 + d12;
+        const double b1 = d2 + d13, b2 = b1 + d13, b3 = b2 + d13;
+        const double d10 = i2 == 0 ? d1 : (i2 == 1 ? a1 : (i2 == 2 ? a2 : a3));
+        const double d11 = i2 == 0 ? d2 : (i2 == 1 ? b1 : (i2 == 2 ? b2 : b3));
+        const double d16 = (d11 - d10) * 0.25;
+        double d15 = d10;
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            if (d15 > 0.0) bits |= 1u << (4 * l1 + k2);
+            d15 += d16;
+        }
+        d1 += d5;
+        d2 += d6;
+        d3 += d7;
+        d4 += d8;
+    }
+    return bits;
+}
 constexpr int kFloodThreads = 512;  // engine flood workgroup: 8 waves per survivor window
 constexpr int kLevels = 128 - kFloorY;   // 60 block rows per x
 constexpr int kRowStride = kLevels + 1;  // 61: odd stride spreads the x rows over the LDS banks
@@ -146,8 +201,10 @@ struct EngineArgs {
     int64_t flood_scratch_words;
     OpenColumn* open;              // Stage 1a -> 1b
     int64_t open_cap;
-    int32_t* surv_flag;            // [pass_slots] a Stage-1b survivor is appended once
-    int32_t stage1_only;           // 1: stop after Stage 1 (survivor audits)
+    unsigned long long* surv_mask; // [pass_slots] chunks (window index bit) with a candidate start;
+                                   // a Stage-1b survivor is appended by the column that sets the first
+    int32_t* s2_list;              // [pass_slots] survivor indices for Stage 2 (pass_ground_kernel)
+    int32_t audit;                 // kAudit*: stop after Stage 1 or after the ground gate
     int32_t pad;
 };
 
@@ -186,6 +243,7 @@ __global__ void pass_stage1b_kernel(EngineArgs a);
 // FILTER: the pass holds both Stage-2 forms, so the kernel skips the survivors of the other form.
 template <bool FMA, bool FILTER>
 __global__ void pass_stage2_kernel(EngineArgs a);
+__global__ void pass_ground_kernel(EngineArgs a);
 template <int W, bool GLOBAL>
 __global__ void pass_flood_kernel(EngineArgs a, int cls);
 

@@ -30,9 +30,11 @@ void finish_hits(std::vector<bsf_island_hit>& hits, bsf_island_hit* out, int64_t
     if (static_cast<int64_t>(hits.size()) > cap) throw CapacityError("island hit capacity exceeded");
 }
 
-// One job through the engine, run to completion.
+// One job through the engine, run to completion.  With `audited`, passes stop after the gate that
+// `audit` names (kAuditStage1, kAuditGround) and its survivors' seeds are collected.
 void run_one(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_island_params& p, const Tuning& t,
-             std::vector<bsf_island_hit>* hits, std::vector<int64_t>* stage1, const uint64_t* sequence_seed = nullptr) {
+             std::vector<bsf_island_hit>* hits, std::vector<int64_t>* audited, const uint64_t* sequence_seed = nullptr,
+             int audit = kAuditStage1) {
     if (count == 0) return;
     ensure_engine();
     IslandEngine& e = engine();
@@ -44,9 +46,9 @@ void run_one(const int64_t* h_seeds, int64_t first, int64_t count, const bsf_isl
     }
     const int slot = e.admit(0, 0, first, d_seeds, count, p, t, sequence_seed);
     if (slot < 0) throw CapacityError("island job table full");
-    e.drive(stage1 != nullptr, [&](IslandEngine::PassResult& r) {
+    e.drive(audited ? audit : kAuditNone, [&](IslandEngine::PassResult& r) {
         if (hits) hits->insert(hits->end(), r.hits.begin(), r.hits.end());
-        if (stage1) stage1->insert(stage1->end(), r.stage1.begin(), r.stage1.end());
+        if (audited) audited->insert(audited->end(), r.stage1.begin(), r.stage1.end());
     });
     e.retire(slot);
 }
@@ -103,13 +105,30 @@ int bsf_island_stage1_range(int64_t first, int64_t count, const bsf_island_param
         validate_island_params(p);
         if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad stage1 arguments");
         Tuning t = tuning_for(*p);
-        t.gate = 1;  // the audit is of the gate itself
+        t.gate = kGateStage1;  // the audit is of the Stage-1 gate itself
         std::vector<int64_t> surv;
         run_one(nullptr, first, count, *p, t, nullptr, &surv);
         std::sort(surv.begin(), surv.end());
         *n_found = static_cast<int64_t>(surv.size());
         std::copy(surv.begin(), surv.begin() + std::min<int64_t>(cap, static_cast<int64_t>(surv.size())), out);
         if (static_cast<int64_t>(surv.size()) > cap) throw CapacityError("stage1 output capacity exceeded");
+    });
+}
+
+int bsf_island_ground_range(int64_t first, int64_t count, const bsf_island_params* p, int64_t* out, int64_t cap,
+                            int64_t* n_found) {
+    return guarded_call([&] {
+        require_init();
+        validate_island_params(p);
+        if (count < 0 || cap < 0 || (cap > 0 && !out) || !n_found) throw ArgError("bad ground-gate arguments");
+        Tuning t = tuning_for(*p);
+        t.gate = kGateGround;  // the audit is of the ground gate itself
+        std::vector<int64_t> surv;
+        run_one(nullptr, first, count, *p, t, nullptr, &surv, nullptr, kAuditGround);
+        std::sort(surv.begin(), surv.end());
+        *n_found = static_cast<int64_t>(surv.size());
+        std::copy(surv.begin(), surv.begin() + std::min<int64_t>(cap, static_cast<int64_t>(surv.size())), out);
+        if (static_cast<int64_t>(surv.size()) > cap) throw CapacityError("ground-gate output capacity exceeded");
     });
 }
 
@@ -161,15 +180,16 @@ int bsf_island_engine_configure(int64_t pass_slots, int32_t max_jobs, int32_t mi
     });
 }
 
-int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits) {
+int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits, uint64_t* stage2) {
     return guarded_call([&] {
         require_init();
-        if (!passes || !seeds || !survivors || !hits) throw ArgError("null stats argument");
+        if (!passes || !seeds || !survivors || !hits || !stage2) throw ArgError("null stats argument");
         const EngineStats st = engine().stats;
         *passes = st.passes;
         *seeds = st.seeds;
         *survivors = st.survivors;
         *hits = st.hits;
+        *stage2 = st.stage2;
     });
 }
 
@@ -183,8 +203,8 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
         const bool small = 25 * p->chunks_x * p->chunks_z <= kNdBucket[0];
         std::vector<Tuning> cands;
         for (int k : {1, 2, 3, 4, 5, 8, 16})
-            if (k == 1 || (k <= kMaxWorldsPerWave && k * ncols <= 64)) cands.push_back(Tuning{k, 1, -1});
-        cands.push_back(Tuning{64, 0, -1});
+            if (k == 1 || (k <= kMaxWorldsPerWave && k * ncols <= 64)) cands.push_back(Tuning{k, kGateGround, -1});
+        cands.push_back(Tuning{64, kGateNone, -1});
         std::vector<bsf_island_hit> ref;
         bool have_ref = false;
         int timed = 0;
@@ -220,6 +240,16 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
             if (r > best_rate) {
                 best_rate = r;
                 best = t;
+            }
+        }
+        // the best Stage-1 form without the ground gate
+        if (best.gate == kGateGround) {
+            Tuning alt = best;
+            alt.gate = kGateStage1;
+            const double r = rate_of(alt);
+            if (r > best_rate) {
+                best_rate = r;
+                best = alt;
             }
         }
         best.stage2_fma = small ? 1 : 0;
@@ -302,7 +332,7 @@ int bsf_island_set_tuning(const bsf_island_params* p, const bsf_island_tuning* t
     return guarded_call([&] {
         validate_island_params(p);
         if (!t) throw ArgError("null tuning");
-        if (t->gate != 0 && t->gate != 1) throw ArgError("gate must be 0 or 1");
+        if (t->gate < kGateNone || t->gate > kGateGround) throw ArgError("gate must be 0, 1 or 2");
         if (t->gate && (t->worlds_per_wave < 1 || t->worlds_per_wave > kMaxWorldsPerWave ||
                         (t->worlds_per_wave > 1 && t->worlds_per_wave * 4 * p->chunks_x * p->chunks_z > 64)))
             throw ArgError("worlds per wave must fit the window's sample columns into 64 lanes");

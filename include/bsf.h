@@ -130,6 +130,12 @@ int bsf_island_sequence(uint64_t sequence_seed, int64_t first_attempt, int64_t c
  * sample column), sorted.  Exposed for gate audits. */
 int bsf_island_stage1_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
                             int64_t* out_seeds, int64_t cap, int64_t* n_found);
+/* Survivors of the ground gate (Stage-1 survivors with a candidate start whose stone component,
+ * searched inside the start's own chunk, reaches no block at y = 68 over stone at y = 67:
+ * floodFillFlat's touchesBottom, IslandSearchEngine.java:4170-4175), sorted.  Every hit is one;
+ * exposed for gate audits. */
+int bsf_island_ground_range(int64_t first_seed, int64_t count, const bsf_island_params* p,
+                            int64_t* out_seeds, int64_t cap, int64_t* n_found);
 
 /* ---- batched island jobs (no counterpart in the reference: the reference worker runs one
  * search per process).  Many jobs with their own seed ranges, windows and gate thresholds share
@@ -151,13 +157,16 @@ int bsf_island_batch(const bsf_island_job* jobs, int32_t njobs, bsf_island_hit* 
 int bsf_island_engine_configure(int64_t pass_slots, int32_t max_jobs, int32_t min_quantum);
 
 /* Totals of the library's engine since bsf_init (or the last configure): passes, seeds,
- * Stage-1 survivors, hits. */
-int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits);
+ * Stage-1 survivors, hits, and the survivors that the ground gate passed to Stage 2. */
+int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivors, uint64_t* hits,
+                            uint64_t* stage2);
 
-/* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave, whether
- * the Stage-1 gate runs (gate = 0 sends every seed to Stage 2), and for windows up to 4x4 chunks
- * Stage 2's corner-gradient form.  bsf_island_autotune times the Stage-1 candidates, then the
- * other Stage-2 form with the best of them, each over two runs of trial_seeds seeds from
+/* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave, the gates
+ * (gate = 0 sends every seed to Stage 2, 1 the seeds with a candidate start, 2 -- the default --
+ * of those the ones with a start that is not grounded within its own chunk), and for windows up
+ * to 4x4 chunks Stage 2's corner-gradient form.  bsf_island_autotune times the Stage-1
+ * candidates, the best of them without the ground gate, then the other Stage-2 form with the
+ * best, each over two runs of trial_seeds seeds from
  * first_seed (the faster run counts), checks that every candidate gives the same hits
  * (BSF_ERR_PARITY otherwise), and keeps the fastest for later jobs of that shape.  The reference
  * sizes its batches for throughput against stop latency (SearchSettings.java:25-37); here the

@@ -754,6 +754,57 @@ int islandHasCandidate(int64_t seed, const bsfo_island_params& p) {
     return 0;
 }
 
+// The ground gate's predicate: 1 iff some candidate start of the window (sample column, y >= 68
+// over non-stone; :2909-2925, :4005-4007) lies in a 6-connected stone component above y = 67
+// that, searched inside the start's own chunk only, holds no block at y = 68 over stone at
+// y = 67 (floodFillFlat's touchesBottom, IslandSearchEngine.java:4170-4175).  A breadth-first
+// search per start.
+int islandGroundGate(int64_t seed, const bsfo_island_params& p) {
+    Terrain t(seed);
+    std::vector<double> lattice(425);
+    std::vector<uint8_t> blocks(32768), seen(32768);
+    std::vector<int> queue;
+    auto at = [](int x, int y, int z) { return (x * 16 + z) * 128 + y; };
+    for (int cx = p.chunk_x0; cx < p.chunk_x0 + p.chunks_x; ++cx) {
+        for (int cz = p.chunk_z0; cz < p.chunk_z0 + p.chunks_z; ++cz) {
+            t.chunkDensity(cx, cz, lattice.data());
+            upsample(lattice.data(), [&](int idx, double d) { blocks[idx] = d > 0.0 ? 1 : 0; });
+            for (int sx : {4, 12}) {
+                for (int sz : {4, 12}) {
+                    for (int sy = kIslandFloorY; sy < 128; ++sy) {
+                        if (!blocks[at(sx, sy, sz)] || blocks[at(sx, sy - 1, sz)]) continue;
+                        std::fill(seen.begin(), seen.end(), 0);
+                        queue.assign(1, at(sx, sy, sz));
+                        seen[queue[0]] = 1;
+                        bool grounded = false;
+                        for (size_t head = 0; head < queue.size() && !grounded; ++head) {
+                            const int i = queue[head];
+                            const int x = i / (16 * 128), z = (i / 128) % 16, y = i % 128;
+                            if (y == kIslandFloorY && blocks[at(x, y - 1, z)]) {
+                                grounded = true;
+                                break;
+                            }
+                            const int nb[6][3] = {{x - 1, y, z}, {x + 1, y, z}, {x, y - 1, z},
+                                                  {x, y + 1, z}, {x, y, z - 1}, {x, y, z + 1}};
+                            for (const auto& n : nb) {
+                                if (n[0] < 0 || n[0] > 15 || n[2] < 0 || n[2] > 15) continue;
+                                if (n[1] < kIslandFloorY || n[1] > 127) continue;
+                                const int j = at(n[0], n[1], n[2]);
+                                if (blocks[j] && !seen[j]) {
+                                    seen[j] = 1;
+                                    queue.push_back(j);
+                                }
+                            }
+                        }
+                        if (!grounded) return 1;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Stage-0 scout statistic (config 2): noise2 octave 0 only, chunk (0,0) 5x17x5.
 // ---------------------------------------------------------------------------
@@ -1187,6 +1238,8 @@ int bsfo_island_from_stone(int64_t seed, const uint8_t* stone, int32_t size_x, i
 }
 
 int bsfo_island_has_candidate(int64_t seed, const bsfo_island_params* p) { return islandHasCandidate(seed, *p); }
+
+int bsfo_island_ground_gate(int64_t seed, const bsfo_island_params* p) { return islandGroundGate(seed, *p); }
 
 uint64_t bsfo_density_digest(const double* d) {
     uint64_t h = 1469598103934665603ULL;

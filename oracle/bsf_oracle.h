@@ -103,6 +103,9 @@ int      bsfo_island_from_stone(int64_t seed, const uint8_t* stone, int32_t size
 /* exact Stage-1 predicate used by the GPU pipeline: 1 if any candidate column of the window
  * has stone at y>=68 with non-stone directly below (a superset of all hits) */
 int      bsfo_island_has_candidate(int64_t seed, const bsfo_island_params* p);
+/* ground-gate predicate: 1 if some candidate start's component, searched inside the start's own
+ * chunk, reaches no bottom contact (a y = 68 block over stone at y = 67) */
+int      bsfo_island_ground_gate(int64_t seed, const bsfo_island_params* p);
 
 /* ---- region attempt of one Stage-0 survivor (IslandSearchEngine.processGpuRegionAttempt,
  * :1878-1971): the exact scan of the (2 radius + 1)^2-chunk window around the centre chunk,

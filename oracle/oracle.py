@@ -112,6 +112,10 @@ def island_has_candidate(seed, params):
     return bool(lib().bsfo_island_has_candidate(ctypes.c_int64(seed), ctypes.byref(params)))
 
 
+def island_ground_gate(seed, params):
+    return bool(lib().bsfo_island_ground_gate(ctypes.c_int64(seed), ctypes.byref(params)))
+
+
 def density_digest(d425):
     d = np.ascontiguousarray(d425, dtype=np.float64)
     return lib().bsfo_density_digest(d.ctypes.data_as(ctypes.c_void_p))

@@ -40,12 +40,12 @@ def test_worlds_per_wave_and_gate_split_keep_hits(bsf, oracle):
     ref = oracle.run_island(first, count, oracle.island_params(**kw(3, -2, 1, 2)), threads=16)
     assert len(ref) > 0
     try:
-        for k, gate in ((1, 1), (3, 1), (8, 1), (1, 0)):
+        for k, gate in ((1, 2), (3, 2), (8, 2), (1, 1), (8, 1), (1, 0)):
             bsf.set_tuning(p, k, gate)
             assert bsf.get_tuning(p).worlds_per_wave == (k if gate else 64)
             assert bsf.island_range(first, count, p) == ref, (k, gate)
     finally:
-        bsf.set_tuning(p, 8, 1)
+        bsf.set_tuning(p, 8, 2)
 
 
 def test_stage2_forms_keep_hits(bsf, oracle):
@@ -58,19 +58,19 @@ def test_stage2_forms_keep_hits(bsf, oracle):
     q = bsf.island_params(**kw(2, 2, 2, 2))
     try:
         for form in (1, 0):
-            bsf.set_tuning(p, 1, 1, stage2_fma=form)
+            bsf.set_tuning(p, 1, 2, stage2_fma=form)
             assert bsf.get_tuning(p).stage2_fma == form
             assert bsf.island_range(first, count, p) == ref, form
         # one job of each form in the same passes
-        bsf.set_tuning(p, 1, 1, stage2_fma=0)
-        bsf.set_tuning(q, 1, 1, stage2_fma=1)
+        bsf.set_tuning(p, 1, 2, stage2_fma=0)
+        bsf.set_tuning(q, 1, 2, stage2_fma=1)
         jobs = [(first, count, p), (first + 7, count, q)]
         hits, _ = bsf.island_batch(jobs)
         for i, (f0, c0, pp) in enumerate(jobs):
             assert [h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in bsf.island_range(f0, c0, pp)], i
     finally:
-        bsf.set_tuning(p, 1, 1, stage2_fma=-1)
-        bsf.set_tuning(q, 1, 1, stage2_fma=-1)
+        bsf.set_tuning(p, 1, 2, stage2_fma=-1)
+        bsf.set_tuning(q, 1, 2, stage2_fma=-1)
 
 
 def test_autotune_agrees_and_records(bsf):

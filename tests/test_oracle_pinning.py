@@ -163,3 +163,34 @@ def test_chunk_climate_matches_reference_coarse_climate(oracle):
             assert rain[idx].view(np.uint64).tolist() == [int(h, 16) for h in ch["rain"]], (case["seed"], cx, cz)
             n += 16
     assert n >= 800
+
+
+# Every floating component with a start matches under these thresholds (isMatch,
+# IslandSearchEngine.java:3972-4003, with only touchesBottom left to reject).
+LOOSE = dict(min_blocks=1, min_columns=1, min_y_for_match=0, min_width=1, min_depth=1, min_avg_thickness=0.0)
+
+
+def _gate_and_scan(oracle, seeds, kw):
+    from concurrent.futures import ThreadPoolExecutor
+    po = oracle.island_params(**kw)
+    with ThreadPoolExecutor(8) as ex:
+        return list(ex.map(lambda s: (oracle.island_ground_gate(s, po), oracle.island_scan(s, po) is not None), seeds))
+
+
+@pytest.mark.parametrize("x0,z0", [(0, 0), (-3, 5)])
+def test_ground_gate_equals_loose_scan_on_one_chunk(oracle, x0, z0):
+    """In a one-chunk window the start's own chunk is the window, so the ground gate passes a seed
+    exactly when the scan finds a floating component under the loosest thresholds."""
+    kw = dict(chunk_x0=x0, chunk_z0=z0, chunks_x=1, chunks_z=1, **LOOSE)
+    res = _gate_and_scan(oracle, range(5000, 13000), kw)
+    assert all(g == h for g, h in res)
+    assert sum(g for g, _ in res) >= 10
+
+
+def test_ground_gate_is_sound_on_wider_windows(oracle):
+    """A component found in one chunk may join a contact through a neighbouring chunk, so on wider
+    windows the gate passes a superset of the seeds with a floating component -- never fewer."""
+    kw = dict(chunk_x0=-2, chunk_z0=-2, chunks_x=4, chunks_z=4, **LOOSE)
+    res = _gate_and_scan(oracle, range(5000, 5400), kw)
+    assert all(g or not h for g, h in res)
+    assert sum(h for _, h in res) >= 5 and sum(g for g, _ in res) > sum(h for _, h in res)
