@@ -1,10 +1,12 @@
 // bsf_engine.cpp -- the island search engine's host side (see bsf_engine.hpp).
 //
-// A pass is launched as one captured hipGraph (or eagerly, when launch timing is on):
-//   reset -> build -> cache -> Stage 1 -> ground gate -> Stage 2 -> flood x 4
-// followed by asynchronous copies of its counters, the job table and the first hits into
-// pinned memory.  Two result buffer sets alternate, so pass k+1 runs while the host reads
-// pass k.
+// A pass is launched as two captured hipGraphs (or eagerly, when launch timing is on):
+//   front, on stream_:       reset -> build -> cache -> Stage 1
+//   back, on back_stream_:   ground gate -> Stage 2 -> flood x 4
+// The back half waits for its own front half; the front half of pass k+1 runs beside the back
+// half of pass k.  Each half is followed by asynchronous copies into pinned memory (the job
+// table's cursors after the front, counters, hit counters and the first hits after the back).
+// Two buffer sets alternate, so pass k+1 runs while the host reads pass k.
 #include "bsf_engine.hpp"
 
 #include "bsf_cache.hpp"
@@ -56,10 +58,13 @@ size_t stage2_lds(int v) { return static_cast<size_t>(2 * kNdBucket[v == kS2Sele
 void IslandEngine::release() {
     drop_graphs();
     if (stream_) (void)hipStreamSynchronize(stream_);
-    for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &perms_, &offs_, &surv_slot_, &surv_seed_, &surv_job_, &lattice_,
-                            &flood_list_, &flood_scratch_, &open_, &surv_mask_, &s2_list_, &ctr_[0], &ctr_[1], &hits_[0],
-                            &hits_[1]})
+    if (back_stream_) (void)hipStreamSynchronize(back_stream_);
+    for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &lattice_, &flood_list_, &flood_scratch_, &open_, &s2_list_})
         b->release();
+    for (int i = 0; i < 2; ++i)
+        for (DeviceBuffer* b : {&perms_[i], &offs_[i], &surv_slot_[i], &surv_seed_[i], &surv_job_[i], &surv_mask_[i],
+                                &ctr_[i], &hits_[i]})
+            b->release();
     auto free_pinned = [](void* p) {
         if (p) (void)hipHostFree(p);
     };
@@ -67,6 +72,10 @@ void IslandEngine::release() {
     h_jobs_ = nullptr;
     for (int i = 0; i < 2; ++i) {
         free_pinned(h_snap_[i]);
+        free_pinned(h_snap_hits_[i]);
+        h_snap_hits_[i] = nullptr;
+        if (front_done_[i]) (void)hipEventDestroy(front_done_[i]);
+        front_done_[i] = nullptr;
         free_pinned(h_ctr_[i]);
         free_pinned(h_hits_[i]);
         free_pinned(h_stage1_[i]);
@@ -80,8 +89,9 @@ void IslandEngine::release() {
         done_[i] = nullptr;
     }
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (back_stream_) (void)hipStreamDestroy(back_stream_);
     if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
-    stream_ = copy_stream_ = nullptr;
+    stream_ = back_stream_ = copy_stream_ = nullptr;
     host_jobs_.clear();
     cfg_slots_ = 0;
     cfg_jobs_ = 0;
@@ -100,28 +110,33 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     cfg_quantum_ = min_quantum;
     timers_ = timers;
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&back_stream_, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "hipStreamCreate");
     const size_t P = static_cast<size_t>(pass_slots);
     jobs_.ensure(static_cast<size_t>(max_jobs) * sizeof(IslandJob));
     segs_.ensure(2 * static_cast<size_t>(max_jobs) * sizeof(PassSeg));
     desc_.ensure(sizeof(PassDesc));
-    perms_.ensure(P * kCachePermBytes);
-    offs_.ensure(P * kGenCount * 3 * sizeof(double));
-    surv_slot_.ensure(P * sizeof(int64_t));
-    surv_seed_.ensure(P * sizeof(int64_t));
-    surv_job_.ensure(P * sizeof(int32_t));
+    for (int i = 0; i < 2; ++i) {
+        perms_[i].ensure(P * kCachePermBytes);
+        offs_[i].ensure(P * kGenCount * 3 * sizeof(double));
+        surv_slot_[i].ensure(P * sizeof(int64_t));
+        surv_seed_[i].ensure(P * sizeof(int64_t));
+        surv_job_[i].ensure(P * sizeof(int32_t));
+        surv_mask_[i].ensure(P * sizeof(unsigned long long));
+    }
     lattice_.ensure(P * static_cast<size_t>(kNdBucket[1]) * 8 * sizeof(double));
     flood_list_.ensure(static_cast<size_t>(kFloodClasses) * P * sizeof(int32_t));
     // every sample column of every one-world seed can be open: 4 columns per chunk, 64 chunks
     open_cap_ = static_cast<int64_t>(P) * 256;
     open_.ensure(static_cast<size_t>(open_cap_) * sizeof(OpenColumn));
-    surv_mask_.ensure(P * sizeof(unsigned long long));
     s2_list_.ensure(P * sizeof(int32_t));
     for (int i = 0; i < 2; ++i) {
         ctr_[i].ensure(kCounters * sizeof(unsigned long long));
         hits_[i].ensure(P * sizeof(bsf_island_hit));
         check(hipMemset(ctr_[i].ptr, 0, kCounters * sizeof(unsigned long long)), "hipMemset");
         h_snap_[i] = pinned<IslandJob>(static_cast<size_t>(max_jobs));
+        h_snap_hits_[i] = pinned<IslandJob>(static_cast<size_t>(max_jobs));
+        check(hipEventCreateWithFlags(&front_done_[i], hipEventDisableTiming), "hipEventCreate");
         h_ctr_[i] = pinned<unsigned long long>(kCounters);
         h_hits_[i] = pinned<bsf_island_hit>(P);
         h_stage1_[i] = pinned<int64_t>(P);
@@ -250,12 +265,12 @@ EngineArgs IslandEngine::args_for(int ab, int audit) const {
     a.pass_slots = cfg_slots_;
     a.desc = static_cast<PassDesc*>(desc_.ptr);
     a.segs = static_cast<PassSeg*>(segs_.ptr);
-    a.perms = static_cast<uint32_t*>(perms_.ptr);
-    a.offsets = static_cast<double*>(offs_.ptr);
+    a.perms = static_cast<uint32_t*>(perms_[ab].ptr);
+    a.offsets = static_cast<double*>(offs_[ab].ptr);
     a.ctr = static_cast<unsigned long long*>(ctr_[ab].ptr);
-    a.surv_slot = static_cast<int64_t*>(surv_slot_.ptr);
-    a.surv_seed = static_cast<int64_t*>(surv_seed_.ptr);
-    a.surv_job = static_cast<int32_t*>(surv_job_.ptr);
+    a.surv_slot = static_cast<int64_t*>(surv_slot_[ab].ptr);
+    a.surv_seed = static_cast<int64_t*>(surv_seed_[ab].ptr);
+    a.surv_job = static_cast<int32_t*>(surv_job_[ab].ptr);
     a.lattice = static_cast<double*>(lattice_.ptr);
     a.lat_stride = static_cast<int64_t>(kNdBucket[1]) * 8;
     a.flood_list = static_cast<int32_t*>(flood_list_.ptr);
@@ -264,15 +279,14 @@ EngineArgs IslandEngine::args_for(int ab, int audit) const {
     a.flood_scratch_words = flood_scratch_words_;
     a.open = static_cast<OpenColumn*>(open_.ptr);
     a.open_cap = open_cap_;
-    a.surv_mask = static_cast<unsigned long long*>(surv_mask_.ptr);
+    a.surv_mask = static_cast<unsigned long long*>(surv_mask_[ab].ptr);
     a.s2_list = static_cast<int32_t*>(s2_list_.ptr);
     a.audit = audit;
     return a;
 }
 
-void IslandEngine::enqueue_kernels(int ab, int audit, int s2_variants) {
+void IslandEngine::enqueue_front(int ab, int audit, hipStream_t s) {
     const EngineArgs a = args_for(ab, audit);
-    hipStream_t s = stream_;
     const uint64_t seeds = static_cast<uint64_t>(flight_[ab].seeds);
     hipLaunchKernelGGL(pass_reset_kernel, dim3(256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(pass_build_kernel, dim3(1), dim3(256), 0, s, a);
@@ -304,6 +318,10 @@ void IslandEngine::enqueue_kernels(int ab, int audit, int s2_variants) {
         check(hipGetLastError(), "stage 1b launch");
         t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[0]));
     }
+}
+
+void IslandEngine::enqueue_back(int ab, int audit, int s2_variants, hipStream_t s) {
+    const EngineArgs a = args_for(ab, audit);
     if (audit == kAuditStage1) return;
     {
         // units: the pass's Stage-1 survivors
@@ -343,11 +361,43 @@ void IslandEngine::enqueue_kernels(int ab, int audit, int s2_variants) {
 }
 
 void IslandEngine::drop_graphs() {
-    for (auto& kv : graphs_) {
-        (void)hipGraphExecDestroy(kv.second.second);
-        (void)hipGraphDestroy(kv.second.first);
+    for (auto* m : {&graphs_front_, &graphs_back_}) {
+        for (auto& kv : *m) {
+            (void)hipGraphExecDestroy(kv.second.second);
+            (void)hipGraphDestroy(kv.second.first);
+        }
+        m->clear();
     }
-    graphs_.clear();
+}
+
+template <class F>
+void IslandEngine::run_half(std::map<int, std::pair<hipGraph_t, hipGraphExec_t>>& graphs, int key, hipStream_t s,
+                            F&& enqueue) {
+    if ((timers_ && timing_enabled()) || !use_graphs_) {
+        enqueue();
+        return;
+    }
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+        const bool saved = timers_;
+        timers_ = false;
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            timers_ = saved;
+            throw;
+        }
+        check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+        timers_ = saved;
+        hipGraphExec_t ge = nullptr;
+        check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        it = graphs.emplace(key, std::make_pair(g, ge)).first;
+    }
+    check(hipGraphLaunch(it->second.second, s), "hipGraphLaunch");
 }
 
 void IslandEngine::launch(int audit) {
@@ -375,52 +425,39 @@ void IslandEngine::launch(int audit) {
     // the case-selection kernel with the large layout's LDS serves small windows too
     if (variants & (1 << kS2SelectLarge)) variants &= ~(1 << kS2SelectSmall);
     if (variants == 0) variants = 1 << kS2FmaSmall;
-    if (timers_ && timing_enabled()) {
-        enqueue_kernels(ab, audit, variants);
-    } else if (!use_graphs_) {
-        enqueue_kernels(ab, audit, variants);
-    } else {
-        const int key = ab | (audit << 1) | (variants << 3);
-        auto it = graphs_.find(key);
-        if (it == graphs_.end()) {
-            const bool saved = timers_;
-            timers_ = false;
-            hipGraph_t g = nullptr;
-            check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            try {
-                enqueue_kernels(ab, audit, variants);
-            } catch (...) {
-                (void)hipStreamEndCapture(s, &g);
-                if (g) (void)hipGraphDestroy(g);
-                timers_ = saved;
-                throw;
-            }
-            check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
-            timers_ = saved;
-            hipGraphExec_t ge = nullptr;
-            check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-            it = graphs_.emplace(key, std::make_pair(g, ge)).first;
-        }
-        check(hipGraphLaunch(it->second.second, s), "hipGraphLaunch");
-    }
-    check(hipMemcpyAsync(h_ctr_[ab], ctr_[ab].ptr, kCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
-          "D2H counters");
+    // the front half on stream_; the back half waits for it (and for the other pass's back half,
+    // whose lattice and lists it reuses) on back_stream_ -- or on stream_ when kernels are timed
+    // one by one, so that no two kernels overlap
+    const bool timed = timers_ && timing_enabled();
+    hipStream_t sb = timed ? stream_ : back_stream_;
+    run_half(graphs_front_, ab | (audit << 1), s, [&] { enqueue_front(ab, audit, s); });
     check(hipMemcpyAsync(h_snap_[ab], jobs_.ptr, static_cast<size_t>(cfg_jobs_) * sizeof(IslandJob),
                          hipMemcpyDeviceToHost, s),
           "D2H job table");
+    check(hipEventRecord(front_done_[ab], s), "hipEventRecord");
+    check(hipStreamWaitEvent(sb, front_done_[ab], 0), "hipStreamWaitEvent");
+    if (inflight_n_ > 0) check(hipStreamWaitEvent(sb, done_[ab ^ 1], 0), "hipStreamWaitEvent");
+    if (audit != kAuditStage1)
+        run_half(graphs_back_, ab | (audit << 1) | (variants << 3), sb,
+                 [&] { enqueue_back(ab, audit, variants, sb); });
+    check(hipMemcpyAsync(h_ctr_[ab], ctr_[ab].ptr, kCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, sb),
+          "D2H counters");
+    check(hipMemcpyAsync(h_snap_hits_[ab], jobs_.ptr, static_cast<size_t>(cfg_jobs_) * sizeof(IslandJob),
+                         hipMemcpyDeviceToHost, sb),
+          "D2H job table");
     const int64_t prefix = std::min(kHitPrefix, cfg_slots_);
     check(hipMemcpyAsync(h_hits_[ab], hits_[ab].ptr, static_cast<size_t>(prefix) * sizeof(bsf_island_hit),
-                         hipMemcpyDeviceToHost, s),
+                         hipMemcpyDeviceToHost, sb),
           "D2H hits");
     if (audit != kAuditNone)
-        check(hipMemcpyAsync(h_stage1_[ab], surv_seed_.ptr, static_cast<size_t>(cfg_slots_) * sizeof(int64_t),
-                             hipMemcpyDeviceToHost, s),
+        check(hipMemcpyAsync(h_stage1_[ab], surv_seed_[ab].ptr, static_cast<size_t>(cfg_slots_) * sizeof(int64_t),
+                             hipMemcpyDeviceToHost, sb),
               "D2H stage 1 survivors");
     if (audit == kAuditGround)
         check(hipMemcpyAsync(h_list_[ab], s2_list_.ptr, static_cast<size_t>(cfg_slots_) * sizeof(int32_t),
-                             hipMemcpyDeviceToHost, s),
+                             hipMemcpyDeviceToHost, sb),
               "D2H ground-gate survivors");
-    check(hipEventRecord(done_[ab], s), "hipEventRecord");
+    check(hipEventRecord(done_[ab], sb), "hipEventRecord");
     next_ab_ ^= 1;
     ++inflight_n_;
 }
@@ -460,7 +497,7 @@ void IslandEngine::complete_oldest(PassResult& out) {
                            " differs from the planned " + std::to_string(sc.second));
         HostJob& h = host_jobs_[sc.first];
         h.done = sc.second;
-        h.hits = dj.hits;
+        h.hits = h_snap_hits_[ab][sc.first].hits;
     }
     out.finished = f.finished;
     stats.passes += 1;

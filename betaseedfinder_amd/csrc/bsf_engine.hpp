@@ -1,6 +1,14 @@
 // bsf_engine.hpp -- host side of the island search engine: device job table, pass buffers,
 // captured pass graphs, the pass driver shared by the plain, batched and served entry points,
 // and the autotuner's table.
+//
+// A pass has a front half (reset, build, cache, Stage 1) and a back half (ground gate, Stage 2,
+// floods), each captured as its own graph and launched on its own stream, so the front half of
+// pass k+1 runs on the device beside the back half of pass k.  What the two halves share lives in
+// two sets indexed by the pass's parity (generator cache, survivor lists, start-chunk masks,
+// counters, hits); what one half alone uses is single (open columns, pass descriptors: front;
+// lattice, Stage-2 and flood lists: back).  At most two passes are in flight, so a pass never
+// reuses a set that a running pass still reads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -102,7 +110,11 @@ class IslandEngine {
         std::vector<std::pair<int, int64_t>> cursors;  // (slot, cursor after this pass)
         std::vector<int> finished;
     };
-    void enqueue_kernels(int ab, int audit, int s2_variants);
+    void enqueue_front(int ab, int audit, hipStream_t s);
+    void enqueue_back(int ab, int audit, int s2_variants, hipStream_t s);
+    // Launches one half: captured as a graph under `key` on first use (or eagerly, with timers).
+    template <class F>
+    void run_half(std::map<int, std::pair<hipGraph_t, hipGraphExec_t>>& graphs, int key, hipStream_t s, F&& enqueue);
     void drop_graphs();
     EngineArgs args_for(int ab, int audit) const;
     int64_t plan_pass(std::vector<std::pair<int, int64_t>>& cursors, std::vector<int>& finished,
@@ -112,20 +124,22 @@ class IslandEngine {
     int cfg_jobs_ = 0;
     int cfg_quantum_ = 1024;
     bool timers_ = false, use_graphs_ = true;
-    hipStream_t stream_ = nullptr, copy_stream_ = nullptr;
-    DeviceBuffer jobs_, segs_, desc_, perms_, offs_, surv_slot_, surv_seed_, surv_job_, lattice_, flood_list_,
-            flood_scratch_, open_, surv_mask_, s2_list_;
+    hipStream_t stream_ = nullptr, back_stream_ = nullptr, copy_stream_ = nullptr;
+    DeviceBuffer jobs_, segs_, desc_, lattice_, flood_list_, flood_scratch_, open_, s2_list_;
+    DeviceBuffer perms_[2], offs_[2], surv_slot_[2], surv_seed_[2], surv_job_[2], surv_mask_[2];
     int64_t open_cap_ = 0;
     int grid_s1b_ = 0, grid_ground_ = 0;
     DeviceBuffer ctr_[2], hits_[2];
     // pinned host memory
     IslandJob* h_jobs_ = nullptr;
-    IslandJob* h_snap_[2] = {nullptr, nullptr};
+    IslandJob* h_snap_[2] = {nullptr, nullptr};      // job table after the front half (cursors)
+    IslandJob* h_snap_hits_[2] = {nullptr, nullptr}; // job table after the back half (hit counters)
     unsigned long long* h_ctr_[2] = {nullptr, nullptr};
     bsf_island_hit* h_hits_[2] = {nullptr, nullptr};
     int64_t* h_stage1_[2] = {nullptr, nullptr};
     int32_t* h_list_[2] = {nullptr, nullptr};    // s2_list of ground-gate audits
     hipEvent_t done_[2] = {nullptr, nullptr};
+    hipEvent_t front_done_[2] = {nullptr, nullptr};
     Flight flight_[2];
     std::vector<HostJob> host_jobs_;
     int host_rr_ = 0;
@@ -133,7 +147,8 @@ class IslandEngine {
     int grid_cache_ = 0, grid_s1_[2] = {0, 0}, grid_s2_[kS2Variants][2] = {}, grid_flood_[kFloodClasses] = {0, 0, 0, 0};
     int64_t flood_scratch_words_ = 0;
     int next_ab_ = 0, inflight_n_ = 0, oldest_ab_ = 0;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;  // key: ab | audit << 1 | Stage-2 variants << 3
+    // front key: ab | audit << 1; back key: the same | Stage-2 variants << 3
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_front_, graphs_back_;
     static constexpr int64_t kHitPrefix = 4096;  // hits copied back with every pass
 };
 
