@@ -64,57 +64,15 @@ def island_workload():
     return start, min_blocks
 
 
-# Algorithmic FP64 operations (add, sub, mul, div, compare, int<->double conversion) of one
-# Stage-1 seed, counted per construct of the Java algorithm with cached axes
-# (NoiseGeneratorPerlin173.addNoiseCachedAxes*, :399-507): the x and z axes once per column and
-# octave, the y axis once per octave, the eight corner gradients + four x-lerps once per y
-# cell (the legacy Y-cell reuse), and per point the y/z lerps, the weight and the accumulate.
-OPS = {
-    "axis": 16,             # coordinate (cvt, mul, add) + Java floor (2 cvt, cmp) + frac (cvt, sub) + fm1 + fade (7)
-    "yaxis_level": 16,      # level coordinate, floor, frac, fade, cell fm1
-    "corners": 20,          # 8 gradients (one add each) + 4 x-lerps (3 each)
-    "point3": 11,           # 2 y-lerps + 1 z-lerp (3 each), weight, accumulate
-    "point2": 18,           # 1 legacy + 3 gradients, 3 lerps, weight, accumulate (+0.0*x)
-    "simplex": 70,          # one simplex2 sample with all three corners live, weight, accumulate
-    "finish": 16,           # climate post-processing
-    "shape": 20,            # column shape d5 / d7
-    "density_level": 15,    # one lattice density
-    "chain_level": 18,      # candidate y-chain per lattice level (sub, mul, 8 x (cmp, add))
-    "bounds_level": 14,     # Stage-1a density bounds of one level (interval ends, blend hull, taper)
-}
-def cells(levels, sy):
-    """Expected Y cells (corner sets) of one octave over `levels` lattice levels: a cell of
-    scale sy covers 1/sy levels."""
-    return 1 + (levels - 1) * min(1.0, sy)
-
-
-def column3d_ops(levels, octaves, y_table=True):
-    """Algorithmic FP64 ops of 3D octaves [(sy), ...] at one column over `levels` levels; the
-    y axis is counted per octave only when it is not shared (y_table=False)."""
-    ops = 0.0
-    for sy in octaves:
-        ops += 2 * OPS["axis"] + cells(levels, sy) * OPS["corners"] + levels * OPS["point3"]
-        if not y_table:
-            ops += levels * OPS["yaxis_level"]
-    return ops
-
-
-N2 = [684.412 / 2 ** o for o in range(16)]
-N1 = [684.412 / 160.0 / 2 ** o for o in range(8)]
-SPLIT = 10   # Stage 1a sums noise2 / noise3 over octaves SPLIT..15
-
-
-def stage1a_ops_per_seed(n_cols=64, levels=8):
-    """Stage 1a: noise2 / noise3 octaves SPLIT..15, noise1, noise4/5, climate, bounds."""
-    o3 = N2[SPLIT:] * 2 + N1
-    per_col = (column3d_ops(levels, o3) + 26 * (2 * OPS["axis"] + OPS["point2"]) + 10 * OPS["simplex"] +
-               OPS["finish"] + OPS["shape"] + levels * (OPS["density_level"] + OPS["bounds_level"]))
-    return n_cols * per_col + len(o3) * levels * OPS["yaxis_level"]
-
-
-def stage1b_ops_per_column(levels=8):
-    """Stage 1b: one open column, noise2 + noise3 over all 16 octaves, densities, y-chain."""
-    return column3d_ops(levels, N2 * 2, y_table=False) + levels * (OPS["density_level"] + OPS["chain_level"])
+# Algorithmic FP64 operations of island Stage 1a per seed, 4x4 window: counted by the
+# instrumented oracle (bsfo_stage1_op_count: the 64 sample columns' noise2 / noise3 octaves 10..15,
+# noise1, noise4/5, climate, column shape and per-level bias, blend and density bounds, run with a
+# counting number type; its plain run reproduces the oracle's densities and Stage-1 decisions),
+# mean over the first 2^16 seeds of the range: 200,993 add/sub + 109,115 mul + 4,936 div + 10,307
+# compare + 20,456 int<->double conversions.  Stage 1b adds 12,537 per open column at 0.914 open
+# columns per seed (python oracle/count_ops.py --seeds 65536; tests/test_op_counts.py checks the
+# Stage-1a figure against a fresh count).
+STAGE1A_FP64_OPS = 345807
 
 
 # Algorithmic FP64 operations of island Stage 2 per survivor, 4x4 window: counted by the
@@ -585,8 +543,9 @@ def main():
     fp64 = {
         "island_stage1": dict(kernel="pass_stage1_kernel<0> (Stage 1a: 64 sample columns per seed, low-frequency "
                                      "noise2/noise3 octaves + bounds)",
-                              unit="seed", ops=stage1a_ops_per_seed(), units=ks["island_stage1"][2],
-                              source="per-construct model (OPS)"),
+                              unit="seed", ops=STAGE1A_FP64_OPS, units=ks["island_stage1"][2],
+                              source="instrumented oracle count (bsfo_stage1_op_count), mean over the first "
+                                     "2^16 seeds of the range"),
         "island_stage2": dict(kernel="pass_stage2_kernel (Stage 2: 17x17 noise lattice + 400 density columns per "
                                      "survivor)",
                               unit="survivor", ops=STAGE2_FP64_OPS, units=stage2_units,

@@ -678,8 +678,11 @@ __device__ __forceinline__ double lattice_density(const ColumnShape& cs, int y, 
 }
 
 // Bounds of density_from when noise2 and noise3 are only known to lie in [n - u, n + u]
-// (widened by `slack` for rounding): the blend is a convex combination and the taper is
-// increasing, so the ends of the interval map to the ends.
+// (widened by `slack` for rounding): with the blend factor d12 in [0, 1] the blend
+// d10 + (d11 - d10) d12 = (1 - d12) d10 + d12 d11 is increasing in both sums, so the intervals'
+// lower ends blend to the lower bound and the upper ends to the upper one (an interval of width
+// 2u / 512 whatever noise2 and noise3 are); the taper is increasing too.  Rounding moves each
+// end by a few ulps of values below 2^9, far below `slack`.
 __device__ __forceinline__ void density_bounds(double d9, double d12, double n2, double n3, double u, int y,
                                                double slack, double& lo, double& hi) {
     const double l2 = (n2 - u) / 512.0, h2 = (n2 + u) / 512.0;
@@ -692,8 +695,8 @@ __device__ __forceinline__ void density_bounds(double d9, double d12, double n2,
         l = l3;
         h = h3;
     } else {
-        l = l2 < l3 ? l2 : l3;
-        h = h2 > h3 ? h2 : h3;
+        l = l2 + (l3 - l2) * d12;
+        h = h2 + (h3 - h2) * d12;
     }
     lo = top_taper(l - d9, y) - slack;
     hi = top_taper(h - d9, y) + slack;

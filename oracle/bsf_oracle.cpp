@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -1626,6 +1627,7 @@ inline CD operator*(CD a, CD b) { ++t_tally->mul; return CD(a.v * b.v); }
 inline CD operator/(CD a, CD b) { ++t_tally->div; return CD(a.v / b.v); }
 inline bool operator<(CD a, CD b) { ++t_tally->cmp; return a.v < b.v; }
 inline bool operator>(CD a, CD b) { ++t_tally->cmp; return a.v > b.v; }
+inline bool operator<=(CD a, CD b) { ++t_tally->cmp; return a.v <= b.v; }
 
 template <class T> T i2d(int i);
 template <> inline double i2d<double>(int i) { return static_cast<double>(i); }
@@ -1915,9 +1917,297 @@ void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 operation count of island Stage 1 for one seed: Stage 1a at the window's sample columns
+// (lattice columns (4 cx + i, 4 cz + l), i, l in {1, 3}: blocks 4 / 12, IslandSearchEngine.java:
+// 4005-4007) -- noise2 / noise3 over octaves 10..15 only, noise1, noise4, noise5 and the climate
+// in full, the column shape, and per level the height bias, the noise1 blend and the density
+// bounds that allow for the left-out octaves (each octave o of a 3D Perlin sum is at most 2 * 2^o
+// in magnitude, so octaves 0..9 add at most 2046) -- then Stage 1b at each column the bounds leave
+// open: noise2 / noise3 over all 16 octaves, the exact densities and generateBareTerrain's y-chain
+// (BetaTerrain173.java:599-664) looking for stone at y >= 68 over non-stone.  A column is open when
+// some level k has lo[k] <= 0 < hi[k + 1].  The x axis of an octave is formed once per distinct
+// lattice x, the y axis once per octave (Stage 1a: shared by the seed's columns; Stage 1b: per
+// column, each open column being evaluated alone).
+// ---------------------------------------------------------------------------
+constexpr int kSplitOctave1 = 10;
+
+// 3D octaves [o0, o1) of one group at lattice columns cols, levels 8..15, accumulated into acc
+// (the first evaluated octave starts each sum at 0.0 + value).
+template <class T>
+void octaves3dAt(const Octaves& oct, double sxz, double sy0, int o0, int o1, const std::vector<std::pair<int, int>>& cols,
+                 std::vector<std::array<T, kS2Levels>>& acc) {
+    T amp = T(1.0);
+    for (int o = 0; o < o0; ++o) amp = amp / T(2.0);
+    for (int o = o0; o < o1; ++o) {
+        const Perlin& p = oct.g[static_cast<size_t>(o)];
+        const T sx = T(sxz) * amp, sy = T(sy0) * amp, w = T(1.0) / amp;
+        int yi[kS2Levels];
+        bool fresh[kS2Levels];
+        T cell_f[kS2Levels], fd[kS2Levels];
+        for (int k = 0; k < kS2Levels; ++k) {
+            const T v = i2d<T>(kS2Y0 + k) * sy + T(p.b);
+            const int fl = floorT(v);
+            yi[k] = fl & 255;
+            const T f = v - i2d<T>(fl);
+            fd[k] = fadeT(f);
+            fresh[k] = k == 0 || yi[k] != yi[k - 1];
+            cell_f[k] = fresh[k] ? f : cell_f[k - 1];
+        }
+        for (int y = kS2Y0 - 1; y >= 0; --y) {
+            const T v = i2d<T>(y) * sy + T(p.b);
+            const int fl = floorT(v);
+            if ((fl & 255) != yi[0]) break;
+            cell_f[0] = v - i2d<T>(fl);
+        }
+        T cell_fm1[kS2Levels];
+        for (int k = 0; k < kS2Levels; ++k)
+            if (fresh[k]) cell_fm1[k] = cell_f[k] - T(1.0);
+            else cell_fm1[k] = cell_fm1[k - 1];
+        std::map<int, AxisT<T>> xaxes;
+        for (size_t c = 0; c < cols.size(); ++c) {
+            const int X = cols[c].first, Z = cols[c].second;
+            auto xa = xaxes.find(X);
+            if (xa == xaxes.end()) xa = xaxes.emplace(X, axisT(i2d<T>(X) * sx + T(p.a))).first;
+            const AxisT<T>& ax = xa->second;
+            const AxisT<T> az = axisT(i2d<T>(Z) * sx + T(p.c));
+            auto& out = acc[c];
+            T q00 = T(0.0), q01 = T(0.0), q10 = T(0.0), q11 = T(0.0);
+            for (int k = 0; k < kS2Levels; ++k) {
+                if (fresh[k]) {
+                    const int p60 = p.d[ax.i] + yi[k], p61 = p.d[p60] + az.i, p62 = p.d[p60 + 1] + az.i;
+                    const int p63 = p.d[ax.i + 1] + yi[k], p33 = p.d[p63] + az.i, p64 = p.d[p63 + 1] + az.i;
+                    const T yf = cell_f[k], yfm1 = cell_fm1[k];
+                    q00 = lerpT(ax.fd, grad3T(p.d[p61], ax.f, yf, az.f), grad3T(p.d[p33], ax.fm1, yf, az.f));
+                    q01 = lerpT(ax.fd, grad3T(p.d[p62], ax.f, yfm1, az.f), grad3T(p.d[p64], ax.fm1, yfm1, az.f));
+                    q10 = lerpT(ax.fd, grad3T(p.d[p61 + 1], ax.f, yf, az.fm1), grad3T(p.d[p33 + 1], ax.fm1, yf, az.fm1));
+                    q11 = lerpT(ax.fd, grad3T(p.d[p62 + 1], ax.f, yfm1, az.fm1),
+                                grad3T(p.d[p64 + 1], ax.fm1, yfm1, az.fm1));
+                }
+                const T r0 = lerpT(fd[k], q00, q01), r1 = lerpT(fd[k], q10, q11);
+                const T val = lerpT(az.fd, r0, r1) * w;
+                out[k] = o == o0 ? T(0.0) + val : out[k] + val;
+            }
+        }
+        amp = amp / T(2.0);
+    }
+}
+
+// The 2D octaves of one group at lattice columns cols.
+template <class T>
+void octaves2dAt(const Octaves& oct, double sc, const std::vector<std::pair<int, int>>& cols, std::vector<T>& acc) {
+    T amp = T(1.0);
+    for (size_t o = 0; o < oct.g.size(); ++o) {
+        const Perlin& p = oct.g[o];
+        const T sx = T(sc) * amp, w = T(1.0) / amp;
+        std::map<int, AxisT<T>> xaxes;
+        for (size_t c = 0; c < cols.size(); ++c) {
+            auto xa = xaxes.find(cols[c].first);
+            if (xa == xaxes.end()) xa = xaxes.emplace(cols[c].first, axisT(i2d<T>(cols[c].first) * sx + T(p.a))).first;
+            const AxisT<T>& ax = xa->second;
+            const AxisT<T> az = axisT(i2d<T>(cols[c].second) * sx + T(p.c));
+            const int p47 = p.d[p.d[ax.i]] + az.i, p20 = p.d[p.d[ax.i + 1]] + az.i;
+            const T q0 = lerpT(ax.fd, grad2T(p.d[p47], ax.f, az.f), grad3T(p.d[p20], ax.fm1, T(0.0), az.f));
+            const T q1 = lerpT(ax.fd, grad3T(p.d[p47 + 1], ax.f, T(0.0), az.fm1),
+                               grad3T(p.d[p20 + 1], ax.fm1, T(0.0), az.fm1));
+            const T val = lerpT(az.fd, q0, q1) * w;
+            acc[c] = o == 0 ? T(0.0) + val : acc[c] + val;
+        }
+        amp = amp / T(2.0);
+    }
+}
+
+template <class T> T taperT(T d8, int y) {
+    if (y > 13) {
+        const T d13 = T(static_cast<double>(static_cast<float>(y - 13) / 3.0F));
+        d8 = d8 * (T(1.0) - d13) + T(-10.0) * d13;
+    }
+    return d8;
+}
+
+template <class T> T densityFromT(T d9, T d12, T n2, T n3, int y) {
+    const T d10 = n2 / T(512.0), d11 = n3 / T(512.0);
+    T d8;
+    if (d12 < T(0.0)) d8 = d10;
+    else if (d12 > T(1.0)) d8 = d11;
+    else d8 = d10 + (d11 - d10) * d12;
+    return taperT(d8 - d9, y);
+}
+
+// Stage 1 of one seed over window p: open[c] (Stage 1a left column c open), found[c] (Stage 1b
+// found a candidate start there), lattice[c * 8 + k] the column's exact densities at levels 8..15
+// (plain evaluation only, for checking).  Stage 1a's operations go to *t1a, Stage 1b's to *t1b.
+template <class T>
+void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpTally* t1b, uint8_t* open,
+                uint8_t* found, double* lattice) {
+    std::vector<std::pair<int, int>> cols;
+    std::vector<std::pair<int, int>> blocks;
+    for (int ch = 0; ch < p.chunks_x * p.chunks_z; ++ch) {
+        const int cx = p.chunk_x0 + ch / p.chunks_z, cz = p.chunk_z0 + ch % p.chunks_z;
+        for (int sc = 0; sc < 4; ++sc) {
+            const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
+            cols.emplace_back(cx * 4 + i, cz * 4 + l);
+            blocks.emplace_back(cx * 16 + 3 * i + 1, cz * 16 + 3 * l + 1);
+        }
+    }
+    const size_t nc = cols.size();
+    // Stage 1a
+    t_tally = t1a;
+    std::vector<std::array<T, kS2Levels>> a2(nc), a3(nc), a1(nc);
+    std::vector<T> a4(nc), a5(nc);
+    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, 16, cols, a2);
+    octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, 16, cols, a3);
+    octaves3dAt<T>(t.n1, 684.412 / 80.0, 684.412 / 160.0, 0, 8, cols, a1);
+    octaves2dAt<T>(t.n4, 1.121, cols, a4);
+    octaves2dAt<T>(t.n5, 200.0, cols, a5);
+    const T u = T(2.0 * ((1 << kSplitOctave1) - 1)), slack = T(1e-6);
+    std::vector<std::array<T, kS2Levels>> d9s(nc), d12s(nc);
+    for (size_t c = 0; c < nc; ++c) {
+        const T tr = stage2ClimateGroup<T>(t.climate.e, 0.02500000037252903, 0.25, blocks[c].first, blocks[c].second);
+        const T rr = stage2ClimateGroup<T>(t.climate.f, 0.05000000074505806, 0.3333333333333333, blocks[c].first,
+                                           blocks[c].second);
+        const T cr = stage2ClimateGroup<T>(t.climate.g, 0.25, 0.5882352941176471, blocks[c].first, blocks[c].second);
+        const T c0 = cr * T(1.1) + T(0.5);
+        T d3 = (tr * T(0.15) + T(0.7)) * (T(1.0) - T(0.01)) + c0 * T(0.01);
+        T d4 = (rr * T(0.15) + T(0.5)) * (T(1.0) - T(0.0020)) + c0 * T(0.0020);
+        d3 = T(1.0) - (T(1.0) - d3) * (T(1.0) - d3);
+        if (d3 < T(0.0)) d3 = T(0.0);
+        if (d4 < T(0.0)) d4 = T(0.0);
+        if (d3 > T(1.0)) d3 = T(1.0);
+        if (d4 > T(1.0)) d4 = T(1.0);
+        const T rain = d4 * d3;
+        T e4 = T(1.0) - rain;
+        e4 = e4 * e4;
+        e4 = e4 * e4;
+        e4 = T(1.0) - e4;
+        T d5 = (a4[c] + T(256.0)) / T(512.0);
+        d5 = d5 * e4;
+        if (d5 > T(1.0)) d5 = T(1.0);
+        T d6 = a5[c] / T(8000.0);
+        if (d6 < T(0.0)) d6 = -d6 * T(0.3);
+        d6 = d6 * T(3.0) - T(2.0);
+        if (d6 < T(0.0)) {
+            d6 = d6 / T(2.0);
+            if (d6 < T(-1.0)) d6 = T(-1.0);
+            d6 = d6 / T(1.4);
+            d6 = d6 / T(2.0);
+            d5 = T(0.0);
+        } else {
+            if (d6 > T(1.0)) d6 = T(1.0);
+            d6 = d6 / T(8.0);
+        }
+        if (d5 < T(0.0)) d5 = T(0.0);
+        d5 = d5 + T(0.5);
+        d6 = d6 * T(17.0) / T(16.0);
+        const T d7 = T(17.0) / T(2.0) + d6 * T(4.0);
+        T lo[kS2Levels + 1], hi[kS2Levels + 1];
+        for (int k = 0; k < kS2Levels; ++k) {
+            const int y = kS2Y0 + k;
+            T d9 = (i2d<T>(y) - d7) * T(12.0) / d5;
+            if (d9 < T(0.0)) d9 = d9 * T(4.0);
+            const T d12 = (a1[c][k] / T(10.0) + T(1.0)) / T(2.0);
+            d9s[c][k] = d9;
+            d12s[c][k] = d12;
+            // density_bounds: for d12 in [0, 1] the blend is increasing in both sums, and the taper
+            // is increasing
+            const T l2 = (a2[c][k] - u) / T(512.0), h2 = (a2[c][k] + u) / T(512.0);
+            const T l3 = (a3[c][k] - u) / T(512.0), h3 = (a3[c][k] + u) / T(512.0);
+            T l, h;
+            if (d12 < T(0.0)) {
+                l = l2;
+                h = h2;
+            } else if (d12 > T(1.0)) {
+                l = l3;
+                h = h3;
+            } else {
+                l = l2 + (l3 - l2) * d12;
+                h = h2 + (h3 - h2) * d12;
+            }
+            lo[k] = taperT(l - d9, y) - slack;
+            hi[k] = taperT(h - d9, y) + slack;
+        }
+        lo[kS2Levels] = hi[kS2Levels] = T(-10.0);
+        bool op = false;
+        for (int k = 0; k < kS2Levels; ++k) {
+            const bool a = lo[k] <= T(0.0);
+            const bool b = hi[k + 1] > T(0.0);
+            op = op || (a && b);
+        }
+        open[c] = op ? 1 : 0;
+    }
+    // Stage 1b at the open columns
+    t_tally = t1b;
+    for (size_t c = 0; c < nc; ++c) {
+        found[c] = 0;
+        if (!open[c]) continue;
+        std::vector<std::pair<int, int>> one(1, cols[c]);
+        std::vector<std::array<T, kS2Levels>> b2(1), b3(1);
+        octaves3dAt<T>(t.n2, 684.412, 684.412, 0, 16, one, b2);
+        octaves3dAt<T>(t.n3, 684.412, 684.412, 0, 16, one, b3);
+        T lat[kS2Levels + 1];
+        for (int k = 0; k < kS2Levels; ++k) lat[k] = densityFromT(d9s[c][k], d12s[c][k], b2[0][k], b3[0][k], kS2Y0 + k);
+        lat[kS2Levels] = T(-10.0);
+        bool below = false, f = false;
+        for (int k1 = 0; k1 < kS2Levels; ++k1) {
+            T d1 = lat[k1];
+            const T d5 = (lat[k1 + 1] - d1) * T(0.125);
+            for (int l1 = 0; l1 < 8; ++l1) {
+                const bool stone = d1 > T(0.0);
+                if (64 + k1 * 8 + l1 >= kIslandFloorY && stone && !below) f = true;
+                below = stone;
+                d1 = d1 + d5;
+            }
+        }
+        found[c] = f ? 1 : 0;
+    }
+    t_tally = nullptr;
+    // the exact densities of every column, for checking (not counted)
+    if (lattice) {
+        OpTally scratch;
+        t_tally = &scratch;
+        std::vector<std::array<T, kS2Levels>> b2(nc), b3(nc);
+        octaves3dAt<T>(t.n2, 684.412, 684.412, 0, 16, cols, b2);
+        octaves3dAt<T>(t.n3, 684.412, 684.412, 0, 16, cols, b3);
+        for (size_t c = 0; c < nc; ++c)
+            for (int k = 0; k < kS2Levels; ++k)
+                lattice[c * kS2Levels + k] = raw(densityFromT(d9s[c][k], d12s[c][k], b2[c][k], b3[c][k], kS2Y0 + k));
+        t_tally = nullptr;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+void bsfo_stage1_op_count(int64_t seed, const bsfo_island_params* p, uint64_t* counts1a, uint64_t* counts1b,
+                          uint8_t* open_out, uint8_t* found_out, double* lattice_out) {
+    Terrain t(seed);
+    const int nc = p->chunks_x * p->chunks_z * 4;
+    OpTally a, b;
+    std::vector<uint8_t> open(nc), found(nc);
+    stage1Seed<CD>(t, *p, &a, &b, open.data(), found.data(), nullptr);
+    for (int i = 0; i < 2; ++i) {
+        const OpTally& tl = i == 0 ? a : b;
+        uint64_t* out = i == 0 ? counts1a : counts1b;
+        out[0] = tl.add;
+        out[1] = tl.mul;
+        out[2] = tl.div;
+        out[3] = tl.cmp;
+        out[4] = tl.cvt;
+    }
+    if (open_out || found_out || lattice_out) {
+        // the plain evaluation takes the same decisions
+        OpTally pa2, pb2;
+        std::vector<uint8_t> open2(nc), found2(nc);
+        std::vector<double> lat(static_cast<size_t>(nc) * kS2Levels);
+        stage1Seed<double>(t, *p, &pa2, &pb2, open2.data(), found2.data(), lat.data());
+        for (int c = 0; c < nc; ++c) {
+            if (open_out) open_out[c] = open[c] == open2[c] ? open[c] : 255;
+            if (found_out) found_out[c] = found[c] == found2[c] ? found[c] : 255;
+        }
+        if (lattice_out) std::memcpy(lattice_out, lat.data(), lat.size() * sizeof(double));
+    }
+}
 
 void bsfo_stage2_op_count(int64_t seed, const bsfo_island_params* p, uint64_t* counts5, double* lattice_out) {
     Terrain t(seed);

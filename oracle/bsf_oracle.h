@@ -185,6 +185,15 @@ void     bsfo_coarse_arrays(int64_t seed, int32_t coarse_offset_x, int32_t coars
                             double* n1, double* n2, double* n3, double* n4, double* n5,
                             double* temp, double* rain, uint8_t* signs);
 
+/* FP64 operations of island Stage 1 for one seed over window p (4 sample columns per chunk):
+ * counts1a / counts1b [add, mul, div, cmp, cvt] of Stage 1a (noise2 / noise3 octaves 10..15 with
+ * bounds for the rest, noise1, noise4/5, climate, column shape, per-level bias, blend and bounds)
+ * and of Stage 1b (all 16 octaves of noise2 / noise3, exact densities and the y-chain at each
+ * column Stage 1a leaves open).  open_out / found_out (optional, 4 per chunk) receive each
+ * column's decisions (255 where the counted and plain evaluations differ), lattice_out the
+ * columns' exact densities at levels 8..15 (8 per column). */
+void     bsfo_stage1_op_count(int64_t seed, const bsfo_island_params* p, uint64_t* counts1a, uint64_t* counts1b,
+                              uint8_t* open_out, uint8_t* found_out, double* lattice_out);
 /* FP64 operations of island Stage 2 for one seed over window p (the 40 3D octaves at levels
  * 8..15 and the 26 2D octaves of the (4w+1)(4h+1) lattice columns, climate, column shape and
  * densities of the 25wh density columns): counts5 = {add/sub, mul, div, compare, int<->double
