@@ -36,7 +36,7 @@ FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no
 # HBM bytes per unit of work of the FP64 kernels: FETCH_SIZE x 1.99975 (the factor by which
 # a 256 MiB device copy under-reads) + WRITE_SIZE, from the rocprofv3 --pmc runs summarised in
 # profiles/island_pipeline_4x4_v1134.md.
-HBM_BYTES_PER_UNIT = {"island_stage1": 52121, "island_stage2": 196257}
+HBM_BYTES_PER_UNIT = {"island_stage1": 50724, "island_stage2": 197030}
 
 
 T_START = time.perf_counter()
@@ -436,7 +436,7 @@ def run_serving(bsf, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=26)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=96 << 18,
                     help="seeds per step (default 96 passes of 2^18 seeds: 20 steps take about a minute)")

@@ -536,7 +536,9 @@ Tuning tuning_for(const bsf_island_params& p) {
     if (it != g_tuning.end()) return it->second;
     Tuning t;
     t.worlds_per_wave = default_worlds_per_wave(p);
-    t.gate = kGateGround;
+    // the ground gate costs about one 64-column noise batch per survivor: it pays where Stage 2
+    // evaluates several (windows of 9 chunks and more; the autotuner decides per shape)
+    t.gate = p.chunks_x * p.chunks_z >= 9 ? kGateGround : kGateStage1;
     return t;
 }
 

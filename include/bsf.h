@@ -162,8 +162,9 @@ int bsf_island_engine_stats(uint64_t* passes, uint64_t* seeds, uint64_t* survivo
                             uint64_t* stage2);
 
 /* Autotuner: per window shape (chunks_x, chunks_z), Stage-1 worlds per 64-lane wave, the gates
- * (gate = 0 sends every seed to Stage 2, 1 the seeds with a candidate start, 2 -- the default --
- * of those the ones with a start that is not grounded within its own chunk), and for windows up
+ * (gate = 0 sends every seed to Stage 2, 1 the seeds with a candidate start, 2 of those the ones
+ * with a start that is not grounded within its own chunk; untuned windows of 9 chunks and more
+ * use 2, smaller ones 1), and for windows up
  * to 4x4 chunks Stage 2's corner-gradient form.  bsf_island_autotune times the Stage-1
  * candidates, the best of them without the ground gate, then the other Stage-2 form with the
  * best, each over two runs of trial_seeds seeds from
