@@ -59,11 +59,11 @@ __device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __
 // Stager for one world per workgroup: every lane reads the same staged generator.  The
 // cache entry of the next generator (g + 1, the usual successor) is loaded into registers
 // while generator g is evaluated, so a stage only waits for its LDS stores.
-template <bool FMA_GRAD>
+template <int FORM>
 struct OneWorldT {
     static constexpr bool kUniform = true;
-    static constexpr bool kFmaGrad = FMA_GRAD;  // 3D corner gradients from st->gcoef
-    using Stage = std::conditional_t<FMA_GRAD, GenStageGc, GenStage>;
+    static constexpr int kGradForm = FORM;  // how 3D corner gradients are evaluated (GradForm)
+    using Stage = std::conditional_t<FORM == kGradTable, GenStageGc, GenStage>;
     Stage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
@@ -109,16 +109,19 @@ struct OneWorldT {
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
     __device__ __forceinline__ const uint2* gcoef() const {
-        if constexpr (FMA_GRAD) return st->gcoef;
+        if constexpr (FORM == kGradTable) return st->gcoef;
         else return nullptr;
     }
     __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
     __device__ __forceinline__ double off(int i) const { return st->off[i]; }
 };
 // OneWorld evaluates the 3D corners by case selection, OneWorldFma by the coefficient table
-// (fewer VALU instructions, one more LDS read per corner; which is faster depends on the kernel).
-using OneWorld = OneWorldT<false>;
-using OneWorldFma = OneWorldT<true>;
+// (fewer VALU instructions, one more LDS read per corner), OneWorldBits by bitfields of
+// constants (no LDS read; a few more VALU instructions than the table).  Which is fastest
+// depends on the kernel.
+using OneWorld = OneWorldT<kGradSelect>;
+using OneWorldFma = OneWorldT<kGradTable>;
+using OneWorldBits = OneWorldT<kGradBits>;
 
 // Stager for up to kMaxWorldsPerWave consecutive cache slots in one 64-lane workgroup: lane
 // group `sub` evaluates world first_slot + sub.  Used when a world needs few columns.
@@ -181,7 +184,7 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
                 const Axis ax = axis(X * sx + s.off(0));
                 const Axis az = axis(Z * sx + s.off(2));
                 double v[NY];
-                perlin3_column_table<Y0, Y1>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
+                perlin3_column_table<Y0, Y1, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
                 for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
@@ -217,7 +220,7 @@ __device__ __forceinline__ void colpar_3d_n(const S& s, int g0, int octaves, dou
             const Axis ax = axis(X * sx + s.off(0));
             const Axis az = axis(Z * sx + s.off(2));
             double v[NY];
-            perlin3_column_table<0, NY>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
+            perlin3_column_table<0, NY, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
             for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
         }

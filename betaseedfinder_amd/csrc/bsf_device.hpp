@@ -36,12 +36,15 @@ __host__ __device__ __forceinline__ uint64_t splitmix_seed(uint64_t sequence_see
 // Division table for Random.nextInt(bound), bound in [1, 256] (filled by the host).
 // For a non-power-of-two bound d with 2^k < d < 2^(k+1): m = ceil(2^(32+k) / d) and
 // floor(n / d) == umulhi(n, m) >> k for every n < 2^31 (error term n*(m - 2^(32+k)/d)
-// / 2^(32+k) < 2^-(k+1) < 1/d).  For d = 2^k, shift holds 31 - k and m is 0.
+// / 2^(32+k) < 2^-(k+1) < 1/d).  One formula serves both kinds of bound:
+//   nextInt(d) = (bits >> pre) - (umulhi(bits, m) >> shift) * d
+// is bits mod d for a non-power-of-two d (pre = 0) and Java's (d * bits) >> 31 = bits >> (31 - k)
+// for d = 2^k (pre = 31 - k, m = 0).
 struct DivMagic {
     uint32_t m;
     uint32_t shift;
     uint32_t limit;  // nextInt(d) redraws bits >= limit: d * floor(2^31 / d), 2^31 (never) for d = 2^k
-    uint32_t pad;
+    uint32_t pre;
 };
 
 // The table for d = 1 .. 256 (entry 0 unused).  Java redraws when bits - bits % d + (d - 1)
@@ -53,7 +56,7 @@ inline void fill_div_magic(DivMagic* t) {
         uint32_t k = 0;
         while ((2u << k) <= d) ++k;  // 2^k <= d < 2^(k+1)
         if ((d & (d - 1)) == 0) {
-            t[d].shift = 31 - k;
+            t[d].pre = 31 - k;
         } else {
             t[d].m = static_cast<uint32_t>(((1ULL << (32 + k)) + d - 1) / d);
             t[d].shift = k;
@@ -61,6 +64,20 @@ inline void fill_div_magic(DivMagic* t) {
         }
     }
 }
+
+// nextInt(bound) of bits that are not redrawn (either kind of bound).
+__host__ __device__ __forceinline__ uint32_t int_from_bits(uint32_t bits, uint32_t bound, DivMagic mg) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
+#else
+    const uint32_t q = static_cast<uint32_t>((static_cast<uint64_t>(bits) * mg.m) >> 32) >> mg.shift;
+#endif
+    return (bits >> mg.pre) - q * bound;
+}
+
+// A draw whose bits are below this is never redrawn, whatever the bound (every limit is at
+// least 2^31 - 255).
+constexpr uint32_t kNoRedrawBelow = 0x80000000u - 256u;
 
 // ---------------------------------------------------------------------------
 // java.util.Random, 48-bit LCG (call sites NoiseGeneratorPerlin173.java:63-78,
@@ -87,11 +104,10 @@ struct Lcg {
 // One nextInt(bound) draw for a wave-uniform bound, Random.nextInt(int) (JDK 17).
 __device__ __forceinline__ uint32_t next_int_uniform_bound(Lcg& r, uint32_t bound, DivMagic mg) {
     uint32_t bits = r.next(31);
-    if ((bound & (bound - 1)) == 0) return bits >> mg.shift;  // (bound * bits) >> 31
-    // Rejection when bits - val + (bound - 1) overflows int32 (probability < bound / 2^31).
+    // Rejection when bits - val + (bound - 1) overflows int32 (probability < bound / 2^31);
+    // never for a power of two (limit 2^31).
     while (bits >= mg.limit) bits = r.next(31);
-    const uint32_t q = __umulhi(bits, mg.m) >> mg.shift;
-    return bits - q * bound;
+    return int_from_bits(bits, bound, mg);
 }
 
 // ---------------------------------------------------------------------------
@@ -121,81 +137,104 @@ struct GenOffsets {
     double a, b, c;
 };
 
-// Two consecutive nextInt draws, bounds b0 then b1.  The two LCG states are formed
-// independently (s1 = A s + C, s2 = A^2 s + (A + 1) C, mod 2^48); if either draw would be
-// rejected (probability < 2^-23) the pair is redone one draw at a time, as Java does.
+// The LCG on S = s << 16: the state's 48 bits sit at the top of a 64-bit word, so the mod 2^48 of
+// A s + C is the natural wrap of A S + (C << 16) mod 2^64 (only A mod 2^48 matters, S's low 16
+// bits being zero), next(31) = S >> 33, and no mask is applied.  Two draws at once: S1 = A S + C',
+// S2 = A^2 S + (A + 1) C' (mod 2^64), formed side by side.
 constexpr uint64_t kLcgMul2 = (kLcgMul * kLcgMul) & kLcgMask;
 constexpr uint64_t kLcgAdd2 = (kLcgMul * kLcgAdd + kLcgAdd) & kLcgMask;
+constexpr uint64_t kLcgAddS = kLcgAdd << 16, kLcgAdd2S = kLcgAdd2 << 16;
 
-// nextInt(bound) of bits that are not rejected.
-__device__ __forceinline__ uint32_t int_from_bits(uint32_t bits, uint32_t bound, DivMagic mg) {
-    if ((bound & (bound - 1)) == 0) return bits >> mg.shift;
-    return bits - (__umulhi(bits, mg.m) >> mg.shift) * bound;
-}
-
-// Two consecutive nextInt draws, bounds b0 then b1, from states s1 = A s + C and
-// s2 = A^2 s + (A + 1) C formed side by side.  When a draw of any lane of the wave would be
-// rejected (probability < 2^-23 per draw) the whole wave redoes the pair one draw at a time,
-// as Java does, so the common path has no per-lane branch.
-__device__ __forceinline__ void next_two_ints(Lcg& r, uint32_t b0, DivMagic m0, uint32_t b1, DivMagic m1,
-                                              uint32_t& v0, uint32_t& v1) {
-    const uint64_t s1 = (r.s * kLcgMul + kLcgAdd) & kLcgMask;
-    const uint64_t s2 = (r.s * kLcgMul2 + kLcgAdd2) & kLcgMask;
-    const uint32_t u0 = static_cast<uint32_t>(s1 >> 17), u1 = static_cast<uint32_t>(s2 >> 17);
-    if (__builtin_expect(__any(u0 >= m0.limit || u1 >= m1.limit), 0)) {
-        v0 = next_int_uniform_bound(r, b0, m0);
-        v1 = next_int_uniform_bound(r, b1, m1);
-        return;
+// The exact shuffle, one draw at a time with Java's redraws (the identity table first).
+__device__ __noinline__ void shuffle_exact(Lcg& r, LanePerm p, const DivMagic* __restrict__ magic) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(p.base);
+    for (int k = 0; k < 64; ++k) w[k * 64] = 0x03020100u + 0x04040404u * static_cast<uint32_t>(k);
+    for (int i = 0; i < 256; ++i) {
+        const uint32_t b = 256u - static_cast<uint32_t>(i);
+        const int j = i + static_cast<int>(next_int_uniform_bound(r, b, magic[b]));
+        const uint8_t vi = static_cast<uint8_t>(p.get(i)), vj = static_cast<uint8_t>(p.get(j));
+        p.set(i, vj);
+        p.set(j, vi);
     }
-    v0 = int_from_bits(u0, b0, m0);
-    v1 = int_from_bits(u1, b1, m1);
-    r.s = s2;
 }
 
 // The shuffle walks two steps (i, i + 1) per LDS round trip: the four bytes are read first
 // and the effect of swap i on the reads of swap i + 1 is applied in registers (j1 >= i + 1,
-// so swap i can only have moved p[i + 1] (when j0 == i + 1) or p[j1] (when j1 == j0)).
+// so swap i can only have moved p[i + 1] (when j0 == i + 1) or p[j1] (when j1 == j0)).  Draws
+// are taken as if none were redrawn; the largest bits of the lane's draws are kept, and if any
+// lane of the wave drew bits that some bound redraws (probability 2^-15 per build and lane), the
+// wave builds the table again from the saved state one exact draw at a time.  Eight steps per
+// loop iteration keep the division constants in scalar registers and the loop overhead small.
 __device__ __forceinline__ GenOffsets build_generator(Lcg& r, LanePerm p, const DivMagic* __restrict__ magic) {
     GenOffsets o;
     o.a = r.next_double() * 256.0;
     o.b = r.next_double() * 256.0;
     o.c = r.next_double() * 256.0;
+    const uint64_t s0 = r.s;
     // identity permutation, four bytes per dword
     uint32_t* w = reinterpret_cast<uint32_t*>(p.base);
 #pragma unroll 8
     for (int k = 0; k < 64; ++k) w[k * 64] = 0x03020100u + 0x04040404u * static_cast<uint32_t>(k);
-#pragma unroll 2
-    for (int i = 0; i < 256; i += 2) {
-        const uint32_t b0 = 256u - static_cast<uint32_t>(i), b1 = b0 - 1u;
-        uint32_t j0, j1;
-        next_two_ints(r, b0, magic[b0], b1, magic[b1], j0, j1);
-        j0 += static_cast<uint32_t>(i);
-        j1 += static_cast<uint32_t>(i) + 1u;
-        uint8_t* pi0 = p.base + LanePerm::off(static_cast<uint32_t>(i));
-        uint8_t* pj0 = p.base + LanePerm::off(j0);
-        uint8_t* pi1 = p.base + LanePerm::off(static_cast<uint32_t>(i) + 1u);
-        uint8_t* pj1 = p.base + LanePerm::off(j1);
-        const uint8_t vi0 = *pi0, vj0 = *pj0, vi1 = *pi1, vj1 = *pj1;
-        const uint8_t a = j0 == static_cast<uint32_t>(i) + 1u ? vi0 : vi1;  // p[i + 1] after swap i
-        const uint8_t c = j1 == j0 ? vi0 : vj1;                              // p[j1] after swap i
-        *pi0 = vj0;
-        *pj0 = vi0;
-        *pi1 = c;
-        *pj1 = a;
+    uint64_t S = s0 << 16;
+    uint32_t top = 0;
+#pragma unroll 1
+    for (int i8 = 0; i8 < 256; i8 += 8) {
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) {
+            const uint32_t i = static_cast<uint32_t>(i8 + c);
+            const uint32_t b0 = 256u - i, b1 = b0 - 1u;
+            const uint64_t S1 = S * kLcgMul + kLcgAddS;
+            const uint64_t S2 = S * kLcgMul2 + kLcgAdd2S;
+            S = S2;
+            const uint32_t u0 = static_cast<uint32_t>(S1 >> 33), u1 = static_cast<uint32_t>(S2 >> 33);
+            top = max(top, max(u0, u1));
+            const uint32_t j0 = i + int_from_bits(u0, b0, magic[b0]);
+            const uint32_t j1 = i + 1u + int_from_bits(u1, b1, magic[b1]);
+            uint8_t* pi0 = p.base + LanePerm::off(i);
+            uint8_t* pj0 = p.base + LanePerm::off(j0);
+            uint8_t* pi1 = p.base + LanePerm::off(i + 1u);
+            uint8_t* pj1 = p.base + LanePerm::off(j1);
+            const uint8_t vi0 = *pi0, vj0 = *pj0, vi1 = *pi1, vj1 = *pj1;
+            const uint8_t a = j0 == i + 1u ? vi0 : vi1;  // p[i + 1] after swap i
+            const uint8_t cc = j1 == j0 ? vi0 : vj1;     // p[j1] after swap i
+            *pi0 = vj0;
+            *pj0 = vi0;
+            *pi1 = cc;
+            *pj1 = a;
+        }
+    }
+    if (__builtin_expect(__any(top >= kNoRedrawBelow), 0)) {
+        r.s = s0;
+        shuffle_exact(r, p, magic);
+    } else {
+        r.s = S >> 16;
     }
     return o;
 }
 
 // Advance the stream exactly like one Perlin constructor without keeping the table
 // (NoiseGeneratorPerlin173.consumeConstructorRandom, :81-88).
-// Only the draws' count matters here, so each draw is just a state step and the rejection test.
+// Only the draws' count matters here: the 256 draws are stepped as if none were redrawn, and if
+// any lane's bits reached a redraw, the wave walks them again one draw at a time with Java's test.
 __device__ __forceinline__ void consume_generator(Lcg& r, const DivMagic* __restrict__ magic) {
     r.step(); r.step(); r.step(); r.step(); r.step(); r.step();  // three nextDouble()
-#pragma unroll 4
+    const uint64_t s0 = r.s;
+    uint64_t S = s0 << 16;
+    uint32_t top = 0;
+#pragma unroll 8
     for (int i = 0; i < 256; ++i) {
-        const uint32_t limit = magic[256u - static_cast<uint32_t>(i)].limit;
-        while (r.next(31) >= limit) {
+        S = S * kLcgMul + kLcgAddS;
+        top = max(top, static_cast<uint32_t>(S >> 33));
+    }
+    if (__builtin_expect(__any(top >= kNoRedrawBelow), 0)) {
+        r.s = s0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t limit = magic[256u - static_cast<uint32_t>(i)].limit;
+            while (r.next(31) >= limit) {
+            }
         }
+    } else {
+        r.s = S >> 16;
     }
 }
 
@@ -245,6 +284,49 @@ __device__ __forceinline__ double grad3_fma(uint2 c, double x, double y, double 
     const double b = __hiloint2double(static_cast<int>(c.y), 0);
     const double cz = __hiloint2double(static_cast<int>(c.x << 20), 0);
     return __fma_rn(a, x, __fma_rn(b, y, cz * z));
+}
+
+// The same coefficients as 2-bit two's-complement fields (01 = +1, 11 = -1, 00 = 0), the field
+// of h = 0..15 at bits [2h, 2h + 1] of one 32-bit constant per axis (which: 0 = x, 1 = y, 2 = z).
+__host__ __device__ constexpr uint32_t grad3_field(int which) {
+    uint32_t k = 0;
+    for (int h = 0; h < 16; ++h) {
+        const int u = h < 8 ? 0 : 1;
+        const int v = h < 4 ? 1 : ((h == 12 || h == 14) ? 0 : 2);
+        int c = 0;
+        if (u == which) c = (h & 1) ? -1 : 1;
+        if (v == which) c = (h & 2) ? -1 : 1;  // u != v for every h
+        k |= (static_cast<uint32_t>(c) & 3u) << (2 * h);
+    }
+    return k;
+}
+constexpr uint32_t kGrad3FieldX = grad3_field(0), kGrad3FieldY = grad3_field(1), kGrad3FieldZ = grad3_field(2);
+
+// grad3_fma with the coefficients extracted from those constants in registers (three signed
+// bitfield extracts and three int -> double conversions) instead of read from an LDS table.
+// The field offset is 2h; the extract uses its low 5 bits, which are 2 (h & 15).  Exact under
+// the same condition as grad3_fma: all three coordinates nonzero.
+// (The builtin's result is converted as a signed int: -1 must become -1.0.)
+__device__ __forceinline__ int grad3_field_at(uint32_t field, uint32_t o) {
+    return static_cast<int>(__builtin_amdgcn_sbfe(static_cast<int>(field), o, 2u));
+}
+__device__ __forceinline__ double grad3_bits(int h, double x, double y, double z) {
+    const uint32_t o = static_cast<uint32_t>(h) << 1;
+    const double a = static_cast<double>(grad3_field_at(kGrad3FieldX, o));
+    const double b = static_cast<double>(grad3_field_at(kGrad3FieldY, o));
+    const double c = static_cast<double>(grad3_field_at(kGrad3FieldZ, o));
+    return __fma_rn(a, x, __fma_rn(b, y, c * z));
+}
+
+// How a kernel evaluates 3D corner gradients: Java's case selection, the LDS coefficient table
+// (grad3_fma) or the register bitfields (grad3_bits).  The last two need nonzero coordinates.
+enum GradForm : int { kGradSelect = 0, kGradTable = 1, kGradBits = 2 };
+
+template <int FORM, class Perm>
+__device__ __forceinline__ double corner_grad(Perm p, const uint2* gc, int idx, double x, double y, double z) {
+    if constexpr (FORM == kGradTable) return grad3_fma(gc[p.get(idx)], x, y, z);
+    else if constexpr (FORM == kGradBits) return grad3_bits(p.get(idx), x, y, z);
+    else return grad3(p.get(idx), x, y, z);
 }
 
 // a(int, x, z) (2-argument legacy gradient, :123-143) used by the yLen == 1 corner 0: Java
@@ -484,7 +566,7 @@ __device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
 // With gc (the grad3_coefficients table of the 256 permutation bytes, in LDS) the corner
 // gradients are three FP64 ops each instead of the case selection, whenever no corner
 // coordinate of the wave's cells is zero (checked per call; the selection path otherwise).
-template <int Y0, int Y1, bool FMA, class Perm>
+template <int Y0, int Y1, int FORM, class Perm>
 __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
                                                     const YTable& yt, uint32_t fresh, double weight, double* out) {
     const int xp0 = p.get(ax.i);
@@ -502,21 +584,14 @@ __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, con
             const int p1 = xp1 + yi;
             const int p10 = p.get(p1) + az.i;
             const int p11 = p.get(p1 + 1) + az.i;
-            if constexpr (FMA) {
-                q00 = lerp(ax.fd, grad3_fma(gc[p.get(p00)], ax.f, yf, az.f), grad3_fma(gc[p.get(p10)], ax.fm1, yf, az.f));
-                q01 = lerp(ax.fd, grad3_fma(gc[p.get(p01)], ax.f, yfm1, az.f),
-                           grad3_fma(gc[p.get(p11)], ax.fm1, yfm1, az.f));
-                q10 = lerp(ax.fd, grad3_fma(gc[p.get(p00 + 1)], ax.f, yf, az.fm1),
-                           grad3_fma(gc[p.get(p10 + 1)], ax.fm1, yf, az.fm1));
-                q11 = lerp(ax.fd, grad3_fma(gc[p.get(p01 + 1)], ax.f, yfm1, az.fm1),
-                           grad3_fma(gc[p.get(p11 + 1)], ax.fm1, yfm1, az.fm1));
-            } else {
-                q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
-                q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
-                q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
-                q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1),
-                           grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
-            }
+            q00 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00, ax.f, yf, az.f),
+                       corner_grad<FORM>(p, gc, p10, ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01, ax.f, yfm1, az.f),
+                       corner_grad<FORM>(p, gc, p11, ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00 + 1, ax.f, yf, az.fm1),
+                       corner_grad<FORM>(p, gc, p10 + 1, ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01 + 1, ax.f, yfm1, az.fm1),
+                       corner_grad<FORM>(p, gc, p11 + 1, ax.fm1, yfm1, az.fm1));
         }
         const double fd = yt.fd[k];
         const double r0 = lerp(fd, q00, q01);
@@ -524,15 +599,20 @@ __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, con
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
 }
-template <int Y0, int Y1, class Perm>
+// FORM kGradTable needs gc (the grad3_coefficients table); the table and bitfield forms fall
+// back to the case selection for a call with a zero corner coordinate somewhere in the wave.
+template <int Y0, int Y1, int FORM, class Perm>
 __device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YTable& yt,
                                                      uint32_t fresh, double weight, double* out,
                                                      const uint2* gc = nullptr) {
     // x and z cells start at ax.f / az.f; their other corners (f - 1) are never zero
-    if (gc != nullptr && (yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0))
-        perlin3_table_cells<Y0, Y1, true>(p, gc, ax, az, yt, fresh, weight, out);
-    else
-        perlin3_table_cells<Y0, Y1, false>(p, gc, ax, az, yt, fresh, weight, out);
+    if constexpr (FORM != kGradSelect) {
+        if ((yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0)) {
+            perlin3_table_cells<Y0, Y1, FORM>(p, gc, ax, az, yt, fresh, weight, out);
+            return;
+        }
+    }
+    perlin3_table_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -361,7 +361,7 @@ __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
         const double* offs = a.offsets + static_cast<size_t>(slot_first) * kGenCount * 3;
         if (KIND == 0) {
             // Stage 1a: open columns go to Stage 1b, which appends the survivors
-            const OneWorld s{&st, perms, offs};
+            const OneWorldBits s{&st, perms, offs};
             const int64_t seed = job_seed(job, g.start + (slot_first - g.slot0));
             for (int base = 0; base < ncols; base += 64) {
                 const int c = base + tid;
@@ -588,12 +588,11 @@ __device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
 }
 
 __global__ __launch_bounds__(kGroundThreads, 3) void pass_ground_kernel(EngineArgs a) {
-    __shared__ OneWorldFma::Stage st;
+    __shared__ OneWorldBits::Stage st;
     __shared__ double s_dens[2][25 * 8];
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
     if (a.audit == kAuditStage1) return;
-    init_gcoef(st.gcoef);
     unsigned long long nsurv = a.ctr[kCtrSurv];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
     for (;;) {
@@ -608,7 +607,7 @@ __global__ __launch_bounds__(kGroundThreads, 3) void pass_ground_kernel(EngineAr
         if (job.gate == kGateGround) {
             const bsf_island_params p = job.p;
             const int cz_n = p.chunks_z;
-            const OneWorldFma s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+            const OneWorldBits s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
                                 a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
             unsigned long long mask = a.surv_mask[slot];
             keep = false;
@@ -659,12 +658,11 @@ constexpr int kS2Threads = 64;
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
 template <bool FMA, bool FILTER>
 __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a) {
-    __shared__ typename OneWorldT<FMA>::Stage st;
+    __shared__ typename OneWorldT<FMA ? kGradBits : kGradSelect>::Stage st;
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
     if (a.audit != kAuditNone) return;
-    if constexpr (FMA) init_gcoef(st.gcoef);
     // the survivors that the ground gate kept
     unsigned long long nsurv = a.ctr[kCtrKept];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
@@ -688,7 +686,7 @@ __global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;
         const int nc = (4 * cx_n + 1) * nzl;
-        const OneWorldT<FMA> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+        const OneWorldT<FMA ? kGradBits : kGradSelect> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
                                a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
         double* lat = a.lattice + static_cast<size_t>(it) * a.lat_stride;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)

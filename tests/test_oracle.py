@@ -503,6 +503,60 @@ def test_nextint_redraw_threshold_matches_java_test():
             assert java == (b >= limit), (d, b)
 
 
+def test_nextint_single_formula_matches_java():
+    """The kernels draw nextInt(d) for both kinds of bound with one formula,
+    (bits >> pre) - (umulhi(bits, m) >> shift) * d, with m = ceil(2^(32+k) / d), shift = k,
+    pre = 0 for 2^k < d < 2^(k+1) and m = 0, shift = 0, pre = 31 - k for d = 2^k; Java's
+    Random.nextInt takes (d * bits) >> 31 for a power of two and bits % d otherwise.  Also every
+    redraw limit is at least 2^31 - 256, the threshold below which a draw is taken as final."""
+    import random
+    rng = random.Random(11)
+    for d in range(1, 257):
+        k = d.bit_length() - 1
+        if d & (d - 1) == 0:
+            m, shift, pre, limit = 0, 0, 31 - k, 1 << 31
+        else:
+            m, shift, pre, limit = -(-(1 << (32 + k)) // d), k, 0, d * ((1 << 31) // d)
+        assert limit >= (1 << 31) - 256
+        cand = [0, 1, d - 1, d, d + 1, limit - 1, (1 << 31) - 257] + [rng.randrange(limit) for _ in range(400)]
+        for b in cand:
+            if b >= limit:
+                continue
+            java = (d * b) >> 31 if d & (d - 1) == 0 else b % d
+            assert (b >> pre) - (((b * m) >> 32) >> shift) * d == java, (d, b)
+
+
+def test_grad3_bitfields_decode_to_java_grad():
+    """Corner gradients from 2-bit two's-complement fields: per axis a 32-bit word holds the
+    coefficient of every h = 0..15 at bits [2h, 2h + 1] (01 = +1, 11 = -1, 00 = 0); a signed
+    extract at offset 2h (the hardware takes the offset's low 5 bits, so (h << 1) of a hash
+    byte works) decodes them.  The words are built here from Java's grad rule
+    (NoiseGeneratorPerlin173.java:145-165) evaluated on unit vectors, and every hash byte's
+    decoded coefficients must give Java's gradient on those vectors."""
+    def java_grad(h, x, y, z):
+        h &= 15
+        u = x if h < 8 else y
+        v = y if h < 4 else (x if h in (12, 14) else z)
+        return (-u if h & 1 else u) + (-v if h & 2 else v)
+
+    words = [0, 0, 0]
+    for h in range(16):
+        for axis, e in enumerate(((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))):
+            c = int(java_grad(h, *e))
+            words[axis] |= (c & 3) << (2 * h)
+
+    def sbfe(word, offset, width):
+        v = (word >> (offset & 31)) & ((1 << width) - 1)
+        return v - (1 << width) if v >> (width - 1) else v
+
+    for hb in range(256):
+        a, b, c = (sbfe(w, hb << 1, 2) for w in words)
+        for x, y, z in ((0.75, 0.5, 0.25), (-0.3, 0.9, -0.6), (0.125, -0.875, 0.5)):
+            assert a * x + b * y + c * z == java_grad(hb, x, y, z), (hb, x, y, z)
+    # the constants the device code derives (grad3_field in bsf_device.hpp) are these words
+    assert words == [0x3100DDDD, 0xDDDD00F5, 0xC4F5F500]
+
+
 def test_scout_statistic_is_octave_zero_sum(oracle):
     """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
     generator's octave-0 values (0.0 + v), index order x, z, y."""
