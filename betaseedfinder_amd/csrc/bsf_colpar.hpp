@@ -22,12 +22,15 @@ namespace bsf {
 constexpr int kGenN2 = 0, kGenN3 = 16, kGenN1 = 32, kGenN4 = 40, kGenN5 = 50;
 constexpr int kGenTemp = 66, kGenRain = 70, kGenBlend = 74, kGenCount = 76;
 
-// A staged generator: the 512-entry permutation (the cached 256 bytes twice, see SharedPerm)
-// and the three offsets.
+// Staged generators: up to kStageBatch consecutive cache entries, each as its 512-entry
+// permutation (the cached 256 bytes twice, see SharedPerm) and three offsets.  A 3D octave
+// stages one generator (slot 0) with its Y axis; the 2D and climate groups, whose octaves
+// need little work each, stage four at a time behind one pair of barriers.
+constexpr int kStageBatch = 4;
 struct GenStage {
-    uint32_t perm[128];
-    double off[3];
-    YTable yt;  // Y axis of the staged 3D octave (stage_generator_y)
+    uint32_t perm[kStageBatch][128];
+    double off[kStageBatch * 3];  // slot q's offsets at [3q, 3q + 3)
+    YTable yt;                    // Y axis of the staged 3D octave (stage_yn)
 };
 // The same with the kernel's grad3_coefficients table (OneWorldFma; filled by init_gcoef).
 struct GenStageGc : GenStage {
@@ -42,23 +45,9 @@ __device__ __forceinline__ void init_gcoef(uint2* t) {
     __syncthreads();
 }
 
-// Stage generator g of the current world into LDS (every thread of the block must call this).
-__device__ __forceinline__ void stage_generator(GenStage& st, const uint32_t* __restrict__ perms,
-                                                const double* __restrict__ offs, int g) {
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < 64) {
-        const uint32_t v = perms[g * 64 + t];
-        st.perm[t] = v;
-        st.perm[t + 64] = v;
-    }
-    if (t < 3) st.off[t] = offs[g * 3 + t];
-    __syncthreads();
-}
-
-// Stager for one world per workgroup: every lane reads the same staged generator.  The
-// cache entry of the next generator (g + 1, the usual successor) is loaded into registers
-// while generator g is evaluated, so a stage only waits for its LDS stores.
+// Stager for one world per workgroup: every lane reads the same staged generators.  The
+// cache entries of the next generators (the usual successors) are loaded into registers
+// while the current ones are evaluated, so a stage only waits for its LDS stores.
 template <int FORM>
 struct OneWorldT {
     static constexpr bool kUniform = true;
@@ -67,53 +56,72 @@ struct OneWorldT {
     Stage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
-    mutable int pf_g = -1;  // generator held in the prefetch registers
-    mutable uint32_t pf_perm = 0;
-    mutable double pf_off = 0.0;  // offset[threadIdx.x] for threads 0..2
-    mutable double pf_b = 0.0;    // offset[1] in every thread (the y offset)
+    mutable int pf_g = -1;  // first generator held in the prefetch registers
+    mutable int pf_n = 0;   // and how many
+    mutable uint32_t pf_perm[kStageBatch] = {};
+    mutable double pf_off = 0.0;  // offsets[pf_g * 3 + t] for threads t < 3 * pf_n (consecutive entries)
+    mutable double pf_b = 0.0;    // offset[1] of generator pf_g in every thread (the y offset)
 
-    __device__ __forceinline__ void fetch(int g) const {
+    __device__ __forceinline__ void fetch(int g, int n) const {
         const int t = threadIdx.x;
-        if (t < 64) pf_perm = perms[g * 64 + t];
-        pf_off = offs[g * 3 + (t < 3 ? t : 0)];
+        if (t < 64) {
+#pragma unroll
+            for (int q = 0; q < kStageBatch; ++q)
+                if (q < n) pf_perm[q] = perms[(g + q) * 64 + t];
+        }
+        pf_off = offs[g * 3 + (t < 3 * n ? t : 0)];
         pf_b = offs[g * 3 + 1];
         pf_g = g;
+        pf_n = n;
     }
-    __device__ __forceinline__ void publish(int g) const {
-        if (pf_g != g) fetch(g);
+    __device__ __forceinline__ void prefetch_after(int g, int next_n) const {
+        if (g < kGenCount) fetch(g, next_n < kGenCount - g ? next_n : kGenCount - g);
+    }
+    __device__ __forceinline__ void publish(int g, int n) const {
+        if (pf_g != g || pf_n < n) fetch(g, n);
         __syncthreads();
         const int t = threadIdx.x;
-
         if (t < 64) {
-            st->perm[t] = pf_perm;
-            st->perm[t + 64] = pf_perm;
+#pragma unroll
+            for (int q = 0; q < kStageBatch; ++q) {
+                if (q < n) {
+                    st->perm[q][t] = pf_perm[q];
+                    st->perm[q][t + 64] = pf_perm[q];
+                }
+            }
         }
-        if (t < 3) st->off[t] = pf_off;
+        if (t < 3 * n) st->off[t] = pf_off;
     }
-    __device__ __forceinline__ void stage(int g) const {
-        publish(g);
+    // Generators g .. g + n - 1 (n <= kStageBatch) into slots 0 .. n - 1; then next_n generators
+    // from next_g (default: the ones that follow) are prefetched.
+    __device__ __forceinline__ void stage_batch(int g, int n, int next_n = kStageBatch, int next_g = -1) const {
+        publish(g, n);
         __syncthreads();
-        if (g + 1 < kGenCount) fetch(g + 1);
+        prefetch_after(next_g < 0 ? g + n : next_g, next_n);
     }
+    __device__ __forceinline__ void stage(int g) const { stage_batch(g, 1, 1); }
     // Generator g of a 3D octave with y scale sy, and its Y-axis table over [y0, y0 + NY).
     template <int NY>
-    __device__ __forceinline__ void stage_yn(int g, double sy, int y0) const {
-        publish(g);
+    __device__ __forceinline__ void stage_yn(int g, double sy, int y0, int next_n = 1, int next_g = -1) const {
+        publish(g, 1);
         if (threadIdx.x < 64) build_ytable_n<NY>(st->yt, sy, pf_b, y0);
         __syncthreads();
-        if (g + 1 < kGenCount) fetch(g + 1);
+        prefetch_after(next_g < 0 ? g + 1 : next_g, next_n);
     }
     template <int Y0, int Y1>
-    __device__ __forceinline__ void stage_y(int g, double sy) const {
-        stage_yn<Y1 - Y0>(g, sy, Y0);
+    __device__ __forceinline__ void stage_y(int g, double sy, int next_n = 1, int next_g = -1) const {
+        stage_yn<Y1 - Y0>(g, sy, Y0, next_n, next_g);
     }
     __device__ __forceinline__ const YTable& ytable() const { return st->yt; }
     __device__ __forceinline__ const uint2* gcoef() const {
         if constexpr (FORM == kGradTable) return st->gcoef;
         else return nullptr;
     }
-    __device__ __forceinline__ SharedPerm perm() const { return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm)}; }
+    __device__ __forceinline__ SharedPerm perm(int q = 0) const {
+        return SharedPerm{reinterpret_cast<const uint8_t*>(st->perm[q])};
+    }
     __device__ __forceinline__ double off(int i) const { return st->off[i]; }
+    __device__ __forceinline__ double off(int q, int i) const { return st->off[3 * q + i]; }
 };
 // OneWorld evaluates the 3D corners by case selection, OneWorldFma by the coefficient table
 // (fewer VALU instructions, one more LDS read per corner), OneWorldBits by bitfields of
@@ -170,14 +178,14 @@ template <int Y0, int Y1, class S>
 __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
                                           double Z, bool active, double* acc, int o0 = 0) {
     constexpr int NY = Y1 - Y0;
-    double amp = 1.0;
-    for (int o = 0; o < o0; ++o) amp /= 2.0;
     for (int o = o0; o < octaves; ++o) {
-        const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
+        // amp = 2^-o (halved exactly per octave): the scales s * amp and the weight 1 / amp = 2^o
+        // are exact power-of-two scalings
+        const double sx = ldexp(s_xz, -o), sy = ldexp(s_y, -o), w = ldexp(1.0, o);
         if constexpr (S::kUniform) {
             // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
             // are the same on every lane, so the corner-reuse branch is scalar
-            s.template stage_y<Y0, Y1>(g0 + o, sy);
+            s.template stage_y<Y0, Y1>(g0 + o, sy, o + 1 < octaves ? 1 : kStageBatch);
             if (active) {
                 const YTable& yt = s.ytable();
                 const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
@@ -201,7 +209,6 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
                 for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
         }
-        amp /= 2.0;
     }
 }
 
@@ -210,10 +217,9 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
 template <int NY, class S>
 __device__ __forceinline__ void colpar_3d_n(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
                                             double Z, bool active, double* acc, int y0) {
-    double amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
-        const double sx = s_xz * amp, sy = s_y * amp, w = 1.0 / amp;
-        s.template stage_yn<NY>(g0 + o, sy, y0);
+        const double sx = ldexp(s_xz, -o), sy = ldexp(s_y, -o), w = ldexp(1.0, o);
+        s.template stage_yn<NY>(g0 + o, sy, y0, o + 1 < octaves ? 1 : kStageBatch);
         if (active) {
             const YTable& yt = s.ytable();
             const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
@@ -224,58 +230,104 @@ __device__ __forceinline__ void colpar_3d_n(const S& s, int g0, int octaves, dou
 #pragma unroll
             for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
         }
-        amp /= 2.0;
     }
 }
 
+// One 2D octave group (noise4 / noise5) at one lattice column.  Octave o samples at scale
+// sc * 2^-o and carries weight 2^o (Java's 1 / amp), both exact power-of-two scalings.
 template <class S>
 __device__ __forceinline__ double colpar_2d(const S& s, int g0, int octaves, double sc, double X, double Z,
                                             bool active) {
-    double acc = 0.0, amp = 1.0;
-    for (int o = 0; o < octaves; ++o) {
-        s.stage(g0 + o);
-        if (active) {
-            const double sx = sc * amp;
-            const double v = perlin2(s.perm(), axis(X * sx + s.off(0)), axis(Z * sx + s.off(2))) * (1.0 / amp);
-            acc = o == 0 ? 0.0 + v : acc + v;
+    double acc = 0.0;
+    if constexpr (S::kUniform) {
+        for (int o0 = 0; o0 < octaves; o0 += kStageBatch) {
+            const int nb = octaves - o0 < kStageBatch ? octaves - o0 : kStageBatch;
+            s.stage_batch(g0 + o0, nb);
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < kStageBatch; ++q) {
+                    if (q < nb) {
+                        const int o = o0 + q;
+                        const double sx = ldexp(sc, -o);
+                        const double v =
+                            ldexp(perlin2(s.perm(q), axis(X * sx + s.off(q, 0)), axis(Z * sx + s.off(q, 2))), o);
+                        acc = o == 0 ? 0.0 + v : acc + v;
+                    }
+                }
+            }
         }
-        amp /= 2.0;
+    } else {
+        for (int o = 0; o < octaves; ++o) {
+            s.stage(g0 + o);
+            if (active) {
+                const double sx = ldexp(sc, -o);
+                const double v = ldexp(perlin2(s.perm(), axis(X * sx + s.off(0)), axis(Z * sx + s.off(2))), o);
+                acc = o == 0 ? 0.0 + v : acc + v;
+            }
+        }
     }
     return acc;
 }
 
 // One climate group (NoiseGeneratorOctaves2173.a, :29-51) for up to N samples per lane.
+// Java's weight 0.55 / d6 with d6 = 2^-o is the exact 0.55 * 2^o; the scale keeps Java's
+// running product d7 *= octave_scale.
+// next_g (optional): the generator to prefetch after the group's last stage.
 template <int N, class S>
 __device__ __forceinline__ void colpar_climate(const S& s, int g0, int octaves, double start_scale,
                                                double octave_scale, const double* BX, const double* BZ, int n,
-                                               double* acc) {
+                                               double* acc, int next_g = -1) {
     const double d2 = start_scale / 1.5;
-    double d6 = 1.0, d7 = 1.0;
-    for (int o = 0; o < octaves; ++o) {
-        s.stage(g0 + o);
-        const SharedPerm P = s.perm();
-        const double sc = d2 * d7;
-        const double w = 0.55 / d6;
+    double d7 = 1.0;
+    if constexpr (S::kUniform) {
+        for (int o0 = 0; o0 < octaves; o0 += kStageBatch) {
+            const int nb = octaves - o0 < kStageBatch ? octaves - o0 : kStageBatch;
+            if (o0 + nb < octaves || next_g < 0) s.stage_batch(g0 + o0, nb);
+            else s.stage_batch(g0 + o0, nb, 1, next_g);
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            if (j < n) {
-                const double v = simplex2(P, BX[j] * sc + s.off(0), BZ[j] * sc + s.off(1)) * w;
-                acc[j] = o == 0 ? 0.0 + v : acc[j] + v;
+            for (int q = 0; q < kStageBatch; ++q) {
+                if (q < nb) {
+                    const int o = o0 + q;
+                    const SharedPerm P = s.perm(q);
+                    const double sc = d2 * d7;
+                    const double w = ldexp(0.55, o);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        if (j < n) {
+                            const double v = simplex2(P, BX[j] * sc + s.off(q, 0), BZ[j] * sc + s.off(q, 1)) * w;
+                            acc[j] = o == 0 ? 0.0 + v : acc[j] + v;
+                        }
+                    }
+                    d7 *= octave_scale;
+                }
             }
         }
-        d7 *= octave_scale;
-        d6 *= 0.5;
+    } else {
+        for (int o = 0; o < octaves; ++o) {
+            s.stage(g0 + o);
+            const SharedPerm P = s.perm();
+            const double sc = d2 * d7;
+            const double w = ldexp(0.55, o);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (j < n) {
+                    const double v = simplex2(P, BX[j] * sc + s.off(0), BZ[j] * sc + s.off(1)) * w;
+                    acc[j] = o == 0 ? 0.0 + v : acc[j] + v;
+                }
+            }
+            d7 *= octave_scale;
+        }
     }
 }
 
 // Climate of up to N samples per lane: finished temperature / rain (WorldChunkManager173.java:85-130).
 template <int N, class S>
 __device__ __forceinline__ void colpar_climate_all(const S& s, const double* BX, const double* BZ, int n,
-                                                   double* temp, double* rain) {
+                                                   double* temp, double* rain, int next_g = -1) {
     double t[N], r[N], c[N];
     colpar_climate<N>(s, kGenTemp, 4, 0.02500000037252903, 0.25, BX, BZ, n, t);
     colpar_climate<N>(s, kGenRain, 4, 0.05000000074505806, 0.3333333333333333, BX, BZ, n, r);
-    colpar_climate<N>(s, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, n, c);
+    colpar_climate<N>(s, kGenBlend, 2, 0.25, 0.5882352941176471, BX, BZ, n, c, next_g);
 #pragma unroll
     for (int j = 0; j < N; ++j)
         if (j < n) climate_finish(t[j], r[j], c[j], temp[j], rain[j]);

@@ -425,9 +425,9 @@ struct YAxis {
 };
 
 // One 3D octave at one lattice column, all levels [Y0, Y1): out[k] = value * weight
-// (the per-point addend of :281-282).  FMA: corner gradients from the grad3_coefficients table
-// gc (only for nonzero corner coordinates; see grad3_coefficients).
-template <int Y0, int Y1, bool FMA, class Perm>
+// (the per-point addend of :281-282).  FORM: how the corner gradients are evaluated (GradForm;
+// the table and bitfield forms only for nonzero corner coordinates, see grad3_coefficients).
+template <int Y0, int Y1, int FORM, class Perm>
 __device__ __forceinline__ void perlin3_column_g(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
                                                  const YAxis<Y0, Y1>& ya, double weight, double* out) {
     const int xp0 = p.get(ax.i);
@@ -444,7 +444,7 @@ __device__ __forceinline__ void perlin3_column_g(Perm p, const uint2* gc, const 
         const int p10 = p.get(p1) + az.i;
         const int p11 = p.get(p1 + 1) + az.i;
         double q00, q01, q10, q11;
-        if constexpr (FMA) {
+        if constexpr (FORM == kGradTable) {
             q00 = lerp(ax.fd, grad3_fma(gc[p.get(p00)], ax.f, yf, az.f), grad3_fma(gc[p.get(p10)], ax.fm1, yf, az.f));
             q01 = lerp(ax.fd, grad3_fma(gc[p.get(p01)], ax.f, yfm1, az.f), grad3_fma(gc[p.get(p11)], ax.fm1, yfm1, az.f));
             q10 = lerp(ax.fd, grad3_fma(gc[p.get(p00 + 1)], ax.f, yf, az.fm1),
@@ -452,23 +452,29 @@ __device__ __forceinline__ void perlin3_column_g(Perm p, const uint2* gc, const 
             q11 = lerp(ax.fd, grad3_fma(gc[p.get(p01 + 1)], ax.f, yfm1, az.fm1),
                        grad3_fma(gc[p.get(p11 + 1)], ax.fm1, yfm1, az.fm1));
         } else {
-            q00 = lerp(ax.fd, grad3(p.get(p00), ax.f, yf, az.f), grad3(p.get(p10), ax.fm1, yf, az.f));
-            q01 = lerp(ax.fd, grad3(p.get(p01), ax.f, yfm1, az.f), grad3(p.get(p11), ax.fm1, yfm1, az.f));
-            q10 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, yf, az.fm1), grad3(p.get(p10 + 1), ax.fm1, yf, az.fm1));
-            q11 = lerp(ax.fd, grad3(p.get(p01 + 1), ax.f, yfm1, az.fm1), grad3(p.get(p11 + 1), ax.fm1, yfm1, az.fm1));
+            q00 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00, ax.f, yf, az.f),
+                       corner_grad<FORM>(p, gc, p10, ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01, ax.f, yfm1, az.f),
+                       corner_grad<FORM>(p, gc, p11, ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00 + 1, ax.f, yf, az.fm1),
+                       corner_grad<FORM>(p, gc, p10 + 1, ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01 + 1, ax.f, yfm1, az.fm1),
+                       corner_grad<FORM>(p, gc, p11 + 1, ax.fm1, yfm1, az.fm1));
         }
         const double r0 = lerp(ya.fd[k], q00, q01);
         const double r1 = lerp(ya.fd[k], q10, q11);
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
 }
+// With gc (the grad3_coefficients table) the table form whenever no corner coordinate of the
+// wave is zero, else (or without gc) the case selection.
 template <int Y0, int Y1, class Perm>
 __device__ __forceinline__ void perlin3_column(Perm p, const Axis& ax, const Axis& az, const YAxis<Y0, Y1>& ya,
                                                double weight, double* out, const uint2* gc = nullptr) {
     if (gc != nullptr && __all(ya.yzero == 0u && ax.f != 0.0 && az.f != 0.0))
-        perlin3_column_g<Y0, Y1, true>(p, gc, ax, az, ya, weight, out);
+        perlin3_column_g<Y0, Y1, kGradTable>(p, gc, ax, az, ya, weight, out);
     else
-        perlin3_column_g<Y0, Y1, false>(p, gc, ax, az, ya, weight, out);
+        perlin3_column_g<Y0, Y1, kGradSelect>(p, gc, ax, az, ya, weight, out);
 }
 
 // Same values as perlin3_column, with the corner lerps computed once per cell run as the
@@ -606,13 +612,14 @@ __device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, con
                                                      uint32_t fresh, double weight, double* out,
                                                      const uint2* gc = nullptr) {
     // x and z cells start at ax.f / az.f; their other corners (f - 1) are never zero
-    if constexpr (FORM != kGradSelect) {
-        if ((yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0)) {
+    if constexpr (FORM == kGradSelect) {
+        perlin3_table_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
+    } else {
+        if ((yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0))
             perlin3_table_cells<Y0, Y1, FORM>(p, gc, ax, az, yt, fresh, weight, out);
-            return;
-        }
+        else
+            perlin3_table_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
     }
-    perlin3_table_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
 }
 
 // ---------------------------------------------------------------------------

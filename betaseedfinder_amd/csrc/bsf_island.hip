@@ -268,38 +268,107 @@ __device__ __forceinline__ bool stage1_column(const S& s, int cx, int cz, int i,
     return found;
 }
 
-// Stage 1a of one sample column: exact noise1 / noise4 / noise5 / climate, noise2 and noise3
-// over octaves kSplitOctave..15; returns whether an upward air-to-stone crossing at
-// y >= 64 is still possible, with the column's exact height biases and blend factors.
-template <class S>
-__device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i, int l, bool active, double* d9,
-                                               double* d12) {
-    const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
-    const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
-    ColAcc<8, 16> acc;
-    colpar_3d<8, 16>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, acc.n2, kSplitOctave);
-    colpar_3d<8, 16>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, acc.n3, kSplitOctave);
-    colpar_3d<8, 16>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, acc.n1);
-    acc.n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
-    acc.n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
-    double temp = 0.0, rain = 0.0;
-    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain);
-    const ColumnShape cs = column_shape(temp, rain, acc.n4, acc.n5);
+// Whether some level k in 8..15 can still hold an upward air-to-stone crossing when the blended
+// noise sum b[k] = (1 - c) n2 + c n3 (c = d12 clamped to [0, 1]) is known to within u.  For d12 in
+// [0, 1] the density blend n2/512 + (n3/512 - n2/512) d12 is increasing in both sums, so its
+// bounds are the blends of theirs, (b -/+ u) / 512; outside [0, 1] it is one of them alone
+// (c = 0 or 1).  A candidate start needs density <= 0 at some block and > 0 at the block above;
+// the blocks of y-cell k run linearly from level k to level k + 1, so that needs
+// lat[k] <= 0 < lat[k + 1] for some k in 8..15 (y >= 64 covers y >= 68; level 16 is -10).
+__device__ __forceinline__ bool stage1a_open(const double* d9, const double* b, double u) {
     double lo[9], hi[9];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        d9[k] = level_bias(cs, 8 + k);
-        d12[k] = n1_blend(acc.n1[k]);
-        density_bounds(d9[k], d12[k], acc.n2[k], acc.n3[k], kSplitBound, 8 + k, kSplitSlack, lo[k], hi[k]);
+        lo[k] = top_taper((b[k] - u) / 512.0 - d9[k], 8 + k) - kSplitSlack;
+        hi[k] = top_taper((b[k] + u) / 512.0 - d9[k], 8 + k) + kSplitSlack;
     }
     lo[8] = hi[8] = -10.0;
-    // a candidate start needs density <= 0 at some block and > 0 at the block above; the
-    // blocks of y-cell k run linearly from level k to level k + 1, so that needs
-    // lat[k] <= 0 < lat[k + 1] for some k in 8..15 (y >= 64 covers y >= 68)
     bool open = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) open |= lo[k] <= 0.0 && hi[k + 1] > 0.0;
-    return active && open;
+    return open;
+}
+
+// Stage 1a of one sample column: exact noise1 / noise4 / noise5 / climate, hence the column's
+// exact height biases d9 and blend factors d12; then the blend of noise2 and noise3 over
+// octaves kSplitOctave + 1..15 and, only when some column of the wave is still open, octave
+// kSplitOctave.  Returns whether an upward air-to-stone crossing at y >= 64 is still possible.
+// The noise2 / noise3 sums here only feed the bounds (Stage 1b sums all 16 octaves in Java
+// order), so they are accumulated blended, in the order evaluated: the rounding differs from
+// Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
+// stash: the wave's LDS area of 16 x 64 doubles, where d9 and d12 wait (lane-minor) while
+// noise2 / noise3 are evaluated, so that they hold no registers there.
+template <class S>
+__device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i, int l, bool active, double* d9,
+                                               double* d12, double* stash) {
+    const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
+    const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
+    double n1[8];
+    colpar_3d<8, 16>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, n1);
+    const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
+    const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
+    // steps 0..4: noise2 octaves 11..15, 5..9: noise3 octaves 11..15, 10: noise2 octave 10,
+    // 11: noise3 octave 10 (only if a column is open after step 9)
+    constexpr int kHigh = 15 - kSplitOctave;
+    auto step_gen = [](int step) {
+        const bool n3 = step < 2 * kHigh ? step >= kHigh : step == 2 * kHigh + 1;
+        const int o = step < 2 * kHigh ? kSplitOctave + 1 + step % kHigh : kSplitOctave;
+        return (n3 ? kGenN3 : kGenN2) + o;
+    };
+    double temp = 0.0, rain = 0.0;
+    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain, step_gen(0));
+    const ColumnShape cs = column_shape(temp, rain, n4, n5);
+    const int lane = threadIdx.x & 63;
+    double b[8], acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        stash[k * 64 + lane] = level_bias(cs, 8 + k);
+        stash[(8 + k) * 64 + lane] = n1_blend(n1[k]);
+        b[k] = 0.0;
+        acc[k] = 0.0;
+    }
+    bool open = active;
+    int step = 0;
+    for (; step < 2 * kHigh + 2; ++step) {
+        if (step == 2 * kHigh) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) d9[k] = stash[k * 64 + lane];
+            open = open && stage1a_open(d9, b, kSplitBoundCoarse);
+            if (!__any(open)) break;
+        }
+        const int g = step_gen(step);
+        const int o = g - (g >= kGenN3 ? kGenN3 : kGenN2);
+        const double sx = ldexp(684.412, -o), w = ldexp(1.0, o);
+        s.template stage_y<8, 16>(g, sx, 1, step + 1 < 2 * kHigh + 2 ? step_gen(step + 1) : kGenCount);
+        if (active) {
+            const YTable& yt = s.ytable();
+            const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
+            const Axis ax = axis(X * sx + s.off(0));
+            const Axis az = axis(Z * sx + s.off(2));
+            double v[8];
+            perlin3_column_table<8, 16, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] += v[k];
+        }
+        // the last step of a group folds its sum into the blend with weight 1 - c (noise2) or c
+        if (step == kHigh - 1 || step >= 2 * kHigh - 1) {
+            const bool n3 = g >= kGenN3;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double d = stash[(8 + k) * 64 + lane];
+                const double c = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+                b[k] += n3 ? c * acc[k] : acc[k] - c * acc[k];
+                acc[k] = 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        d9[k] = stash[k * 64 + lane];
+        d12[k] = stash[(8 + k) * 64 + lane];
+    }
+    if (step == 2 * kHigh + 2) open = open && stage1a_open(d9, b, kSplitBound);
+    return open;
 }
 
 // Wave-aggregated append of open columns.
@@ -336,6 +405,7 @@ template <int KIND>
 __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
     __shared__ GenStage st;
     __shared__ GenStageMulti stm;
+    __shared__ double s_stash[16 * 64];
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
     const int nseg = a.desc->nseg[KIND];
@@ -370,7 +440,7 @@ __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
                 const int i = (sc >> 1) ? 3 : 1, l = (sc & 1) ? 3 : 1;
                 const int cx = p.chunk_x0 + ch / cz_n, cz = p.chunk_z0 + ch % cz_n;
                 double d9[8], d12[8];
-                const bool open = stage1a_column(s, cx, cz, i, l, active, d9, d12);
+                const bool open = stage1a_column(s, cx, cz, i, l, active, d9, d12, s_stash);
                 append_open(a, open, seed, static_cast<int>(slot_first), g.job, cx * 4 + i, cz * 4 + l, d9, d12);
             }
         } else if (!job.gate) {
@@ -414,7 +484,6 @@ __device__ __forceinline__ void lane_octaves3d(LanePerm P, const uint32_t* __res
                                                double s_y, double X, double Z, double* acc) {
     constexpr int NY = Y1 - Y0;
     uint32_t* w = reinterpret_cast<uint32_t*>(P.base);
-    double amp = 1.0;
     for (int o = 0; o < octaves; ++o) {
         const uint4* src = reinterpret_cast<const uint4*>(perms + (g0 + o) * 64);
 #pragma unroll 4
@@ -426,7 +495,7 @@ __device__ __forceinline__ void lane_octaves3d(LanePerm P, const uint32_t* __res
             w[(4 * q + 3) * 64] = v.w;
         }
         const double oa = offs[(g0 + o) * 3 + 0], ob = offs[(g0 + o) * 3 + 1], oc = offs[(g0 + o) * 3 + 2];
-        const double sx = s_xz * amp, sy = s_y * amp, wgt = 1.0 / amp;
+        const double sx = ldexp(s_xz, -o), sy = ldexp(s_y, -o), wgt = ldexp(1.0, o);  // amp = 2^-o
         YAxis<Y0, Y1> ya;
         ya.build(sy, ob);
         const Axis ax = axis(X * sx + oa);
@@ -435,7 +504,6 @@ __device__ __forceinline__ void lane_octaves3d(LanePerm P, const uint32_t* __res
         perlin3_column_cells<Y0, Y1>(P, ax, az, ya, ya.fresh, wgt, v);
 #pragma unroll
         for (int k = 0; k < NY; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
-        amp /= 2.0;
     }
 }
 

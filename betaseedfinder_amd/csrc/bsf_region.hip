@@ -67,10 +67,9 @@ __global__ __launch_bounds__(kRegionItemCols, 2) void region_climate_kernel(Regi
 // The window's noise lattice, levels 8..15, one noise column per lane, emitted into the
 // density columns it belongs to (with the climate the previous kernel wrote).
 __global__ __launch_bounds__(kRegionItemCols, 3) void region_lattice_kernel(RegionArgs a) {
-    __shared__ GenStageGc st;
+    __shared__ GenStage st;
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
-    init_gcoef(st.gcoef);
     for (;;) {
         const int64_t item = region_next_item(&a.ctr[kRegionCtrLattice], &s_item);
         if (item >= a.lat_items) break;
@@ -83,7 +82,7 @@ __global__ __launch_bounds__(kRegionItemCols, 3) void region_lattice_kernel(Regi
         const int u = active ? static_cast<int>(c / nzl) : 0, v = active ? static_cast<int>(c % nzl) : 0;
         const double X = static_cast<double>(p.chunk_x0 * 4 + u);
         const double Z = static_cast<double>(p.chunk_z0 * 4 + v);
-        const OneWorldFma s{&st, a.perms + static_cast<size_t>(t.slot) * kGenCount * 64,
+        const OneWorldBits s{&st, a.perms + static_cast<size_t>(t.slot) * kGenCount * 64,
                          a.offsets + static_cast<size_t>(t.slot) * kGenCount * 3};
         ColAcc<8, 16> acc;
         colpar_noise<8, 16, true>(s, X, Z, active, acc);

@@ -86,9 +86,9 @@ struct S0Column {
 };
 
 template <int Y0, int Y1, bool WITH_SHAPE>
-__device__ __forceinline__ void evaluate_cached_column(GenStageGc& st, const uint32_t* perms, const double* offs,
+__device__ __forceinline__ void evaluate_cached_column(GenStage& st, const uint32_t* perms, const double* offs,
                                                        double X, double Z, bool active, S0Column<Y0, Y1>& c) {
-    const OneWorldFma s{&st, perms, offs};
+    const OneWorldBits s{&st, perms, offs};
     colpar_noise<Y0, Y1, WITH_SHAPE>(s, X, Z, active, c.acc);
     if (WITH_SHAPE) {
         const double BX = X * 4.0 + 2.0, BZ = Z * 4.0 + 2.0;
@@ -257,10 +257,9 @@ __device__ __forceinline__ int64_t stage_items(const S0Args& a) {
 // gated P20 of applyP20MinimumGateKernel) and appends the items reaching upper_min.
 template <int MODE>
 __global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
-    __shared__ GenStageGc st;
+    __shared__ GenStage st;
     __shared__ int s_count;
     const int tid = threadIdx.x;
-    init_gcoef(st.gcoef);
     const int ncols = MODE == S0_P20 ? 64 : 192;
     const int64_t n = stage_items(a);
     for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
@@ -317,14 +316,13 @@ __global__ __launch_bounds__(256, 3) void s0_upper_kernel(S0Args a) {
 // are appended for the coarse stage.  One 64-lane wave per item: it compacts the item's
 // active columns (lane t reads the masks of columns 4t .. 4t + 3) and walks them 64 at a time.
 __global__ __launch_bounds__(64, 2) void s0_lower_kernel(S0Args a) {
-    __shared__ GenStageGc st;
+    __shared__ GenStage st;
     __shared__ uint8_t s_active[256];
     __shared__ uint8_t s_highest[256];
     __shared__ int s_high;
     __shared__ P19Features s_f;
     __shared__ double s_leaf[16];
     const int tid = threadIdx.x;
-    init_gcoef(st.gcoef);
     const int64_t n = stage_items(a);
     for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
         const int64_t item = a.items[it];
@@ -424,10 +422,9 @@ __global__ __launch_bounds__(64, 2) void s0_lower_kernel(S0Args a) {
 // The 61x61x17 coarse sign grid of one item (coarse_exact_core.hpp:218-386), built in LDS as
 // one 61-bit z mask per (x, y) row and written to coarse_bits[list index].
 __global__ __launch_bounds__(64, 3) void s0_coarse_signs_kernel(S0Args a) {
-    __shared__ GenStageGc st;
+    __shared__ GenStage st;
     __shared__ unsigned long long rem[kCoarseRows];
     const int tid = threadIdx.x;
-    init_gcoef(st.gcoef);
     const int64_t n = stage_items(a);
     for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
         const int64_t item = a.items[it];
@@ -446,7 +443,7 @@ __global__ __launch_bounds__(64, 3) void s0_coarse_signs_kernel(S0Args a) {
             // the column shape (2D noise and climate) first, then the 3D sums of levels 0..7 and
             // of levels 8..15 in one loop body: eight levels of three octave sums at a time, with
             // nothing else live, fit two waves per SIMD
-            const OneWorldFma s{&st, perms, offs};
+            const OneWorldBits s{&st, perms, offs};
             ColumnShape cs;
             {
                 const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);

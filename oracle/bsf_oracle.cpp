@@ -1920,15 +1920,17 @@ void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
 // ---------------------------------------------------------------------------
 // FP64 operation count of island Stage 1 for one seed: Stage 1a at the window's sample columns
 // (lattice columns (4 cx + i, 4 cz + l), i, l in {1, 3}: blocks 4 / 12, IslandSearchEngine.java:
-// 4005-4007) -- noise2 / noise3 over octaves 10..15 only, noise1, noise4, noise5 and the climate
-// in full, the column shape, and per level the height bias, the noise1 blend and the density
-// bounds that allow for the left-out octaves (each octave o of a 3D Perlin sum is at most 2 * 2^o
-// in magnitude, so octaves 0..9 add at most 2046) -- then Stage 1b at each column the bounds leave
-// open: noise2 / noise3 over all 16 octaves, the exact densities and generateBareTerrain's y-chain
-// (BetaTerrain173.java:599-664) looking for stone at y >= 68 over non-stone.  A column is open when
-// some level k has lo[k] <= 0 < hi[k + 1].  The x axis of an octave is formed once per distinct
-// lattice x, the y axis once per octave (Stage 1a: shared by the seed's columns; Stage 1b: per
-// column, each open column being evaluated alone).
+// 4005-4007) -- noise1, noise4, noise5 and the climate in full, the column shape, and per level the
+// height bias and the noise1 blend; then noise2 / noise3 over octaves 11..15, blended per level
+// with the clamped noise1 blend factor, and the density bounds that allow for the left-out octaves
+// (each octave o of a 3D Perlin sum is at most 2 * 2^o in magnitude, so octaves 0..10 add at most
+// 4094); only if some column of the seed is still open, octave 10 of both and the bounds again
+// (octaves 0..9: 2046) -- then Stage 1b at each column the bounds leave open: noise2 / noise3 over
+// all 16 octaves, the exact densities and generateBareTerrain's y-chain (BetaTerrain173.java:
+// 599-664) looking for stone at y >= 68 over non-stone.  A column is open when some level k has
+// lo[k] <= 0 < hi[k + 1].  The x axis of an octave is formed once per distinct lattice x, the y
+// axis once per octave (Stage 1a: shared by the seed's columns; Stage 1b: per column, each open
+// column being evaluated alone).
 // ---------------------------------------------------------------------------
 constexpr int kSplitOctave1 = 10;
 
@@ -2053,14 +2055,12 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
     const size_t nc = cols.size();
     // Stage 1a
     t_tally = t1a;
-    std::vector<std::array<T, kS2Levels>> a2(nc), a3(nc), a1(nc);
+    std::vector<std::array<T, kS2Levels>> a1(nc);
     std::vector<T> a4(nc), a5(nc);
-    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, 16, cols, a2);
-    octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, 16, cols, a3);
     octaves3dAt<T>(t.n1, 684.412 / 80.0, 684.412 / 160.0, 0, 8, cols, a1);
     octaves2dAt<T>(t.n4, 1.121, cols, a4);
     octaves2dAt<T>(t.n5, 200.0, cols, a5);
-    const T u = T(2.0 * ((1 << kSplitOctave1) - 1)), slack = T(1e-6);
+    const T slack = T(1e-6);
     std::vector<std::array<T, kS2Levels>> d9s(nc), d12s(nc);
     for (size_t c = 0; c < nc; ++c) {
         const T tr = stage2ClimateGroup<T>(t.climate.e, 0.02500000037252903, 0.25, blocks[c].first, blocks[c].second);
@@ -2100,40 +2100,61 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         d5 = d5 + T(0.5);
         d6 = d6 * T(17.0) / T(16.0);
         const T d7 = T(17.0) / T(2.0) + d6 * T(4.0);
-        T lo[kS2Levels + 1], hi[kS2Levels + 1];
         for (int k = 0; k < kS2Levels; ++k) {
             const int y = kS2Y0 + k;
             T d9 = (i2d<T>(y) - d7) * T(12.0) / d5;
             if (d9 < T(0.0)) d9 = d9 * T(4.0);
-            const T d12 = (a1[c][k] / T(10.0) + T(1.0)) / T(2.0);
             d9s[c][k] = d9;
-            d12s[c][k] = d12;
-            // density_bounds: for d12 in [0, 1] the blend is increasing in both sums, and the taper
-            // is increasing
-            const T l2 = (a2[c][k] - u) / T(512.0), h2 = (a2[c][k] + u) / T(512.0);
-            const T l3 = (a3[c][k] - u) / T(512.0), h3 = (a3[c][k] + u) / T(512.0);
-            T l, h;
-            if (d12 < T(0.0)) {
-                l = l2;
-                h = h2;
-            } else if (d12 > T(1.0)) {
-                l = l3;
-                h = h3;
-            } else {
-                l = l2 + (l3 - l2) * d12;
-                h = h2 + (h3 - h2) * d12;
+            d12s[c][k] = (a1[c][k] / T(10.0) + T(1.0)) / T(2.0);
+        }
+    }
+    // noise2 / noise3, blended: b = (1 - cl) n2 + cl n3 with cl = d12 clamped to [0, 1] (for d12 in
+    // [0, 1] the density blend n2/512 + (n3/512 - n2/512) d12 is increasing in both sums, so its
+    // bounds are (b -/+ u) / 512; outside it is one sum alone).  Each group's octaves are summed
+    // first, then folded in.
+    std::vector<std::array<T, kS2Levels>> b(nc), g(nc);
+    for (size_t c = 0; c < nc; ++c)
+        for (int k = 0; k < kS2Levels; ++k) b[c][k] = T(0.0);
+    auto fold = [&](bool n3) {
+        for (size_t c = 0; c < nc; ++c)
+            for (int k = 0; k < kS2Levels; ++k) {
+                const T d = d12s[c][k];
+                const T cl = d < T(0.0) ? T(0.0) : (d > T(1.0) ? T(1.0) : d);
+                b[c][k] = b[c][k] + (n3 ? cl * g[c][k] : g[c][k] - cl * g[c][k]);
             }
-            lo[k] = taperT(l - d9, y) - slack;
-            hi[k] = taperT(h - d9, y) + slack;
+    };
+    auto bounds_open = [&](size_t c, T u) {
+        T lo[kS2Levels + 1], hi[kS2Levels + 1];
+        for (int k = 0; k < kS2Levels; ++k) {
+            const int y = kS2Y0 + k;
+            lo[k] = taperT((b[c][k] - u) / T(512.0) - d9s[c][k], y) - slack;
+            hi[k] = taperT((b[c][k] + u) / T(512.0) - d9s[c][k], y) + slack;
         }
         lo[kS2Levels] = hi[kS2Levels] = T(-10.0);
         bool op = false;
         for (int k = 0; k < kS2Levels; ++k) {
             const bool a = lo[k] <= T(0.0);
-            const bool b = hi[k + 1] > T(0.0);
-            op = op || (a && b);
+            const bool bb = hi[k + 1] > T(0.0);
+            op = op || (a && bb);
         }
-        open[c] = op ? 1 : 0;
+        return op;
+    };
+    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g);
+    fold(false);
+    octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g);
+    fold(true);
+    bool any = false;
+    for (size_t c = 0; c < nc; ++c) {
+        open[c] = bounds_open(c, T(2.0 * ((1 << (kSplitOctave1 + 1)) - 1))) ? 1 : 0;
+        any = any || open[c];
+    }
+    if (any) {
+        octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g);
+        fold(false);
+        octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g);
+        fold(true);
+        for (size_t c = 0; c < nc; ++c)
+            if (open[c]) open[c] = bounds_open(c, T(2.0 * ((1 << kSplitOctave1) - 1))) ? 1 : 0;
     }
     // Stage 1b at the open columns
     t_tally = t1b;
