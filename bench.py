@@ -535,7 +535,7 @@ def main():
 
     seeds = args.steps * batch
     value = seeds / elapsed
-    names = ["island_cache", "island_stage1", "island_stage1_multi", "island_stage1b", "island_ground", "island_stage2",
+    names = ["island_cache", "island_stage1", "island_stage1_multi", "island_defer", "island_stage1b", "island_ground", "island_stage2",
              "island_flood"]
     ks = {n: kstats(n) for n in names}
     passes = stats1[0] - stats0[0]

@@ -32,15 +32,37 @@ __device__ __forceinline__ void store_generator(const LanePerm& p, const GenOffs
     off_dst[2] = g.c;
 }
 
+// Generators of noise2 and noise3 below this octave are "deferrable": island Stage 1a never reads
+// them, so the cache kernel can step over their draws and pass_defer_kernel builds them only for
+// the seeds that reach Stage 1b.
+constexpr int kDeferOctaves = 10;
+__device__ __forceinline__ bool deferrable(int g) {
+    return (g >= kGenN2 && g < kGenN2 + kDeferOctaves) || (g >= kGenN3 && g < kGenN3 + kDeferOctaves);
+}
+
+constexpr int kDeferCount = 2 * kDeferOctaves;
+__device__ __forceinline__ int deferred_gen(int q) { return q < kDeferOctaves ? kGenN2 + q : kGenN3 + q - kDeferOctaves; }
+
 // All generators of one world into its cache entry (perms: [kGenCount][64], offs:
-// [kGenCount][3]).  Inactive lanes draw the same sequence but store nothing.
+// [kGenCount][3]).  Inactive lanes draw the same sequence but store nothing.  With defer, the
+// deferrable generators' draws are only stepped over (consume_generator: the same count of draws,
+// redraws included) and states[q] receives the Random state at the start of the q-th of them, so
+// that each can be built on its own later.
 __device__ __forceinline__ void build_world_cache(int64_t seed, bool active, LanePerm P, uint32_t* perms,
-                                                  double* offs, const DivMagic* __restrict__ magic) {
+                                                  double* offs, const DivMagic* __restrict__ magic,
+                                                  bool defer = false, uint64_t* states = nullptr) {
     Lcg r;
     r.set_seed(seed);
     // BetaTerrain173 constructor order (:84-92): noise2 16, noise3 16, noise1 8, two consumed
     // 4-octave generators, noise4 10, noise5 16.
+    int q = 0;
     for (int g = 0; g < kGenN4; ++g) {
+        if (defer && deferrable(g)) {
+            if (active) states[q] = r.s;
+            ++q;
+            consume_generator(r, magic);
+            continue;
+        }
         const GenOffsets o = build_generator(r, P, magic);
         if (active) store_generator(P, o, perms + g * 64, offs + g * 3);
     }
@@ -63,5 +85,7 @@ __device__ __forceinline__ void build_world_cache(int64_t seed, bool active, Lan
         }
     }
 }
+
+
 
 }  // namespace bsf

@@ -59,7 +59,8 @@ void IslandEngine::release() {
     drop_graphs();
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (back_stream_) (void)hipStreamSynchronize(back_stream_);
-    for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &lattice_, &flood_list_, &flood_scratch_, &open_, &s2_list_})
+    for (DeviceBuffer* b : {&jobs_, &segs_, &desc_, &lattice_, &flood_list_, &flood_scratch_, &open_, &s2_list_,
+                            &defer_slot_, &defer_state_})
         b->release();
     for (int i = 0; i < 2; ++i)
         for (DeviceBuffer* b : {&perms_[i], &offs_[i], &surv_slot_[i], &surv_seed_[i], &surv_job_[i], &surv_mask_[i],
@@ -130,6 +131,8 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     open_cap_ = static_cast<int64_t>(P) * 256;
     open_.ensure(static_cast<size_t>(open_cap_) * sizeof(OpenColumn));
     s2_list_.ensure(P * sizeof(int32_t));
+    defer_slot_.ensure(P * sizeof(int64_t));
+    defer_state_.ensure(P * kDeferCount * sizeof(uint64_t));
     for (int i = 0; i < 2; ++i) {
         ctr_[i].ensure(kCounters * sizeof(unsigned long long));
         hits_[i].ensure(P * sizeof(bsf_island_hit));
@@ -153,6 +156,7 @@ void IslandEngine::configure(int64_t pass_slots, int max_jobs, int min_quantum, 
     grid_s1_[0] = occupancy_grid(pass_stage1_kernel<0>, 64, 0, 0);
     grid_s1_[1] = occupancy_grid(pass_stage1_kernel<1>, 64, 0, 0);
     grid_s1b_ = occupancy_grid(pass_stage1b_kernel, 64, 0, 0);
+    grid_defer_ = occupancy_grid(pass_defer_kernel, 64, 0, 0);
     grid_ground_ = occupancy_grid(pass_ground_kernel, 64, 0, 0);
     for (int v = 0; v < kS2Variants; ++v)
         for (int f = 0; f < 2; ++f) grid_s2_[v][f] = occupancy_grid(stage2_kernel(v, f != 0), 64, stage2_lds(v), 0);
@@ -281,6 +285,8 @@ EngineArgs IslandEngine::args_for(int ab, int audit) const {
     a.open_cap = open_cap_;
     a.surv_mask = static_cast<unsigned long long*>(surv_mask_[ab].ptr);
     a.s2_list = static_cast<int32_t*>(s2_list_.ptr);
+    a.defer_slot = static_cast<int64_t*>(defer_slot_.ptr);
+    a.defer_state = static_cast<uint64_t*>(defer_state_.ptr);
     a.audit = audit;
     return a;
 }
@@ -311,6 +317,13 @@ void IslandEngine::enqueue_front(int ab, int audit, hipStream_t s) {
         hipLaunchKernelGGL(s1_multi, dim3(grid_s1_[1]), dim3(64), 0, s, a);
         check(hipGetLastError(), "stage 1 launch");
         t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[1]));
+    }
+    {
+        // units: the pass's one-world seeds (the kernel builds for those with an open column)
+        LaunchTimer t("island_defer", s, timers_);
+        hipLaunchKernelGGL(pass_defer_kernel, dim3(grid_defer_), dim3(64), 0, s, a);
+        check(hipGetLastError(), "deferred build launch");
+        t.stop(static_cast<uint64_t>(flight_[ab].seeds_kind[0]));
     }
     {
         LaunchTimer t("island_stage1b", s, timers_);

@@ -126,9 +126,10 @@ class IslandEngine {
     bool timers_ = false, use_graphs_ = true;
     hipStream_t stream_ = nullptr, back_stream_ = nullptr, copy_stream_ = nullptr;
     DeviceBuffer jobs_, segs_, desc_, lattice_, flood_list_, flood_scratch_, open_, s2_list_;
+    DeviceBuffer defer_slot_, defer_state_;
     DeviceBuffer perms_[2], offs_[2], surv_slot_[2], surv_seed_[2], surv_job_[2], surv_mask_[2];
     int64_t open_cap_ = 0;
-    int grid_s1b_ = 0, grid_ground_ = 0;
+    int grid_s1b_ = 0, grid_ground_ = 0, grid_defer_ = 0;
     DeviceBuffer ctr_[2], hits_[2];
     // pinned host memory
     IslandJob* h_jobs_ = nullptr;

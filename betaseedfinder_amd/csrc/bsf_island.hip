@@ -207,9 +207,46 @@ __global__ __launch_bounds__(64) void pass_cache_kernel(EngineArgs a) {
         const bool active = w < n;
         const int64_t slot = active ? w : n - 1;
         const PassSeg& g = seg_of_slot(a, slot);
-        const int64_t seed = job_seed(a.jobs[g.job], g.start + (slot - g.slot0));
+        const IslandJob& job = a.jobs[g.job];
+        const int64_t seed = job_seed(job, g.start + (slot - g.slot0));
+        // one-world (Stage 1a) seeds leave the deferrable generators to pass_defer_kernel
+        const bool defer = stage1_kind(job.worlds_per_wave) == 0;
         build_world_cache(seed, active, P, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
-                          a.offsets + static_cast<size_t>(slot) * kGenCount * 3, isl_magic);
+                          a.offsets + static_cast<size_t>(slot) * kGenCount * 3, isl_magic, defer,
+                          a.defer_state + static_cast<size_t>(slot) * kDeferCount);
+    }
+}
+
+// The deferred generators of the one-world seeds that Stage 1a left with an open column, before
+// Stage 1b, the ground gate and Stage 2 read them: one generator per lane, from the Random state
+// the cache build saved at its start.
+__global__ __launch_bounds__(64) void pass_defer_kernel(EngineArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm_lds[64 * 256];
+    __shared__ unsigned long long s_grp;
+    const int lane = threadIdx.x;
+    LanePerm P;
+    P.base = perm_lds + lane * 4;
+    unsigned long long n = a.ctr[kCtrDefer];
+    if (n > static_cast<unsigned long long>(a.pass_slots)) n = static_cast<unsigned long long>(a.pass_slots);
+    n *= kDeferCount;
+    for (;;) {
+        if (lane == 0) s_grp = atomicAdd(&a.ctr[kCtrDeferNext], 1ULL);
+        __syncthreads();
+        const unsigned long long grp = s_grp;
+        __syncthreads();
+        if (grp * 64 >= n) break;
+        const unsigned long long idx = grp * 64 + lane;
+        const bool active = idx < n;
+        const unsigned long long e = (active ? idx : n - 1) / kDeferCount;
+        const int q = static_cast<int>((active ? idx : n - 1) % kDeferCount);
+        const int64_t slot = a.defer_slot[e];
+        const int g = deferred_gen(q);
+        Lcg r;
+        r.s = a.defer_state[static_cast<size_t>(slot) * kDeferCount + q];
+        const GenOffsets o = build_generator(r, P, isl_magic);
+        if (active)
+            store_generator(P, o, a.perms + (static_cast<size_t>(slot) * kGenCount + g) * 64,
+                            a.offsets + (static_cast<size_t>(slot) * kGenCount + g) * 3);
     }
 }
 
@@ -433,6 +470,7 @@ __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
             // Stage 1a: open columns go to Stage 1b, which appends the survivors
             const OneWorldBits s{&st, perms, offs};
             const int64_t seed = job_seed(job, g.start + (slot_first - g.slot0));
+            bool any_open = false;
             for (int base = 0; base < ncols; base += 64) {
                 const int c = base + tid;
                 const bool active = c < ncols;
@@ -441,7 +479,13 @@ __global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
                 const int cx = p.chunk_x0 + ch / cz_n, cz = p.chunk_z0 + ch % cz_n;
                 double d9[8], d12[8];
                 const bool open = stage1a_column(s, cx, cz, i, l, active, d9, d12, s_stash);
+                any_open = any_open || __any(open);
                 append_open(a, open, seed, static_cast<int>(slot_first), g.job, cx * 4 + i, cz * 4 + l, d9, d12);
+            }
+            // the seed's deferred generators are built before Stage 1b reads them
+            if (any_open && tid == 0) {
+                const unsigned long long k = atomicAdd(&a.ctr[kCtrDefer], 1ULL);
+                a.defer_slot[k] = slot_first;
             }
         } else if (!job.gate) {
             // no gate for this job: every seed of the item goes to Stage 2

@@ -84,7 +84,9 @@ enum {
     kCtrStage2Select = 16, // Stage-2 queue head of the case-selection kernel (kCtrStage2: FMA form)
     kCtrGround = 17,     // ground-gate queue head
     kCtrKept = 18,       // survivors the ground gate passes to Stage 2 (s2_list entries)
-    kCounters = 19
+    kCtrDefer = 19,      // seeds whose deferred generators must be built (defer_slot entries)
+    kCtrDeferNext = 20,  // deferred-build queue head
+    kCounters = 21
 };
 
 // Density columns per survivor of the two Stage-2 lattice layouts (windows up to 4x4 chunks, up
@@ -208,6 +210,8 @@ struct EngineArgs {
     unsigned long long* surv_mask; // [pass_slots] chunks (window index bit) with a candidate start;
                                    // a Stage-1b survivor is appended by the column that sets the first
     int32_t* s2_list;              // [pass_slots] survivor indices for Stage 2 (pass_ground_kernel)
+    int64_t* defer_slot;           // [pass_slots] one-world seeds with an open column (pass slots)
+    uint64_t* defer_state;         // [pass_slots][kDeferCount] Random state at each deferred generator
     int32_t audit;                 // kAudit*: stop after Stage 1 or after the ground gate
     int32_t pad;
 };
@@ -241,6 +245,7 @@ __global__ void pass_build_kernel(EngineArgs a);
 __global__ void pass_cache_kernel(EngineArgs a);
 template <int KIND>
 __global__ void pass_stage1_kernel(EngineArgs a);
+__global__ void pass_defer_kernel(EngineArgs a);
 __global__ void pass_stage1b_kernel(EngineArgs a);
 // FMA: corner gradients from an LDS coefficient table (2 KB more LDS per workgroup; used where
 // LDS does not bound the occupancy, the 4 x 4 lattice bucket).
