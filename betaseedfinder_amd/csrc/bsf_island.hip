@@ -439,7 +439,7 @@ __device__ __forceinline__ void append_open(const EngineArgs& a, bool open, int6
 // worlds side by side, or no gate.  Separate kernels keep the register budget of the common
 // one-world form (and so its occupancy) free of the other form.
 template <int KIND>
-__global__ __launch_bounds__(64, 3) void pass_stage1_kernel(EngineArgs a) {
+__global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
     __shared__ GenStage st;
     __shared__ GenStageMulti stm;
     __shared__ double s_stash[16 * 64];
@@ -699,7 +699,7 @@ __device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
     return __any(sample && (starts & ~g[0]) != 0);
 }
 
-__global__ __launch_bounds__(kGroundThreads, 3) void pass_ground_kernel(EngineArgs a) {
+__global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineArgs a) {
     __shared__ OneWorldBits::Stage st;
     __shared__ double s_dens[2][25 * 8];
     __shared__ unsigned long long s_item;
@@ -769,7 +769,7 @@ constexpr int kS2Threads = 64;
 // One 64-lane wave per survivor: climate samples and lattice columns are walked in batches
 // of 64 lanes, each batch re-staging the survivor's generators (L2-resident after the first).
 template <bool FMA, bool FILTER>
-__global__ __launch_bounds__(kS2Threads, 3) void pass_stage2_kernel(EngineArgs a) {
+__global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a) {
     __shared__ typename OneWorldT<FMA ? kGradBits : kGradSelect>::Stage st;
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
