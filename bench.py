@@ -437,7 +437,7 @@ def run_serving(bsf, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=26)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=96 << 18,
                     help="seeds per step (default 96 passes of 2^18 seeds: 20 steps take about a minute)")

@@ -48,10 +48,14 @@ T* pinned(size_t n) {
 // launch variant.
 using Stage2Kernel = void (*)(EngineArgs);
 Stage2Kernel stage2_kernel(int v, bool filter) {
-    if (v == kS2FmaSmall) return filter ? pass_stage2_kernel<true, true> : pass_stage2_kernel<true, false>;
+    if (v == kS2FmaSmall || v == kS2FmaLarge)
+        return filter ? pass_stage2_kernel<true, true> : pass_stage2_kernel<true, false>;
     return filter ? pass_stage2_kernel<false, true> : pass_stage2_kernel<false, false>;
 }
-size_t stage2_lds(int v) { return static_cast<size_t>(2 * kNdBucket[v == kS2SelectLarge ? 1 : 0]) * sizeof(double); }
+size_t stage2_lds(int v) {
+    const bool large = v == kS2SelectLarge || v == kS2FmaLarge;
+    return static_cast<size_t>(2 * kNdBucket[large ? 1 : 0]) * sizeof(double);
+}
 
 }  // namespace
 
@@ -205,7 +209,7 @@ int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64
     if (tuning.gate && j.worlds_per_wave * 4 * p.chunks_x * p.chunks_z > 64) j.worlds_per_wave = 1;
     j.flood_class = flood_class_of(p);
     const int nd = 25 * p.chunks_x * p.chunks_z;
-    j.stage2_fma = nd <= kNdBucket[0] && tuning.stage2_fma != 0 ? 1 : 0;
+    j.stage2_fma = tuning.stage2_fma != 0 ? 1 : 0;
     h_jobs_[slot] = j;
     HostJob& h = host_jobs_[slot];
     h = HostJob{};
@@ -216,7 +220,8 @@ int IslandEngine::admit(int64_t id, int32_t tag, int64_t first_seed, const int64
     h.worlds_per_wave = j.worlds_per_wave;
     h.upload = true;
     h.nd = nd;
-    h.s2 = j.stage2_fma ? kS2FmaSmall : (nd <= kNdBucket[0] ? kS2SelectSmall : kS2SelectLarge);
+    const bool small = nd <= kNdBucket[0];
+    h.s2 = j.stage2_fma ? (small ? kS2FmaSmall : kS2FmaLarge) : (small ? kS2SelectSmall : kS2SelectLarge);
     return slot;
 }
 
@@ -435,8 +440,9 @@ void IslandEngine::launch(int audit) {
         if (h.used && h.cursor < h.count) variants |= 1 << h.s2;
     }
     f.seeds = plan_pass(f.cursors, f.finished, f.seeds_kind);
-    // the case-selection kernel with the large layout's LDS serves small windows too
+    // a kernel with the large layout's LDS serves small windows of its form too
     if (variants & (1 << kS2SelectLarge)) variants &= ~(1 << kS2SelectSmall);
+    if (variants & (1 << kS2FmaLarge)) variants &= ~(1 << kS2FmaSmall);
     if (variants == 0) variants = 1 << kS2FmaSmall;
     // the front half on stream_; the back half waits for it (and for the other pass's back half,
     // whose lattice and lists it reuses) on back_stream_ -- or on stream_ when kernels are timed

@@ -51,7 +51,7 @@ struct EngineStats {
 struct Tuning {
     int worlds_per_wave = 1;
     int gate = kGateGround;  // kGate* (tuning_for's default: the ground gate from 9 chunks up)
-    int stage2_fma = -1;  // 1: FMA gradients, 0: case selection, -1: FMA for windows up to 4x4 chunks
+    int stage2_fma = -1;  // 1: FMA gradients, 0: case selection, -1: the default (FMA gradients)
 };
 
 class IslandEngine {

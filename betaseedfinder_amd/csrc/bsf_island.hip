@@ -778,8 +778,8 @@ __global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a
     // the survivors that the ground gate kept
     unsigned long long nsurv = a.ctr[kCtrKept];
     if (nsurv > static_cast<unsigned long long>(a.pass_slots)) nsurv = static_cast<unsigned long long>(a.pass_slots);
-    // each job names its form (the FMA form only for windows up to 4x4 chunks); when a pass holds
-    // both, both kernels run with FILTER and each takes the survivors of its own form
+    // each job names its form; when a pass holds both, both kernels run with FILTER and each takes
+    // the survivors of its own form
     const int queue = FMA ? kCtrStage2 : kCtrStage2Select;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&a.ctr[queue], 1ULL);

@@ -36,8 +36,8 @@ struct IslandJob {
     int32_t gate;              // kGate* below
     int32_t flood_class;       // kFloodClass* below
     int32_t seed_source;       // kSeed*
-    int32_t stage2_fma;        // 1: Stage-2 corner gradients from the coefficient table (windows up to 4x4
-                               // chunks); 0: Java's case selection
+    int32_t stage2_fma;        // 1: Stage-2 corner gradients as FMAs of coefficients from register
+                               // bitfields; 0: Java's case selection
     unsigned long long sequence_seed;  // kSeedSplitMix: splitmix_seed(sequence_seed, first_seed + i)
 };
 static_assert(sizeof(IslandJob) == 120, "job-table entry size (server state memory is counted in entries)");
@@ -95,7 +95,7 @@ constexpr int kNdBucket[2] = {400, 1600};
 // Stage-2 launches: the FMA-gradient kernel (small layout only) and the case-selection kernel
 // with the small or the large layout's LDS.  A pass launches one per form present among its
 // jobs; each takes the survivors of its form from its own queue.
-enum { kS2FmaSmall = 0, kS2SelectSmall = 1, kS2SelectLarge = 2, kS2Variants = 3 };
+enum { kS2FmaSmall = 0, kS2SelectSmall = 1, kS2SelectLarge = 2, kS2FmaLarge = 3, kS2Variants = 4 };
 
 // Flood classes by window size (block bitsets of one window: rows x 64-bit z words).
 enum {
@@ -247,8 +247,8 @@ template <int KIND>
 __global__ void pass_stage1_kernel(EngineArgs a);
 __global__ void pass_defer_kernel(EngineArgs a);
 __global__ void pass_stage1b_kernel(EngineArgs a);
-// FMA: corner gradients from an LDS coefficient table (2 KB more LDS per workgroup; used where
-// LDS does not bound the occupancy, the 4 x 4 lattice bucket).
+// FMA: corner gradients as FMAs of coefficients from register bitfields (grad3_bits), with the
+// case selection for calls where a corner coordinate is zero; otherwise the case selection.
 // FILTER: the pass holds both Stage-2 forms, so the kernel skips the survivors of the other form.
 template <bool FMA, bool FILTER>
 __global__ void pass_stage2_kernel(EngineArgs a);

@@ -200,7 +200,6 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
         if (trial_seeds < 64 || !out) throw ArgError("bad autotune arguments");
         ensure_engine();
         const int ncols = 4 * p->chunks_x * p->chunks_z;
-        const bool small = 25 * p->chunks_x * p->chunks_z <= kNdBucket[0];
         std::vector<Tuning> cands;
         for (int k : {1, 2, 3, 4, 5, 8, 16})
             if (k == 1 || (k <= kMaxWorldsPerWave && k * ncols <= 64)) cands.push_back(Tuning{k, kGateGround, -1});
@@ -252,8 +251,8 @@ int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t 
                 best = alt;
             }
         }
-        best.stage2_fma = small ? 1 : 0;
-        if (small) {
+        best.stage2_fma = 1;
+        {
             Tuning alt = best;
             alt.stage2_fma = 0;
             const double r = rate_of(alt);
@@ -324,7 +323,7 @@ int bsf_island_get_tuning(const bsf_island_params* p, bsf_island_tuning* out) {
         std::memset(out, 0, sizeof(*out));
         out->worlds_per_wave = t.worlds_per_wave;
         out->gate = t.gate;
-        out->stage2_fma = 25 * p->chunks_x * p->chunks_z <= kNdBucket[0] && t.stage2_fma != 0 ? 1 : 0;
+        out->stage2_fma = t.stage2_fma != 0 ? 1 : 0;
     });
 }
 

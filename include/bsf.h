@@ -177,9 +177,10 @@ typedef struct bsf_island_tuning {
     int32_t gate;
     double seeds_per_sec;             /* measured rate of the chosen setting (autotune only) */
     int32_t candidates;               /* settings timed (autotune only) */
-    int32_t stage2_fma;               /* 1: corner gradients from the coefficient table, 0: Java's case
-                                         selection; windows above 4x4 chunks always select; -1 in
-                                         set_tuning: the default (table form up to 4x4) */
+    int32_t stage2_fma;               /* 1: corner gradients as FMAs of coefficients from register
+                                         bitfields (exact where no corner coordinate is zero, the case
+                                         selection otherwise), 0: Java's case selection; -1 in
+                                         set_tuning: the default (FMA form) */
 } bsf_island_tuning;
 int bsf_island_autotune(const bsf_island_params* p, int64_t first_seed, int64_t trial_seeds,
                         bsf_island_tuning* out);

@@ -49,8 +49,8 @@ def test_worlds_per_wave_and_gate_split_keep_hits(bsf, oracle):
 
 
 def test_stage2_forms_keep_hits(bsf, oracle):
-    """Stage 2's corner gradients from the coefficient table or by Java's case selection, for a
-    window alone and for two windows of one shape sharing passes with different forms."""
+    """Stage 2's corner gradients as FMAs of bitfield coefficients or by Java's case selection, for
+    a window alone and for two windows of one shape sharing passes with different forms."""
     p = bsf.island_params(**kw(-1, 0, 3, 2))
     first, count = 61000, 3000
     ref = oracle.run_island(first, count, oracle.island_params(**kw(-1, 0, 3, 2)), threads=16)
@@ -68,6 +68,30 @@ def test_stage2_forms_keep_hits(bsf, oracle):
         hits, _ = bsf.island_batch(jobs)
         for i, (f0, c0, pp) in enumerate(jobs):
             assert [h[:-1] for h in hits if h[-1] == i] == [h[:-1] for h in bsf.island_range(f0, c0, pp)], i
+    finally:
+        bsf.set_tuning(p, 1, 2, stage2_fma=-1)
+        bsf.set_tuning(q, 1, 2, stage2_fma=-1)
+
+
+def test_stage2_forms_keep_hits_large_window(bsf, oracle):
+    """Both Stage-2 forms on a window in the large lattice layout (6 x 5 chunks, 750 density
+    columns), alone and with a small window of the other form in the same passes (the large-layout
+    kernel of each form then serves both sizes)."""
+    p = bsf.island_params(**kw(-3, -2, 6, 5))
+    q = bsf.island_params(**kw(0, 0, 2, 2))
+    first, count = 70000, 1200
+    ref = oracle.run_island(first, count, oracle.island_params(**kw(-3, -2, 6, 5)), threads=16)
+    assert len(ref) > 0
+    try:
+        for form in (1, 0):
+            bsf.set_tuning(p, 1, 2, stage2_fma=form)
+            assert bsf.get_tuning(p).stage2_fma == form
+            assert bsf.island_range(first, count, p) == ref, form
+            bsf.set_tuning(q, 1, 2, stage2_fma=1 - form)
+            jobs = [(first, count, p), (first + 11, count, q)]
+            hits, _ = bsf.island_batch(jobs)
+            assert [h[:-1] for h in hits if h[-1] == 0] == [h[:-1] for h in ref], form
+            assert [h[:-1] for h in hits if h[-1] == 1] == [h[:-1] for h in bsf.island_range(first + 11, count, q)], form
     finally:
         bsf.set_tuning(p, 1, 2, stage2_fma=-1)
         bsf.set_tuning(q, 1, 2, stage2_fma=-1)
