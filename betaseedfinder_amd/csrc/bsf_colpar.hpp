@@ -48,14 +48,19 @@ __device__ __forceinline__ void init_gcoef(uint2* t) {
 // Stager for one world per workgroup: every lane reads the same staged generators.  The
 // cache entries of the next generators (the usual successors) are loaded into registers
 // while the current ones are evaluated, so a stage only waits for its LDS stores.
-template <int FORM>
+// YT: levels of the Y-axis tables that a 3D octave group builds at once into ytabs (colpar_3d,
+// one table per octave; 0: each staged octave builds its own in st->yt).
+template <int FORM, int YT = 0>
 struct OneWorldT {
     static constexpr bool kUniform = true;
     static constexpr int kGradForm = FORM;  // how 3D corner gradients are evaluated (GradForm)
+    static constexpr int kYTabLevels = YT;
     using Stage = std::conditional_t<FORM == kGradTable, GenStageGc, GenStage>;
+    using YTabs = YTab<YT == 0 ? 1 : YT>;
     Stage* st;
     const uint32_t* perms;  // the world's cache entry [kGenCount][64]
     const double* offs;     // [kGenCount][3]
+    YTabs* ytabs = nullptr; // LDS, kMaxOctaveTables tables (YT > 0)
     mutable int pf_g = -1;  // first generator held in the prefetch registers
     mutable int pf_n = 0;   // and how many
     mutable uint32_t pf_perm[kStageBatch] = {};
@@ -130,6 +135,8 @@ struct OneWorldT {
 using OneWorld = OneWorldT<kGradSelect>;
 using OneWorldFma = OneWorldT<kGradTable>;
 using OneWorldBits = OneWorldT<kGradBits>;
+// Up to this many octaves' Y-axis tables are built at once (the largest 3D group, 16 octaves).
+constexpr int kMaxOctaveTables = 16;
 
 // Stager for up to kMaxWorldsPerWave consecutive cache slots in one 64-lane workgroup: lane
 // group `sub` evaluates world first_slot + sub.  Used when a world needs few columns.
@@ -140,6 +147,7 @@ struct GenStageMulti {
 };
 struct MultiWorld {
     static constexpr bool kUniform = false;
+    static constexpr int kYTabLevels = 0;
     GenStageMulti* st;
     const uint32_t* perms;  // cache entry of the first world; world j at + j * kGenCount * 64
     const double* offs;
@@ -178,16 +186,17 @@ template <int Y0, int Y1, class S>
 __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
                                           double Z, bool active, double* acc, int o0 = 0) {
     constexpr int NY = Y1 - Y0;
-    for (int o = o0; o < octaves; ++o) {
-        // amp = 2^-o (halved exactly per octave): the scales s * amp and the weight 1 / amp = 2^o
-        // are exact power-of-two scalings
-        const double sx = ldexp(s_xz, -o), sy = ldexp(s_y, -o), w = ldexp(1.0, o);
-        if constexpr (S::kUniform) {
-            // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
-            // are the same on every lane, so the corner-reuse branch is scalar
-            s.template stage_y<Y0, Y1>(g0 + o, sy, o + 1 < octaves ? 1 : kStageBatch);
+    if constexpr (S::kUniform && S::kYTabLevels == NY) {
+        // the group's Y-axis tables at once (lane = octave x level); each octave then stages its
+        // permutation only, and the first stage's barrier publishes the tables
+        __syncthreads();
+        build_ytabs<NY>(s.ytabs, octaves - o0, s_y, Y0, [&](int q) { return o0 + q; },
+                        [&](int q) { return s.offs[(g0 + o0 + q) * 3 + 1]; });
+        for (int o = o0; o < octaves; ++o) {
+            const double sx = ldexp(s_xz, -o), w = ldexp(1.0, o);
+            s.stage_batch(g0 + o, 1, o + 1 < octaves ? 1 : kStageBatch);
             if (active) {
-                const YTable& yt = s.ytable();
+                const auto& yt = s.ytabs[o - o0];
                 const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
                 const Axis ax = axis(X * sx + s.off(0));
                 const Axis az = axis(Z * sx + s.off(2));
@@ -196,17 +205,38 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
 #pragma unroll
                 for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
             }
-        } else {
-            s.stage(g0 + o);
-            if (active) {
-                YAxis<Y0, Y1> ya;
-                ya.build(sy, s.off(1));
-                const Axis ax = axis(X * sx + s.off(0));
-                const Axis az = axis(Z * sx + s.off(2));
-                double v[NY];
-                perlin3_column_cells<Y0, Y1>(s.perm(), ax, az, ya, ya.fresh, w, v);
+        }
+    } else {
+        for (int o = o0; o < octaves; ++o) {
+            // amp = 2^-o (halved exactly per octave): the scales s * amp and the weight 1 / amp = 2^o
+            // are exact power-of-two scalings
+            const double sx = ldexp(s_xz, -o), sy = ldexp(s_y, -o), w = ldexp(1.0, o);
+            if constexpr (S::kUniform) {
+                // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
+                // are the same on every lane, so the corner-reuse branch is scalar
+                s.template stage_y<Y0, Y1>(g0 + o, sy, o + 1 < octaves ? 1 : kStageBatch);
+                if (active) {
+                    const YTable& yt = s.ytable();
+                    const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
+                    const Axis ax = axis(X * sx + s.off(0));
+                    const Axis az = axis(Z * sx + s.off(2));
+                    double v[NY];
+                    perlin3_column_table<Y0, Y1, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
-                for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
+                    for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
+                }
+            } else {
+                s.stage(g0 + o);
+                if (active) {
+                    YAxis<Y0, Y1> ya;
+                    ya.build(sy, s.off(1));
+                    const Axis ax = axis(X * sx + s.off(0));
+                    const Axis az = axis(Z * sx + s.off(2));
+                    double v[NY];
+                    perlin3_column_cells<Y0, Y1>(s.perm(), ax, az, ya, ya.fresh, w, v);
+#pragma unroll
+                    for (int k = 0; k < NY; ++k) acc[k] = o == o0 ? 0.0 + v[k] : acc[k] + v[k];
+                }
             }
         }
     }

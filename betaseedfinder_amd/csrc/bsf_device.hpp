@@ -568,13 +568,74 @@ __device__ __forceinline__ void build_ytable(YTable& t, double sy, double b) {
     build_ytable_n<Y1 - Y0>(t, sy, b, Y0);
 }
 
+// A Y-axis table of NY levels (the same fields as YTable), for tables built several at once.
+template <int NY>
+struct YTab {
+    double cell_f[NY], cell_fm1[NY], fd[NY];
+    int yi[NY];
+    uint32_t fresh;
+    uint32_t yzero;
+};
+
+// The tables of n octaves over levels [y0, y0 + NY) at once: octave q has y scale
+// ldexp(sy0, -oct(q)) and y offset boff(q).  Each of the 64 lanes of the calling wave takes one
+// (octave, level) pair, 64 / NY octaves per round; the values are build_ytable_n's.  Every lane
+// of the (one-wave) workgroup calls it; a barrier must follow before the tables are read.
+template <int NY, class OctF, class OffF>
+__device__ __forceinline__ void build_ytabs(YTab<NY>* tabs, int n, double sy0, int y0, OctF oct, OffF boff) {
+    static_assert(64 % NY == 0 && NY <= 16, "levels per table must divide the wave");
+    constexpr int kPer = 64 / NY;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int ql = lane / NY, k = lane % NY;
+    for (int q0 = 0; q0 < n; q0 += kPer) {
+        const int q = q0 + ql;
+        const bool mine = q < n;
+        const int qq = mine ? q : n - 1;
+        const double sy = ldexp(sy0, -oct(qq));
+        const double b = boff(qq);
+        const double v = static_cast<double>(y0 + k) * sy + b;
+        const int fl = java_floor(v);
+        const int idx = fl & 255;
+        const double f = v - static_cast<double>(fl);
+        const int idx_prev = __shfl_up(idx, 1);
+        const bool fresh = mine && (k == 0 || idx != idx_prev);
+        const uint32_t gm = static_cast<uint32_t>(__ballot(fresh) >> (ql * NY)) & ((1u << NY) - 1u);
+        // the cell of level y0 may begin below y0: walk back while the masked index stays the same
+        double start0 = f;
+        if (k == 0 && y0 > 0) {
+            for (int y = y0 - 1; y >= 0; --y) {
+                const double vy = static_cast<double>(y) * sy + b;
+                const int fy = java_floor(vy);
+                if ((fy & 255) != idx) break;
+                start0 = vy - static_cast<double>(fy);
+            }
+        }
+        const int start = 31 - __clz(static_cast<int>(gm & ((2u << k) - 1u)));
+        double cf = __shfl(f, ql * NY + start);
+        const double s0 = __shfl(start0, ql * NY);
+        if (start == 0) cf = s0;
+        const uint32_t zm = static_cast<uint32_t>(__ballot(mine && cf == 0.0) >> (ql * NY)) & ((1u << NY) - 1u);
+        if (mine) {
+            YTab<NY>& t = tabs[q];
+            t.cell_f[k] = cf;
+            t.cell_fm1[k] = cf - 1.0;
+            t.fd[k] = fade(f);
+            t.yi[k] = idx;
+            if (k == 0) {
+                t.fresh = gm;
+                t.yzero = zm;
+            }
+        }
+    }
+}
+
 // perlin3_column_cells with the Y axis read from a shared table; `fresh` must be uniform.
 // With gc (the grad3_coefficients table of the 256 permutation bytes, in LDS) the corner
 // gradients are three FP64 ops each instead of the case selection, whenever no corner
 // coordinate of the wave's cells is zero (checked per call; the selection path otherwise).
-template <int Y0, int Y1, int FORM, class Perm>
+template <int Y0, int Y1, int FORM, class Perm, class YT>
 __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
-                                                    const YTable& yt, uint32_t fresh, double weight, double* out) {
+                                                    const YT& yt, uint32_t fresh, double weight, double* out) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
     double q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
@@ -607,8 +668,8 @@ __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, con
 }
 // FORM kGradTable needs gc (the grad3_coefficients table); the table and bitfield forms fall
 // back to the case selection for a call with a zero corner coordinate somewhere in the wave.
-template <int Y0, int Y1, int FORM, class Perm>
-__device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YTable& yt,
+template <int Y0, int Y1, int FORM, class Perm, class YT>
+__device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, const Axis& az, const YT& yt,
                                                      uint32_t fresh, double weight, double* out,
                                                      const uint2* gc = nullptr) {
     // x and z cells start at ax.f / az.f; their other corners (f - 1) are never zero

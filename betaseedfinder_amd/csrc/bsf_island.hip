@@ -364,6 +364,13 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
         b[k] = 0.0;
         acc[k] = 0.0;
     }
+    auto step_oct = [](int step) { return step < 2 * kHigh ? kSplitOctave + 1 + step % kHigh : kSplitOctave; };
+    constexpr bool kTabs = S::kYTabLevels == 8;
+    if constexpr (kTabs) {
+        // the Y-axis tables of steps 0..9 at once (the stage of step 0 publishes them)
+        __syncthreads();
+        build_ytabs<8>(s.ytabs, 2 * kHigh, 684.412, 8, step_oct, [&](int q) { return s.offs[step_gen(q) * 3 + 1]; });
+    }
     bool open = active;
     int step = 0;
     for (; step < 2 * kHigh + 2; ++step) {
@@ -372,13 +379,21 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
             for (int k = 0; k < 8; ++k) d9[k] = stash[k * 64 + lane];
             open = open && stage1a_open(d9, b, kSplitBoundCoarse);
             if (!__any(open)) break;
+            if constexpr (kTabs)
+                build_ytabs<8>(s.ytabs + 2 * kHigh, 2, 684.412, 8, [&](int q) { return step_oct(2 * kHigh + q); },
+                               [&](int q) { return s.offs[step_gen(2 * kHigh + q) * 3 + 1]; });
         }
         const int g = step_gen(step);
-        const int o = g - (g >= kGenN3 ? kGenN3 : kGenN2);
+        const int o = step_oct(step);
         const double sx = ldexp(684.412, -o), w = ldexp(1.0, o);
-        s.template stage_y<8, 16>(g, sx, 1, step + 1 < 2 * kHigh + 2 ? step_gen(step + 1) : kGenCount);
+        const int next_g = step + 1 < 2 * kHigh + 2 ? step_gen(step + 1) : kGenCount;
+        if constexpr (kTabs) s.stage_batch(g, 1, 1, next_g);
+        else s.template stage_y<8, 16>(g, sx, 1, next_g);
         if (active) {
-            const YTable& yt = s.ytable();
+            const auto& yt = [&]() -> const auto& {
+                if constexpr (kTabs) return s.ytabs[step];
+                else return s.ytable();
+            }();
             const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
             const Axis ax = axis(X * sx + s.off(0));
             const Axis az = axis(Z * sx + s.off(2));
@@ -468,6 +483,8 @@ __global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
         const double* offs = a.offsets + static_cast<size_t>(slot_first) * kGenCount * 3;
         if (KIND == 0) {
             // Stage 1a: open columns go to Stage 1b, which appends the survivors
+            // Stage 1a builds each octave's Y-axis table as it stages it: the tables built at once
+            // cost it more spilled registers than they save
             const OneWorldBits s{&st, perms, offs};
             const int64_t seed = job_seed(job, g.start + (slot_first - g.slot0));
             bool any_open = false;
@@ -701,6 +718,7 @@ __device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
 
 __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineArgs a) {
     __shared__ OneWorldBits::Stage st;
+    __shared__ YTab<8> s_ytabs[kMaxOctaveTables];
     __shared__ double s_dens[2][25 * 8];
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
@@ -719,8 +737,8 @@ __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineAr
         if (job.gate == kGateGround) {
             const bsf_island_params p = job.p;
             const int cz_n = p.chunks_z;
-            const OneWorldBits s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
-                                a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
+            const OneWorldT<kGradBits, 8> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                                            a.offsets + static_cast<size_t>(slot) * kGenCount * 3, s_ytabs};
             unsigned long long mask = a.surv_mask[slot];
             keep = false;
             while (mask != 0 && !keep) {
@@ -771,6 +789,7 @@ constexpr int kS2Threads = 64;
 template <bool FMA, bool FILTER>
 __global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a) {
     __shared__ typename OneWorldT<FMA ? kGradBits : kGradSelect>::Stage st;
+    __shared__ YTab<8> s_ytabs[kMaxOctaveTables];
     __shared__ unsigned long long s_item;
     extern __shared__ double s_clim[];  // temperature[nd] then rain[nd] of the density columns
     const int tid = threadIdx.x;
@@ -798,8 +817,8 @@ __global__ __launch_bounds__(kS2Threads, 2) void pass_stage2_kernel(EngineArgs a
         double* s_rain = s_clim + nd;
         const int nzl = 4 * cz_n + 1;
         const int nc = (4 * cx_n + 1) * nzl;
-        const OneWorldT<FMA ? kGradBits : kGradSelect> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
-                               a.offsets + static_cast<size_t>(slot) * kGenCount * 3};
+        const OneWorldT<FMA ? kGradBits : kGradSelect, 8> s{&st, a.perms + static_cast<size_t>(slot) * kGenCount * 64,
+                                                          a.offsets + static_cast<size_t>(slot) * kGenCount * 3, s_ytabs};
         double* lat = a.lattice + static_cast<size_t>(it) * a.lat_stride;
         // climate of every density column: d = (chunk * 25) + i*5 + l, block (cx*16+3i+1, cz*16+3l+1)
         for (int base = 0; base < nd; base += kS2Threads) {
