@@ -280,7 +280,7 @@ __device__ __forceinline__ double colpar_2d(const S& s, int g0, int octaves, dou
                         const int o = o0 + q;
                         const double sx = ldexp(sc, -o);
                         const double v =
-                            ldexp(perlin2(s.perm(q), axis(X * sx + s.off(q, 0)), axis(Z * sx + s.off(q, 2))), o);
+                            ldexp(perlin2<S::kGradForm>(s.perm(q), axis(X * sx + s.off(q, 0)), axis(Z * sx + s.off(q, 2))), o);
                         acc = o == 0 ? 0.0 + v : acc + v;
                     }
                 }

@@ -357,14 +357,35 @@ __device__ __forceinline__ Axis axis(double v) {
     return a;
 }
 
+// The 2D corners as a x + c z: grad3 at y = 0.0 and grad2_legacy take x with the same
+// coefficient (u = x for h < 8, plus v = x for h in {12, 14}; grad2_legacy's u = 0.0 * x for h >= 8
+// is a zero) and z with the same one (v = z), and have no other term.  For nonzero x and z one of
+// the two terms is nonzero, so fma(a, x, c * z) rounds once like Java's (+/-u) + (+/-v): bit for bit
+// equal.  The coefficients are grad3_bits' x and z fields.
+__device__ __forceinline__ double grad2_bits(int h, double x, double z) {
+    const uint32_t o = static_cast<uint32_t>(h) << 1;
+    const double a = static_cast<double>(grad3_field_at(kGrad3FieldX, o));
+    const double c = static_cast<double>(grad3_field_at(kGrad3FieldZ, o));
+    return __fma_rn(a, x, c * z);
+}
+
 // The 2D legacy column value (yLen == 1 branch of a(...), :188-223) at an already
-// offset x / z: the y index is 0 and y never enters.
-template <class Perm>
+// offset x / z: the y index is 0 and y never enters.  FORM kGradBits: the corners by grad2_bits
+// when the x and z cell fractions are nonzero on every lane (the f - 1 ones never are zero).
+template <int FORM = kGradSelect, class Perm>
 __device__ __forceinline__ double perlin2(Perm p, const Axis& ax, const Axis& az) {
     const int xp0 = p.get(ax.i);
     const int xp1 = p.get(ax.i + 1);
     const int p00 = p.get(xp0) + az.i;
     const int p10 = p.get(xp1) + az.i;
+    if constexpr (FORM == kGradBits) {
+        if (__all(ax.f != 0.0 && az.f != 0.0)) {
+            const double q0 = lerp(ax.fd, grad2_bits(p.get(p00), ax.f, az.f), grad2_bits(p.get(p10), ax.fm1, az.f));
+            const double q1 =
+                lerp(ax.fd, grad2_bits(p.get(p00 + 1), ax.f, az.fm1), grad2_bits(p.get(p10 + 1), ax.fm1, az.fm1));
+            return lerp(az.fd, q0, q1);
+        }
+    }
     const double q0 = lerp(ax.fd, grad2_legacy(p.get(p00), ax.f, az.f), grad3(p.get(p10), ax.fm1, 0.0, az.f));
     const double q1 = lerp(ax.fd, grad3(p.get(p00 + 1), ax.f, 0.0, az.fm1), grad3(p.get(p10 + 1), ax.fm1, 0.0, az.fm1));
     return lerp(az.fd, q0, q1);
@@ -694,16 +715,28 @@ __device__ __forceinline__ int simplex_floor(double v) {
     const int t = static_cast<int>(v);
     return v > 0.0 ? t : t - 1;
 }
-// grad3[g][0] * x + grad3[g][1] * z of NoiseGenerator2173 (g < 12): each product is +/-x, or
-// 0.0 * x, the zero carrying x's sign.  g < 8: x * (+/-1 by bit 0); else 0.  g < 4: z * (+/-1 by
-// bit 1); 4..7: 0; 8..11: z * (+/-1 by bit 0).
+// grad3[g][0] * x + grad3[g][1] * z of NoiseGenerator2173 (g < 12), the components in {-1, 0, 1}:
+// g < 8: x by +/-1 (bit 0), else 0; g < 4: z by +/-1 (bit 1), 4..7: 0, 8..11: z by +/-1 (bit 0).
+// Java forms both products exactly (0.0 * x a zero with x's sign) and rounds their sum once;
+// fma(gx, x, gz * z) forms the same exact sum and rounds it once, and a zero sum takes the sign
+// that the addition of the two exact products gives.  So the FMA form is bit for bit Java's for
+// every x and z.  The components are 2-bit two's-complement fields of g at bits [2g, 2g + 1].
+__host__ __device__ constexpr uint32_t simplex_field(int which) {
+    uint32_t k = 0;
+    for (int g = 0; g < 12; ++g) {
+        int c;
+        if (which == 0) c = g < 8 ? ((g & 1) ? -1 : 1) : 0;
+        else c = g < 4 ? ((g & 2) ? -1 : 1) : (g < 8 ? 0 : ((g & 1) ? -1 : 1));
+        k |= (static_cast<uint32_t>(c) & 3u) << (2 * g);
+    }
+    return k;
+}
+constexpr uint32_t kSimplexFieldX = simplex_field(0), kSimplexFieldZ = simplex_field(1);
 __device__ __forceinline__ double simplex_grad(int g, double x, double z) {
-    const uint32_t ug = static_cast<uint32_t>(g);
-    const double zero_x = __hiloint2double(__double2hiint(x) & static_cast<int>(0x80000000u), 0);
-    const double zero_z = __hiloint2double(__double2hiint(z) & static_cast<int>(0x80000000u), 0);
-    const double tx = g < 8 ? flip_sign(x, ug << 31) : zero_x;
-    const double tz = g < 4 ? flip_sign(z, (ug << 30) & 0x80000000u) : (g < 8 ? zero_z : flip_sign(z, ug << 31));
-    return tx + tz;
+    const uint32_t o = static_cast<uint32_t>(g) << 1;
+    const double a = static_cast<double>(grad3_field_at(kSimplexFieldX, o));
+    const double c = static_cast<double>(grad3_field_at(kSimplexFieldZ, o));
+    return __fma_rn(a, x, c * z);
 }
 template <class Perm>
 __device__ __forceinline__ double simplex2(Perm p, double d5, double d6) {

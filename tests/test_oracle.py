@@ -557,6 +557,83 @@ def test_grad3_bitfields_decode_to_java_grad():
     assert words == [0x3100DDDD, 0xDDDD00F5, 0xC4F5F500]
 
 
+def test_grad2_bitfield_form_is_exact():
+    """The 2D corners as fma(a, x, c * z) with the x and z coefficient fields of the 3D form: bit
+    for bit Java's grad at y = 0.0 (NoiseGeneratorPerlin173.java:145-165) and its 2-argument
+    legacy gradient, u = (1 - bit 3) * x (:123-143), for nonzero x and z, cancelling sums
+    included."""
+    import random
+    import struct
+    libm = ctypes.CDLL("libm.so.6")
+    libm.fma.restype = ctypes.c_double
+    libm.fma.argtypes = [ctypes.c_double] * 3
+
+    def bits(v):
+        return struct.unpack("<Q", struct.pack("<d", v))[0]
+
+    def java_grad(h, x, y, z):
+        h &= 15
+        u = x if h < 8 else y
+        v = y if h < 4 else (x if h in (12, 14) else z)
+        return (-u if h & 1 else u) + (-v if h & 2 else v)
+
+    def java_grad2(h, x, z):
+        h &= 15
+        u = float(1 - ((h & 8) >> 3)) * x
+        v = 0.0 if h < 4 else (x if h in (12, 14) else z)
+        return (-u if h & 1 else u) + (-v if h & 2 else v)
+
+    def field(word, h):
+        v = (word >> (2 * (h & 15))) & 3
+        return float(v - 4 if v & 2 else v)
+
+    fx, fz = 0x3100DDDD, 0xC4F5F500  # the x and z words of test_grad3_bitfields_decode_to_java_grad
+    rng = random.Random(5)
+    coords = [rng.random() for _ in range(200)] + [rng.random() - 1.0 for _ in range(200)]
+    coords += [2.0 ** -52, 1.0 - 2.0 ** -53, -1.0, -2.0 ** -53, 0.5, -0.5]
+    for h in range(256):
+        a, c = field(fx, h), field(fz, h)
+        for _ in range(200):
+            x, z = rng.choice(coords), rng.choice(coords)
+            for xx, zz in ((x, z), (x, x), (x, -x)):
+                got = libm.fma(a, xx, c * zz)
+                assert bits(got) == bits(java_grad(h, xx, 0.0, zz)), (h, xx, zz)
+                assert bits(got) == bits(java_grad2(h, xx, zz)), (h, xx, zz)
+
+
+def test_simplex_gradient_fma_form_is_exact():
+    """The simplex gradient grad3[g][0] * x + grad3[g][1] * z of NoiseGenerator2173 (:117-172)
+    as fma(gx, x, gz * z), the components decoded from 2-bit fields: bit for bit Java's for every
+    x and z, zeros and signed zeros included (both form the two exact products and round their
+    sum once)."""
+    import random
+    import struct
+    libm = ctypes.CDLL("libm.so.6")
+    libm.fma.restype = ctypes.c_double
+    libm.fma.argtypes = [ctypes.c_double] * 3
+    java_grad3 = [(1, 1), (-1, 1), (1, -1), (-1, -1), (1, 0), (-1, 0), (1, 0), (-1, 0),
+                  (0, 1), (0, -1), (0, 1), (0, -1)]
+
+    def bits(v):
+        return struct.unpack("<Q", struct.pack("<d", v))[0]
+
+    def field(word, g):
+        v = (word >> (2 * g)) & 3
+        return float(v - 4 if v & 2 else v)
+
+    wx, wz = 0xDDDD, 0xDD00F5  # simplex_field(0), simplex_field(1) in bsf_device.hpp
+    rng = random.Random(7)
+    coords = [rng.random() for _ in range(100)] + [rng.random() - 1.0 for _ in range(100)]
+    coords += [0.0, -0.0, 2.0 ** -52, -1.0, 0.5, -0.5]
+    for g, (gx, gz) in enumerate(java_grad3):
+        assert (field(wx, g), field(wz, g)) == (float(gx), float(gz)), g
+        for _ in range(400):
+            x, z = rng.choice(coords), rng.choice(coords)
+            for xx, zz in ((x, z), (x, -x), (z, z), (0.0, -0.0), (-0.0, 0.0), (-0.0, -0.0)):
+                java = float(gx) * xx + float(gz) * zz
+                assert bits(libm.fma(field(wx, g), xx, field(wz, g) * zz)) == bits(java), (g, xx, zz)
+
+
 def test_scout_statistic_is_octave_zero_sum(oracle):
     """S(seed) restated from the single-point Perlin: sum over the 5x17x5 lattice of the first
     generator's octave-0 values (0.0 + v), index order x, z, y."""
