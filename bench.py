@@ -7,12 +7,14 @@ Workload (the full-pipeline configuration): a 2^36-seed contiguous range startin
 splitmix64(0xB5F00004) & (2^48 - 1); a 4x4 chunk window {-2..1}^2 around the origin;
 minimum island volume V = 200 + floor(1801 u), u = splitmix64(0xB5F00005) as a 53-bit
 double; the other island gates at the reference's SearchSettings defaults.  One step is one
-batch of consecutive seeds of that range (96 passes of 2^18 seeds by default; inputs are a seed
-range: nothing is copied to the device).  The timed region covers K steps on the production
-path (every pass one captured hipGraph, no timers); seeds/s = K * seeds per step / elapsed.
+batch of consecutive seeds of that range (96 x 2^18 seeds by default; inputs are a seed range:
+nothing is copied to the device).  The engine's pass size is its autotuner's choice among
+2^16 .. 2^19 seeds, timed on the 2^22 seeds before the range (or --pass-log2).  The timed region
+covers K steps on the production path (every pass one captured hipGraph, no timers); seeds/s =
+K * seeds per step / elapsed.
 
 The JSON line also carries:
-  roofline      FP64 VALU roofline of the dominant FP64 kernel (island Stage 2 here): its
+  roofline      FP64 VALU roofline of the dominant FP64 kernel (island Stage 1a here): its
                 launches timed with HIP events on the engine's stream over the next steps of the
                 same range (timers need the kernels launched one by one, so not inside the graphs
                 of the timed region); HBM traffic per unit from the rocprofv3 counter runs named
@@ -440,8 +442,12 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=96 << 18,
-                    help="seeds per step (default 96 passes of 2^18 seeds: 20 steps take about a minute)")
+                    help="seeds per step (default 96 x 2^18 seeds: 30 steps take about a minute)")
     ap.add_argument("--batch-log2", type=int, default=None, help="seeds per step as a power of two (overrides --batch)")
+    ap.add_argument("--pass-log2", type=int, default=None,
+                    help="engine pass size as a power of two (default: the engine autotuner's choice)")
+    ap.add_argument("--pass-tune-seeds", type=int, default=1 << 22,
+                    help="trial seeds per candidate pass size for the engine autotuner")
     ap.add_argument("--timing-steps", type=int, default=2,
                     help="steps of the same workload after the timed region with per-kernel HIP events")
     ap.add_argument("--no-cpu", action="store_true")
@@ -495,6 +501,15 @@ def main():
     start, min_blocks = island_workload()
     params = bsf.island_params(-2, -2, 4, 4, min_blocks=min_blocks)
     batch = (1 << args.batch_log2) if args.batch_log2 is not None else args.batch
+    # pass size: the engine's own autotuner (passes of 2^16 .. 2^19 seeds timed on a trial range
+    # before the timed one, each checked to give the same hit list), or a fixed size
+    if args.pass_log2 is not None:
+        bsf.engine_configure(1 << args.pass_log2)
+        pass_slots, pass_choice = 1 << args.pass_log2, "fixed (--pass-log2)"
+    else:
+        pass_slots, _ = bsf.autotune_engine(params, start - args.pass_tune_seeds, args.pass_tune_seeds)
+        pass_choice = "engine autotuner over 2^16 .. 2^19 (%d trial seeds before the timed range)" % args.pass_tune_seeds
+    progress("pass size 2^%d (%s)" % (pass_slots.bit_length() - 1, pass_choice))
 
     def step(k):
         return bsf.island_range(start + k * batch, batch, params, cap=1 << 20)
@@ -578,6 +593,8 @@ def main():
             "workload": "full floating-island pipeline: seeds [start, start+2^36) with start = splitmix64(0xB5F00004) "
                         "& (2^48-1); 4x4 chunk window {-2..1}^2; min island volume %d" % min_blocks,
             "seeds_per_step": batch,
+            "seeds_per_pass": pass_slots,
+            "pass_size_choice": pass_choice,
             "first_seed": start,
             "min_island_blocks": min_blocks,
             "hits_in_timed_region": hits,

@@ -724,9 +724,8 @@ __device__ __forceinline__ int simplex_floor(double v) {
 __host__ __device__ constexpr uint32_t simplex_field(int which) {
     uint32_t k = 0;
     for (int g = 0; g < 12; ++g) {
-        int c;
-        if (which == 0) c = g < 8 ? ((g & 1) ? -1 : 1) : 0;
-        else c = g < 4 ? ((g & 2) ? -1 : 1) : (g < 8 ? 0 : ((g & 1) ? -1 : 1));
+        const int c = which == 0 ? (g < 8 ? ((g & 1) ? -1 : 1) : 0)
+                                 : (g < 4 ? ((g & 2) ? -1 : 1) : (g < 8 ? 0 : ((g & 1) ? -1 : 1)));
         k |= (static_cast<uint32_t>(c) & 3u) << (2 * g);
     }
     return k;
