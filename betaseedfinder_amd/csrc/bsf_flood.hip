@@ -259,16 +259,82 @@ __device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, 
     return changed;
 }
 
+// One directional pass of slice x along y (UP: y ascending, else descending): each row takes the
+// blocks of rem that touch comp in the row itself, the row passed before it or the neighbouring
+// slices' rows.  Returns whether it grew.
+template <int W, bool UP>
+__device__ int pass_y(unsigned long long* comp, const unsigned long long* rem, int nx, int x) {
+    unsigned long long* row = comp + static_cast<size_t>(x) * kRowStride * W;
+    const unsigned long long* mrow = rem + static_cast<size_t>(x) * kRowStride * W;
+    const unsigned long long* left = x > 0 ? row - kRowStride * W : nullptr;
+    const unsigned long long* right = x < nx - 1 ? row + kRowStride * W : nullptr;
+    int grew = 0;
+    Z<W> prev = zzero<W>();
+    for (int k = 0; k < kLevels; ++k) {
+        const int y = UP ? k : kLevels - 1 - k;
+        const Z<W> m = zload<W>(mrow + y * W);
+        const Z<W> c = zload<W>(row + y * W);
+        Z<W> nb = zor(c, prev);
+        if (left) nb = zor(nb, zload<W>(left + y * W));
+        if (right) nb = zor(nb, zload<W>(right + y * W));
+        const Z<W> n = zfill(zand(nb, m), m);
+        if (!zeq(n, c)) {
+            zstore(row + y * W, n);
+            grew = 1;
+        }
+        prev = n;
+    }
+    return grew;
+}
+
+// The same along x for level y (FWD: x ascending, else descending), with the rows above and
+// below as neighbours.
+template <int W, bool FWD>
+__device__ int pass_x(unsigned long long* comp, const unsigned long long* rem, int nx, int y) {
+    int grew = 0;
+    Z<W> prev = zzero<W>();
+    for (int k = 0; k < nx; ++k) {
+        const int x = FWD ? k : nx - 1 - k;
+        const size_t r = (static_cast<size_t>(x) * kRowStride + y) * W;
+        const Z<W> m = zload<W>(rem + r);
+        const Z<W> c = zload<W>(comp + r);
+        Z<W> nb = zor(c, prev);
+        if (y > 0) nb = zor(nb, zload<W>(comp + r - W));
+        if (y < kLevels - 1) nb = zor(nb, zload<W>(comp + r + W));
+        const Z<W> n = zfill(zand(nb, m), m);
+        if (!zeq(n, c)) {
+            zstore(comp + r, n);
+            grew = 1;
+        }
+        prev = n;
+    }
+    return grew;
+}
+
 // Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
-// (x, y in [68, 127]) with z along the bits.  Rounds alternate the two sweeps; z runs fill in
-// one step (zfill).  Every step only adds blocks of rem adjacent to comp, so the fixpoint is
-// the component and the order of the sweeps cannot change it.
+// (x, y in [68, 127]) with z along the bits (z runs fill in one step, zfill).  Each round runs
+// the four directional passes at once, one per wave (y up and y down over the slices, waves 0 /
+// 4 and 1 / 5 taking alternate groups of 64 slices; x forward and backward over the levels,
+// waves 2 and 3).  Every step only adds blocks of rem adjacent to comp, so whatever a pass
+// reads of the others' concurrent writes, comp stays inside the component.  A round in which
+// no pass grew wrote nothing, so all four read the same comp, and each row was closed against
+// the row below and its x neighbours (y up), the row above (y down) and its x and y neighbours
+// (x passes): comp is then the whole component.
 template <int W>
 __device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
+    static_assert(kFloodThreads == 512, "the directional passes take eight waves");
+    const int w = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
     for (;;) {
-        int changed = sweep_y<W>(comp, rem, nx, 0, nx - 1);
-        __syncthreads();
-        changed |= sweep_x<W>(comp, rem, nx);
+        int changed = 0;
+        if ((w & 3) == 0) {
+            for (int x = lane + 64 * (w >> 2); x < nx; x += 128) changed |= pass_y<W, true>(comp, rem, nx, x);
+        } else if ((w & 3) == 1) {
+            for (int x = lane + 64 * (w >> 2); x < nx; x += 128) changed |= pass_y<W, false>(comp, rem, nx, x);
+        } else if (w == 2) {
+            for (int y = lane; y < kLevels; y += 64) changed |= pass_x<W, true>(comp, rem, nx, y);
+        } else if (w == 3) {
+            for (int y = lane; y < kLevels; y += 64) changed |= pass_x<W, false>(comp, rem, nx, y);
+        }
         if (!__syncthreads_or(changed)) break;
     }
 }
