@@ -259,6 +259,23 @@ __device__ int sweep_x(unsigned long long* comp, const unsigned long long* rem, 
     return changed;
 }
 
+// Number of blocks in rem and not in comp, over the whole window (block-wide sum).
+template <int W>
+__device__ int count_outside(const unsigned long long* rem, const unsigned long long* comp, int rows, int* s_total) {
+    if (threadIdx.x == 0) *s_total = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        const Z<W> m = zload<W>(rem + static_cast<size_t>(r) * W);
+        const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
+#pragma unroll
+        for (int i = 0; i < W; ++i) mine += __popcll(m.v[i] & ~c.v[i]);
+    }
+    atomicAdd(s_total, mine);
+    __syncthreads();
+    return *s_total;
+}
+
 // One directional pass of slice x along y (UP: y ascending, else descending): each row takes the
 // blocks of rem that touch comp in the row itself, the row passed before it or the neighbouring
 // slices' rows.  Returns whether it grew.
@@ -312,7 +329,9 @@ __device__ int pass_x(unsigned long long* comp, const unsigned long long* rem, i
 }
 
 // Grow comp to its fixpoint inside rem: 6-neighbour connectivity over the rows of the window
-// (x, y in [68, 127]) with z along the bits (z runs fill in one step, zfill).  Each round runs
+// (x, y in [68, 127]) with z along the bits (z runs fill in one step, zfill).  Returns false,
+// without finishing, when after the first round fewer than min_blocks blocks of rem lie outside
+// comp (comp only grows, so that bounds what the fixpoint leaves outside).  Each round runs
 // the four directional passes at once, one per wave (y up and y down over the slices, waves 0 /
 // 4 and 1 / 5 taking alternate groups of 64 slices; x forward and backward over the levels,
 // waves 2 and 3).  Every step only adds blocks of rem adjacent to comp, so whatever a pass
@@ -321,10 +340,11 @@ __device__ int pass_x(unsigned long long* comp, const unsigned long long* rem, i
 // the row below and its x neighbours (y up), the row above (y down) and its x and y neighbours
 // (x passes): comp is then the whole component.
 template <int W>
-__device__ void grow_component(unsigned long long* comp, const unsigned long long* rem, int nx) {
+__device__ bool grow_component(unsigned long long* comp, const unsigned long long* rem, int nx, int rows,
+                               int min_blocks, int* s_total) {
     static_assert(kFloodThreads == 512, "the directional passes take eight waves");
     const int w = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
-    for (;;) {
+    for (bool first = true;; first = false) {
         int changed = 0;
         if ((w & 3) == 0) {
             for (int x = lane + 64 * (w >> 2); x < nx; x += 128) changed |= pass_y<W, true>(comp, rem, nx, x);
@@ -336,7 +356,9 @@ __device__ void grow_component(unsigned long long* comp, const unsigned long lon
             for (int y = lane; y < kLevels; y += 64) changed |= pass_x<W, false>(comp, rem, nx, y);
         }
         if (!__syncthreads_or(changed)) break;
+        if (first && count_outside<W>(rem, comp, rows, s_total) < min_blocks) return false;
     }
+    return true;
 }
 
 // The same for a connected comp whose blocks lie in slices *s_lo .. *s_hi (a connected set's
@@ -356,23 +378,6 @@ __device__ void grow_component_in(unsigned long long* comp, const unsigned long 
         changed |= sweep_x<W>(comp, rem, nx, *s_lo, *s_hi, s_lo, s_hi);
         if (!__syncthreads_or(changed)) break;
     }
-}
-
-// Number of blocks in rem and not in comp, over the whole window (block-wide sum).
-template <int W>
-__device__ int count_outside(const unsigned long long* rem, const unsigned long long* comp, int rows, int* s_total) {
-    if (threadIdx.x == 0) *s_total = 0;
-    __syncthreads();
-    int mine = 0;
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-        const Z<W> m = zload<W>(rem + static_cast<size_t>(r) * W);
-        const Z<W> c = zload<W>(comp + static_cast<size_t>(r) * W);
-#pragma unroll
-        for (int i = 0; i < W; ++i) mine += __popcll(m.v[i] & ~c.v[i]);
-    }
-    atomicAdd(s_total, mine);
-    __syncthreads();
-    return *s_total;
 }
 
 // Shared state of one workgroup's flood.
@@ -430,14 +435,10 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
         zstore(comp + static_cast<size_t>(r) * W, v);
     }
     __syncthreads();
-    // 2a. one sweep along y from the contacts already reaches most grounded stone; what is
-    //     left bounds the floating stone from above, and fewer than minBlocks floating
-    //     blocks cannot hold a match
-    sweep_y<W>(comp, rem, nx, 0, nx - 1);
-    __syncthreads();
-    if (count_outside<W>(rem, comp, rows, &sh.total) < p.min_blocks) return false;
-    // 2b. the ground components in full
-    grow_component<W>(comp, rem, nx);
+    // The first round from the contacts already reaches most grounded stone; what it leaves
+    // bounds the floating stone from above, and fewer than minBlocks floating blocks cannot
+    // hold a match (most windows end there).  Otherwise the ground components in full.
+    if (!grow_component<W>(comp, rem, nx, rows, p.min_blocks, &sh.total)) return false;
     // 3. all remaining stone is floating
     const int floating = count_outside<W>(rem, comp, rows, &sh.total);
     if (floating < p.min_blocks) return false;
