@@ -101,29 +101,46 @@ __device__ __forceinline__ Z<W> zfill(const Z<W>& s, const Z<W>& m) {
     return r;
 }
 
-// Word wi of the eight block rows y = 8*k1 .. 8*k1+7 of window x (stone z-masks of chunks
-// 4 wi .. 4 wi + 3 in z) from one lattice y-cell per chunk (cell_stone_bits).
-__device__ void stone_cell_rows(const double* __restrict__ lat, int x, int k1, int cz_n, int wi,
-                                unsigned long long rows[8]) {
-    const int cxi = x >> 4, lx = x & 15;
-    const int i1 = lx >> 2, i2 = lx & 3;
+// The stone blocks of one lattice y-cell for all four block columns along x at once: out[i2]
+// holds cell_stone_bits(..., i2) (bit 4 l1 + k2).  The same sums in the same order (Java's
+// d10 += d12 walk over i2 is shared by the four columns instead of redone per column).
+__device__ __forceinline__ void cell_stone_bits4(double d1, double d2, double d3, double d4, double e1, double e2,
+                                                 double e3, double e4, uint32_t out[4]) {
+    const double mx = fmax(fmax(fmax(d1, d2), fmax(d3, d4)), fmax(fmax(e1, e2), fmax(e3, e4)));
+    const double mn = fmin(fmin(fmin(d1, d2), fmin(d3, d4)), fmin(fmin(e1, e2), fmin(e3, e4)));
+    if (mx <= -kSignMargin || mn > kSignMargin) {
+        const uint32_t v = mx <= -kSignMargin ? 0u : 0xFFFFFFFFu;
 #pragma unroll
-    for (int l1 = 0; l1 < 8; ++l1) rows[l1] = 0;
-    const int cz_end = 4 * wi + 4 < cz_n ? 4 * wi + 4 : cz_n;
-    for (int czi = 4 * wi; czi < cz_end; ++czi) {
-        const double* cl = lat + static_cast<size_t>(cxi * cz_n + czi) * 25 * 8;
-        for (int j1 = 0; j1 < 4; ++j1) {
-            auto L = [&](int ii, int jj, int kk) -> double {
-                return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
-            };
-            const uint32_t m = cell_stone_bits(L(i1, j1, k1), L(i1, j1 + 1, k1), L(i1 + 1, j1, k1), L(i1 + 1, j1 + 1, k1),
-                                               L(i1, j1, k1 + 1), L(i1, j1 + 1, k1 + 1), L(i1 + 1, j1, k1 + 1),
-                                               L(i1 + 1, j1 + 1, k1 + 1), i2);
-            if (m == 0) continue;
-            const int sh = (czi - 4 * wi) * 16 + j1 * 4;
+        for (int i2 = 0; i2 < 4; ++i2) out[i2] = v;
+        return;
+    }
+    const double d5 = (e1 - d1) * 0.125;
+    const double d6 = (e2 - d2) * 0.125;
+    const double d7 = (e3 - d3) * 0.125;
+    const double d8 = (e4 - d4) * 0.125;
 #pragma unroll
-            for (int l1 = 0; l1 < 8; ++l1) rows[l1] |= static_cast<unsigned long long>((m >> (4 * l1)) & 0xFu) << sh;
+    for (int i2 = 0; i2 < 4; ++i2) out[i2] = 0;
+#pragma unroll
+    for (int l1 = 0; l1 < 8; ++l1) {
+        const double d12 = (d3 - d1) * 0.25;
+        const double d13 = (d4 - d2) * 0.25;
+        double d10 = d1, d11 = d2;
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2) {
+            const double d16 = (d11 - d10) * 0.25;
+            double d15 = d10;
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) {
+                if (d15 > 0.0) out[i2] |= 1u << (4 * l1 + k2);
+                d15 += d16;
+            }
+            d10 += d12;
+            d11 += d13;
         }
+        d1 += d5;
+        d2 += d6;
+        d3 += d7;
+        d4 += d8;
     }
 }
 
@@ -408,21 +425,59 @@ __device__ bool flood_window(unsigned long long* const base, const double* __res
     unsigned long long* const st67 = base + static_cast<size_t>(2 * rows) * W;
     const int minWZ = p.chunk_z0 * 16;
 
-    // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15
-    for (int task = tid; task < nx * 8 * W; task += blockDim.x) {
-        // level-major: the lanes of a wave share one y-cell level, so the all-air levels above
-        // the terrain take the short path together
-        const int x = task % nx, rest = task / nx;
-        const int k1 = 8 + rest % 8, wi = rest / 8;
-        unsigned long long r8[8];
-        stone_cell_rows(lat, x, k1, p.chunks_z, wi, r8);
+    // 1. upsample the window: blocks y in [67, 127] from lattice y-cells 8..15.  A task is one
+    //    y-cell level of one x cell (four block columns) in one 16-block z slot (one chunk): its
+    //    four lattice cells along z, each for all four block columns (cell_stone_bits4), written
+    //    as the 16-bit pieces of the rows' z words that the slot covers (zero past the window).
+    //    Level-major: the lanes of a wave share a level, so the all-air levels above the
+    //    terrain take the short path together.
+    {
+        const int nxc = nx >> 2, nzc = 4 * W;
+        uint16_t* const rem16 = reinterpret_cast<uint16_t*>(rem);
+        uint16_t* const st16 = reinterpret_cast<uint16_t*>(st67);
+        for (int task = tid; task < 8 * nxc * nzc; task += blockDim.x) {
+            const int k1 = 8 + task / (nxc * nzc), rest = task % (nxc * nzc);
+            const int xc = rest % nxc, zc = rest / nxc;
+            uint32_t piece[4][8];
 #pragma unroll
-        for (int l1 = 0; l1 < 8; ++l1) {
-            const int y = k1 * 8 + l1;
-            if (y >= kFloorY) rem[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W + wi] = r8[l1];
-            else if (y == kFloorY - 1) st67[static_cast<size_t>(x) * W + wi] = r8[l1];
+            for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+                for (int l1 = 0; l1 < 8; ++l1) piece[i2][l1] = 0;
+            if (zc < p.chunks_z) {
+                const int i1 = xc & 3;
+                const double* cl = lat + static_cast<size_t>((xc >> 2) * p.chunks_z + zc) * 25 * 8;
+                auto L = [&](int ii, int jj, int kk) -> double {
+                    return kk >= 16 ? -10.0 : cl[(ii * 5 + jj) * 8 + (kk - 8)];
+                };
+#pragma unroll 1
+                for (int j1 = 0; j1 < 4; ++j1) {
+                    uint32_t m[4];
+                    cell_stone_bits4(L(i1, j1, k1), L(i1, j1 + 1, k1), L(i1 + 1, j1, k1), L(i1 + 1, j1 + 1, k1),
+                                     L(i1, j1, k1 + 1), L(i1, j1 + 1, k1 + 1), L(i1 + 1, j1, k1 + 1),
+                                     L(i1 + 1, j1 + 1, k1 + 1), m);
+#pragma unroll
+                    for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+                        for (int l1 = 0; l1 < 8; ++l1) piece[i2][l1] |= ((m[i2] >> (4 * l1)) & 0xFu) << (4 * j1);
+                }
+            }
+            // piece zc of word zc / 4 of a row: its 16-bit half-word 4 (row word) + zc % 4
+            const int wq = (zc >> 2) * 4 + (zc & 3);
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+                const int x = 4 * xc + i2;
+#pragma unroll
+                for (int l1 = 0; l1 < 8; ++l1) {
+                    const int y = k1 * 8 + l1;
+                    if (y >= kFloorY)
+                        rem16[(static_cast<size_t>(x) * kRowStride + (y - kFloorY)) * W * 4 + wq] =
+                                static_cast<uint16_t>(piece[i2][l1]);
+                    else if (y == kFloorY - 1)
+                        st16[static_cast<size_t>(x) * W * 4 + wq] = static_cast<uint16_t>(piece[i2][l1]);
+                }
+                if (k1 == 8) rem16[(static_cast<size_t>(x) * kRowStride + kLevels) * W * 4 + wq] = 0;  // row padding
+            }
         }
-        if (k1 == 8) rem[(static_cast<size_t>(x) * kRowStride + kLevels) * W + wi] = 0;  // row padding
     }
     __syncthreads();
     // 2. every component that reaches a y = 68 block with stone below touches the bottom
