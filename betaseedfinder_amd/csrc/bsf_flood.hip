@@ -677,7 +677,7 @@ __device__ bsf_island_hit make_hit(int64_t seed, const bsf_island_params& p, con
 }
 
 template <int W, bool GLOBAL>
-__global__ __launch_bounds__(kFloodThreads) void pass_flood_kernel(EngineArgs a, int cls) {
+__global__ __launch_bounds__(kFloodThreads, 4) void pass_flood_kernel(EngineArgs a, int cls) {
     extern __shared__ __attribute__((aligned(16))) unsigned long long flood_lds[];
     __shared__ FloodShared<W> sh;
     __shared__ unsigned long long s_item;
