@@ -542,45 +542,36 @@ struct YTable {
     uint32_t yzero;  // levels whose cell starts at a zero fraction (no FMA gradients there)
 };
 
-// Called by every lane of wave 0; levels [y0, y0 + NY).
+// Called by every lane of wave 0; levels [y0, y0 + NY).  Lane y takes level y for every level
+// from 0 up (y0 + NY <= 64): a level starts a cell when its masked index differs from the level
+// below (level 0 always), and the cell-start fraction of a table level is the fraction at the
+// last start at or below it, which for level y0 may lie below y0 -- the levels below y0 that
+// share its index, as Java's cell walk finds them.
 template <int NY>
 __device__ __forceinline__ void build_ytable_n(YTable& t, double sy, double b, int y0) {
     static_assert(NY <= kMaxLevels, "too many levels");
-    const int Y0 = y0;
-    const int k = static_cast<int>(threadIdx.x);
-    const bool mine = k < NY;
-    const double v = static_cast<double>(Y0 + (mine ? k : 0)) * sy + b;
+    const int y = static_cast<int>(threadIdx.x & 63);
+    const bool mine = y < y0 + NY;
+    const double v = static_cast<double>(mine ? y : 0) * sy + b;
     const int fl = java_floor(v);
     const int idx = fl & 255;
     const double f = v - static_cast<double>(fl);
     const int idx_prev = __shfl_up(idx, 1);
-    const bool fresh = mine && (k == 0 || idx != idx_prev);
-    const uint32_t mask = static_cast<uint32_t>(__ballot(fresh));
-    // the cell of level Y0 may begin below Y0: walk back while the masked index stays the same
-    double v0 = static_cast<double>(Y0) * sy + b;
-    const int fl0 = java_floor(v0);
-    double start0 = v0 - static_cast<double>(fl0);
-    if (Y0 > 0) {
-        for (int y = Y0 - 1; y >= 0; --y) {
-            const double vy = static_cast<double>(y) * sy + b;
-            const int fy = java_floor(vy);
-            if ((fy & 255) != (fl0 & 255)) break;
-            start0 = vy - static_cast<double>(fy);
-        }
-    }
-    const int kk = mine ? k : 0;
-    const int start = 31 - __clz(static_cast<int>(mask & ((2u << kk) - 1u)));
-    double cf = __shfl(f, start);
-    if (start == 0) cf = start0;
-    if (mine) {
+    const unsigned long long starts = __ballot(mine && (y == 0 || idx != idx_prev));
+    const int start = 63 - __clzll(static_cast<long long>(starts & ((2ULL << (mine ? y : 0)) - 1ULL)));
+    const double cf = __shfl(f, start);
+    const int k = y - y0;
+    const bool tab = mine && k >= 0;
+    if (tab) {
         t.cell_f[k] = cf;
         t.cell_fm1[k] = cf - 1.0;
         t.fd[k] = fade(f);
         t.yi[k] = idx;
     }
-    const uint32_t zero = static_cast<uint32_t>(__ballot(mine && cf == 0.0));
-    if (k == 0) {
-        t.fresh = mask;
+    const uint32_t levels = (1u << NY) - 1u;
+    const uint32_t zero = static_cast<uint32_t>(__ballot(tab && cf == 0.0) >> y0) & levels;
+    if (y == 0) {
+        t.fresh = (static_cast<uint32_t>(starts >> y0) | 1u) & levels;
         t.yzero = zero;
     }
 }
@@ -599,51 +590,44 @@ struct YTab {
 };
 
 // The tables of n octaves over levels [y0, y0 + NY) at once: octave q has y scale
-// ldexp(sy0, -oct(q)) and y offset boff(q).  Each of the 64 lanes of the calling wave takes one
-// (octave, level) pair, 64 / NY octaves per round; the values are build_ytable_n's.  Every lane
-// of the (one-wave) workgroup calls it; a barrier must follow before the tables are read.
+// ldexp(sy0, -oct(q)) and y offset boff(q).  The lanes of the calling wave take (octave, level)
+// pairs for the levels 0 .. y0 + NY - 1 of 64 / (y0 + NY) octaves per round; the values are
+// build_ytable_n's.  Every lane of the (one-wave) workgroup calls it; a barrier must follow before
+// the tables are read.
 template <int NY, class OctF, class OffF>
 __device__ __forceinline__ void build_ytabs(YTab<NY>* tabs, int n, double sy0, int y0, OctF oct, OffF boff) {
-    static_assert(64 % NY == 0 && NY <= 16, "levels per table must divide the wave");
-    constexpr int kPer = 64 / NY;
+    static_assert(NY <= 16, "levels per table");
+    const int L = y0 + NY;  // levels per octave, at most 64
+    const int per = 64 / L;
     const int lane = static_cast<int>(threadIdx.x & 63);
-    const int ql = lane / NY, k = lane % NY;
-    for (int q0 = 0; q0 < n; q0 += kPer) {
+    const int ql = lane / L, y = lane % L;
+    const unsigned long long gmask = L == 64 ? ~0ULL : ((1ULL << L) - 1ULL);
+    const uint32_t levels = (1u << NY) - 1u;
+    for (int q0 = 0; q0 < n; q0 += per) {
         const int q = q0 + ql;
-        const bool mine = q < n;
+        const bool mine = ql < per && q < n;
         const int qq = mine ? q : n - 1;
         const double sy = ldexp(sy0, -oct(qq));
         const double b = boff(qq);
-        const double v = static_cast<double>(y0 + k) * sy + b;
+        const double v = static_cast<double>(y) * sy + b;
         const int fl = java_floor(v);
         const int idx = fl & 255;
         const double f = v - static_cast<double>(fl);
         const int idx_prev = __shfl_up(idx, 1);
-        const bool fresh = mine && (k == 0 || idx != idx_prev);
-        const uint32_t gm = static_cast<uint32_t>(__ballot(fresh) >> (ql * NY)) & ((1u << NY) - 1u);
-        // the cell of level y0 may begin below y0: walk back while the masked index stays the same
-        double start0 = f;
-        if (k == 0 && y0 > 0) {
-            for (int y = y0 - 1; y >= 0; --y) {
-                const double vy = static_cast<double>(y) * sy + b;
-                const int fy = java_floor(vy);
-                if ((fy & 255) != idx) break;
-                start0 = vy - static_cast<double>(fy);
-            }
-        }
-        const int start = 31 - __clz(static_cast<int>(gm & ((2u << k) - 1u)));
-        double cf = __shfl(f, ql * NY + start);
-        const double s0 = __shfl(start0, ql * NY);
-        if (start == 0) cf = s0;
-        const uint32_t zm = static_cast<uint32_t>(__ballot(mine && cf == 0.0) >> (ql * NY)) & ((1u << NY) - 1u);
-        if (mine) {
+        const unsigned long long gs = (__ballot(mine && (y == 0 || idx != idx_prev)) >> (ql * L)) & gmask;
+        const int start = 63 - __clzll(static_cast<long long>(gs & ((2ULL << y) - 1ULL)));
+        const double cf = __shfl(f, ql * L + start);
+        const int k = y - y0;
+        const bool tab = mine && k >= 0;
+        const uint32_t zm = static_cast<uint32_t>(((__ballot(tab && cf == 0.0) >> (ql * L)) & gmask) >> y0) & levels;
+        if (tab) {
             YTab<NY>& t = tabs[q];
             t.cell_f[k] = cf;
             t.cell_fm1[k] = cf - 1.0;
             t.fd[k] = fade(f);
             t.yi[k] = idx;
             if (k == 0) {
-                t.fresh = gm;
+                t.fresh = (static_cast<uint32_t>(gs >> y0) | 1u) & levels;
                 t.yzero = zm;
             }
         }
