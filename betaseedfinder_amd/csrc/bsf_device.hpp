@@ -671,6 +671,60 @@ __device__ __forceinline__ void perlin3_table_cells(Perm p, const uint2* gc, con
         out[k] = lerp(az.fd, r0, r1) * weight;
     }
 }
+// An octave's values at the levels of one column for bounds only (not bit-exact): within a Y
+// cell Java's lerp(fz, lerp(fd, q00, q01), lerp(fd, q10, q11)) is the polynomial A + fd B with
+// A = lerp(fz, q00, q10) and B = (q01 - q00) + fz ((q11 - q10) - (q01 - q00)), so each level takes
+// one FMA of the cell's weighted A and B (the weight a power of two).  The result differs from
+// Java's by a few units in the last place of the corner values.  The corners are exact as in
+// perlin3_table_cells.
+template <int Y0, int Y1, int FORM, class Perm, class YT>
+__device__ __forceinline__ void perlin3_bound_cells(Perm p, const uint2* gc, const Axis& ax, const Axis& az,
+                                                    const YT& yt, uint32_t fresh, double weight, double* out) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+    double A = 0.0, B = 0.0;
+#pragma unroll
+    for (int k = 0; k < Y1 - Y0; ++k) {
+        if ((fresh >> k) & 1u) {
+            const int yi = yt.yi[k];
+            const double yf = yt.cell_f[k];
+            const double yfm1 = yt.cell_fm1[k];
+            const int p0 = xp0 + yi;
+            const int p00 = p.get(p0) + az.i;
+            const int p01 = p.get(p0 + 1) + az.i;
+            const int p1 = xp1 + yi;
+            const int p10 = p.get(p1) + az.i;
+            const int p11 = p.get(p1 + 1) + az.i;
+            const double q00 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00, ax.f, yf, az.f),
+                                    corner_grad<FORM>(p, gc, p10, ax.fm1, yf, az.f));
+            const double q01 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01, ax.f, yfm1, az.f),
+                                    corner_grad<FORM>(p, gc, p11, ax.fm1, yfm1, az.f));
+            const double q10 = lerp(ax.fd, corner_grad<FORM>(p, gc, p00 + 1, ax.f, yf, az.fm1),
+                                    corner_grad<FORM>(p, gc, p10 + 1, ax.fm1, yf, az.fm1));
+            const double q11 = lerp(ax.fd, corner_grad<FORM>(p, gc, p01 + 1, ax.f, yfm1, az.fm1),
+                                    corner_grad<FORM>(p, gc, p11 + 1, ax.fm1, yfm1, az.fm1));
+            const double d0 = q01 - q00, d1 = q11 - q10;
+            A = lerp(az.fd, q00, q10) * weight;
+            B = (d0 + az.fd * (d1 - d0)) * weight;
+        }
+        out[k] = __fma_rn(yt.fd[k], B, A);
+    }
+}
+// perlin3_column_table's dispatch for the bound form.
+template <int Y0, int Y1, int FORM, class Perm, class YT>
+__device__ __forceinline__ void perlin3_column_bound(Perm p, const Axis& ax, const Axis& az, const YT& yt,
+                                                     uint32_t fresh, double weight, double* out,
+                                                     const uint2* gc = nullptr) {
+    if constexpr (FORM == kGradSelect) {
+        perlin3_bound_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
+    } else {
+        if ((yt.yzero & fresh) == 0u && __all(ax.f != 0.0 && az.f != 0.0))
+            perlin3_bound_cells<Y0, Y1, FORM>(p, gc, ax, az, yt, fresh, weight, out);
+        else
+            perlin3_bound_cells<Y0, Y1, kGradSelect>(p, gc, ax, az, yt, fresh, weight, out);
+    }
+}
+
 // FORM kGradTable needs gc (the grad3_coefficients table); the table and bitfield forms fall
 // back to the case selection for a call with a zero corner coordinate somewhere in the wave.
 template <int Y0, int Y1, int FORM, class Perm, class YT>

@@ -331,8 +331,9 @@ __device__ __forceinline__ bool stage1a_open(const double* d9, const double* b, 
 // octaves kSplitOctave + 1..15 and, only when some column of the wave is still open, octave
 // kSplitOctave.  Returns whether an upward air-to-stone crossing at y >= 64 is still possible.
 // The noise2 / noise3 sums here only feed the bounds (Stage 1b sums all 16 octaves in Java
-// order), so they are accumulated blended, in the order evaluated: the rounding differs from
-// Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
+// order), so they are accumulated blended, in the order evaluated, and each octave's level values
+// take one FMA per level from its Y cell's corners (perlin3_column_bound): the rounding differs
+// from Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
 // stash: the wave's LDS area of 16 x 64 doubles, where d9 and d12 wait (lane-minor) while
 // noise2 / noise3 are evaluated, so that they hold no registers there.
 template <class S>
@@ -398,7 +399,7 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
             const Axis ax = axis(X * sx + s.off(0));
             const Axis az = axis(Z * sx + s.off(2));
             double v[8];
-            perlin3_column_table<8, 16, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
+            perlin3_column_bound<8, 16, S::kGradForm>(s.perm(), ax, az, yt, fresh, w, v, s.gcoef());
 #pragma unroll
             for (int k = 0; k < 8; ++k) acc[k] += v[k];
         }

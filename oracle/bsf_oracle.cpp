@@ -1930,15 +1930,27 @@ void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
 // 599-664) looking for stone at y >= 68 over non-stone.  A column is open when some level k has
 // lo[k] <= 0 < hi[k + 1].  The x axis of an octave is formed once per distinct lattice x, the y
 // axis once per octave (Stage 1a: shared by the seed's columns; Stage 1b: per column, each open
-// column being evaluated alone).
+// column being evaluated alone).  Stage 1a's noise2 / noise3 octaves only feed the bounds and take
+// one FMA per level from their Y cell's corners (octaves3dAt's bound form), as the kernel does.
 // ---------------------------------------------------------------------------
 constexpr int kSplitOctave1 = 10;
 
+template <class T> T fmaT(T a, T b, T c);
+template <> inline double fmaT<double>(double a, double b, double c) { return std::fma(a, b, c); }
+template <> inline CD fmaT<CD>(CD a, CD b, CD c) {
+    ++t_tally->mul;
+    ++t_tally->add;
+    return CD(std::fma(a.v, b.v, c.v));
+}
+
 // 3D octaves [o0, o1) of one group at lattice columns cols, levels 8..15, accumulated into acc
-// (the first evaluated octave starts each sum at 0.0 + value).
+// (the first evaluated octave starts each sum at 0.0 + value).  With bound (values that only feed
+// Stage 1a's bounds), each level is A + fd B from its Y cell's weighted A = lerp(fz, q00, q10) and
+// B = (q01 - q00) + fz ((q11 - q10) - (q01 - q00)), one FMA (counted as a multiply and an add),
+// as the Stage-1a kernel forms it; otherwise Java's three lerps.
 template <class T>
 void octaves3dAt(const Octaves& oct, double sxz, double sy0, int o0, int o1, const std::vector<std::pair<int, int>>& cols,
-                 std::vector<std::array<T, kS2Levels>>& acc) {
+                 std::vector<std::array<T, kS2Levels>>& acc, bool bound = false) {
     T amp = T(1.0);
     for (int o = 0; o < o0; ++o) amp = amp / T(2.0);
     for (int o = o0; o < o1; ++o) {
@@ -1974,7 +1986,7 @@ void octaves3dAt(const Octaves& oct, double sxz, double sy0, int o0, int o1, con
             const AxisT<T>& ax = xa->second;
             const AxisT<T> az = axisT(i2d<T>(Z) * sx + T(p.c));
             auto& out = acc[c];
-            T q00 = T(0.0), q01 = T(0.0), q10 = T(0.0), q11 = T(0.0);
+            T q00 = T(0.0), q01 = T(0.0), q10 = T(0.0), q11 = T(0.0), A = T(0.0), B = T(0.0);
             for (int k = 0; k < kS2Levels; ++k) {
                 if (fresh[k]) {
                     const int p60 = p.d[ax.i] + yi[k], p61 = p.d[p60] + az.i, p62 = p.d[p60 + 1] + az.i;
@@ -1985,9 +1997,19 @@ void octaves3dAt(const Octaves& oct, double sxz, double sy0, int o0, int o1, con
                     q10 = lerpT(ax.fd, grad3T(p.d[p61 + 1], ax.f, yf, az.fm1), grad3T(p.d[p33 + 1], ax.fm1, yf, az.fm1));
                     q11 = lerpT(ax.fd, grad3T(p.d[p62 + 1], ax.f, yfm1, az.fm1),
                                 grad3T(p.d[p64 + 1], ax.fm1, yfm1, az.fm1));
+                    if (bound) {
+                        const T d0 = q01 - q00, d1 = q11 - q10;
+                        A = lerpT(az.fd, q00, q10) * w;
+                        B = (d0 + az.fd * (d1 - d0)) * w;
+                    }
                 }
-                const T r0 = lerpT(fd[k], q00, q01), r1 = lerpT(fd[k], q10, q11);
-                const T val = lerpT(az.fd, r0, r1) * w;
+                T val;
+                if (bound) {
+                    val = fmaT(fd[k], B, A);
+                } else {
+                    const T r0 = lerpT(fd[k], q00, q01), r1 = lerpT(fd[k], q10, q11);
+                    val = lerpT(az.fd, r0, r1) * w;
+                }
                 out[k] = o == o0 ? T(0.0) + val : out[k] + val;
             }
         }
@@ -2139,9 +2161,9 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         }
         return op;
     };
-    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g);
+    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g, true);
     fold(false);
-    octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g);
+    octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g, true);
     fold(true);
     bool any = false;
     for (size_t c = 0; c < nc; ++c) {
@@ -2149,9 +2171,9 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         any = any || open[c];
     }
     if (any) {
-        octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g);
+        octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g, true);
         fold(false);
-        octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g);
+        octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g, true);
         fold(true);
         for (size_t c = 0; c < nc; ++c)
             if (open[c]) open[c] = bounds_open(c, T(2.0 * ((1 << kSplitOctave1) - 1))) ? 1 : 0;
