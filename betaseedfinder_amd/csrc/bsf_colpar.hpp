@@ -182,9 +182,10 @@ struct ColAcc {
 
 // 3D octave group g0 over octaves [o0, octaves): with o0 > 0 the sum of the
 // low-frequency part only (octave o carries weight 2^o; amp halves exactly per octave).
+// next_g (optional, per-octave Y tables only): the generator to prefetch after the last octave.
 template <int Y0, int Y1, class S>
 __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
-                                          double Z, bool active, double* acc, int o0 = 0) {
+                                          double Z, bool active, double* acc, int o0 = 0, int next_g = -1) {
     constexpr int NY = Y1 - Y0;
     if constexpr (S::kUniform && S::kYTabLevels == NY) {
         // the group's Y-axis tables at once (lane = octave x level); each octave then stages its
@@ -214,7 +215,8 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
             if constexpr (S::kUniform) {
                 // one world per workgroup: the Y axis is built once, in LDS, and its cell runs
                 // are the same on every lane, so the corner-reuse branch is scalar
-                s.template stage_y<Y0, Y1>(g0 + o, sy, o + 1 < octaves ? 1 : kStageBatch);
+                if (o + 1 < octaves || next_g < 0) s.template stage_y<Y0, Y1>(g0 + o, sy, o + 1 < octaves ? 1 : kStageBatch);
+                else s.template stage_y<Y0, Y1>(g0 + o, sy, 1, next_g);
                 if (active) {
                     const YTable& yt = s.ytable();
                     const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);

@@ -326,75 +326,71 @@ __device__ __forceinline__ bool stage1a_open(const double* d9, const double* b, 
     return open;
 }
 
-// Stage 1a of one sample column: exact noise1 / noise4 / noise5 / climate, hence the column's
-// exact height biases d9 and blend factors d12; then the blend of noise2 and noise3 over
-// octaves kSplitOctave + 1..15 and, only when some column of the wave is still open, octave
-// kSplitOctave.  Returns whether an upward air-to-stone crossing at y >= 64 is still possible.
+// Whether some level can still hold an upward crossing when the blend factor is not known: for
+// any c in [0, 1] the blend (1 - c) n2 + c n3 lies between the two sums, so with each known to
+// within u the density lies within [min(s2, s3) - u, max(s2, s3) + u] / 512 - d9 (tapered).
+__device__ __forceinline__ bool stage1a_open_hull(const double* d9, const double* s2, const double* s3, double u) {
+    double lo[9], hi[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        lo[k] = top_taper((fmin(s2[k], s3[k]) - u) / 512.0 - d9[k], 8 + k) - kSplitSlack;
+        hi[k] = top_taper((fmax(s2[k], s3[k]) + u) / 512.0 - d9[k], 8 + k) + kSplitSlack;
+    }
+    lo[8] = hi[8] = -10.0;
+    bool open = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) open |= lo[k] <= 0.0 && hi[k + 1] > 0.0;
+    return open;
+}
+
+// Stage 1a of one sample column.  First noise4 / noise5 / climate, hence the column's exact height
+// biases d9, and noise2 and noise3 over octaves kSplitOctave + 1..15; a wave whose columns the
+// hull of the two sums closes (the blend factor unknown) stops there, without noise1 (a quarter of
+// the seeds).  Otherwise exact noise1, hence the blend factors d12, the blended bound and, only
+// when some column of the wave is still open, octave kSplitOctave.  Returns whether an upward
+// air-to-stone crossing at y >= 64 is still possible; d9 / d12 are exact for such a column.
 // The noise2 / noise3 sums here only feed the bounds (Stage 1b sums all 16 octaves in Java
-// order), so they are accumulated blended, in the order evaluated, and each octave's level values
-// take one FMA per level from its Y cell's corners (perlin3_column_bound): the rounding differs
-// from Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
-// stash: the wave's LDS area of 16 x 64 doubles, where d9 and d12 wait (lane-minor) while
-// noise2 / noise3 are evaluated, so that they hold no registers there.
+// order), so they are accumulated in the order evaluated, and each octave's level values take one
+// FMA per level from its Y cell's corners (perlin3_column_bound): the rounding differs from
+// Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
+// stash: the wave's LDS area of 16 x 64 doubles (lane-minor): d9 in rows 0..7; the noise2 sums in
+// rows 8..15 while noise3 and noise1 are evaluated, then d12 there, so that they hold no
+// registers meanwhile.
 template <class S>
 __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i, int l, bool active, double* d9,
                                                double* d12, double* stash) {
     const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
     const double BX = static_cast<double>(cx * 16 + 3 * i + 1), BZ = static_cast<double>(cz * 16 + 3 * l + 1);
-    double n1[8];
-    colpar_3d<8, 16>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, n1);
-    const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
-    const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
     // steps 0..4: noise2 octaves 11..15, 5..9: noise3 octaves 11..15, 10: noise2 octave 10,
-    // 11: noise3 octave 10 (only if a column is open after step 9)
+    // 11: noise3 octave 10
     constexpr int kHigh = 15 - kSplitOctave;
     auto step_gen = [](int step) {
         const bool n3 = step < 2 * kHigh ? step >= kHigh : step == 2 * kHigh + 1;
         const int o = step < 2 * kHigh ? kSplitOctave + 1 + step % kHigh : kSplitOctave;
         return (n3 ? kGenN3 : kGenN2) + o;
     };
-    double temp = 0.0, rain = 0.0;
-    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain, step_gen(0));
-    const ColumnShape cs = column_shape(temp, rain, n4, n5);
-    const int lane = threadIdx.x & 63;
-    double b[8], acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        stash[k * 64 + lane] = level_bias(cs, 8 + k);
-        stash[(8 + k) * 64 + lane] = n1_blend(n1[k]);
-        b[k] = 0.0;
-        acc[k] = 0.0;
-    }
     auto step_oct = [](int step) { return step < 2 * kHigh ? kSplitOctave + 1 + step % kHigh : kSplitOctave; };
-    constexpr bool kTabs = S::kYTabLevels == 8;
-    if constexpr (kTabs) {
-        // the Y-axis tables of steps 0..9 at once (the stage of step 0 publishes them)
-        __syncthreads();
-        build_ytabs<8>(s.ytabs, 2 * kHigh, 684.412, 8, step_oct, [&](int q) { return s.offs[step_gen(q) * 3 + 1]; });
-    }
-    bool open = active;
-    int step = 0;
-    for (; step < 2 * kHigh + 2; ++step) {
-        if (step == 2 * kHigh) {
+    const int lane = threadIdx.x & 63;
+    {
+        const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
+        const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
+        double temp = 0.0, rain = 0.0;
+        colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain, step_gen(0));
+        const ColumnShape cs = column_shape(temp, rain, n4, n5);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) d9[k] = stash[k * 64 + lane];
-            open = open && stage1a_open(d9, b, kSplitBoundCoarse);
-            if (!__any(open)) break;
-            if constexpr (kTabs)
-                build_ytabs<8>(s.ytabs + 2 * kHigh, 2, 684.412, 8, [&](int q) { return step_oct(2 * kHigh + q); },
-                               [&](int q) { return s.offs[step_gen(2 * kHigh + q) * 3 + 1]; });
-        }
+        for (int k = 0; k < 8; ++k) stash[k * 64 + lane] = level_bias(cs, 8 + k);
+    }
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    // one bound-only octave of a step
+    auto octave = [&](int step, int next_g) {
         const int g = step_gen(step);
         const int o = step_oct(step);
         const double sx = ldexp(684.412, -o), w = ldexp(1.0, o);
-        const int next_g = step + 1 < 2 * kHigh + 2 ? step_gen(step + 1) : kGenCount;
-        if constexpr (kTabs) s.stage_batch(g, 1, 1, next_g);
-        else s.template stage_y<8, 16>(g, sx, 1, next_g);
+        s.template stage_y<8, 16>(g, sx, 1, next_g);
         if (active) {
-            const auto& yt = [&]() -> const auto& {
-                if constexpr (kTabs) return s.ytabs[step];
-                else return s.ytable();
-            }();
+            const YTable& yt = s.ytable();
             const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
             const Axis ax = axis(X * sx + s.off(0));
             const Axis az = axis(Z * sx + s.off(2));
@@ -403,9 +399,57 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
 #pragma unroll
             for (int k = 0; k < 8; ++k) acc[k] += v[k];
         }
-        // the last step of a group folds its sum into the blend with weight 1 - c (noise2) or c
-        if (step == kHigh - 1 || step >= 2 * kHigh - 1) {
-            const bool n3 = g >= kGenN3;
+    };
+    for (int step = 0; step < 2 * kHigh; ++step) {
+        octave(step, step + 1 < 2 * kHigh ? step_gen(step + 1) : kGenN1);
+        if (step == kHigh - 1) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                stash[(8 + k) * 64 + lane] = acc[k];
+                acc[k] = 0.0;
+            }
+        }
+    }
+    bool open = active;
+    {
+        double s2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            d9[k] = stash[k * 64 + lane];
+            s2[k] = stash[(8 + k) * 64 + lane];
+        }
+        open = open && stage1a_open_hull(d9, s2, acc, kSplitBoundCoarse);
+    }
+    if (!__any(open)) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d12[k] = 0.0;
+        return false;
+    }
+    // noise1: the blend factors, and the noise2 / noise3 sums blended with them (as a fold of the
+    // noise2 group with weight 1 - c, then of the noise3 group with weight c, would form it)
+    double b[8];
+    {
+        double n1[8];
+        colpar_3d<8, 16>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, n1, 0, step_gen(2 * kHigh));
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double d = n1_blend(n1[k]);
+            const double c = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+            const double s2 = stash[(8 + k) * 64 + lane];
+            b[k] = 0.0 + (s2 - c * s2);
+            b[k] += c * acc[k];
+            acc[k] = 0.0;
+            stash[(8 + k) * 64 + lane] = d;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d9[k] = stash[k * 64 + lane];
+    open = open && stage1a_open(d9, b, kSplitBoundCoarse);
+    if (__any(open)) {
+        // octave kSplitOctave of noise2, then of noise3, each folded in with weight 1 - c or c
+        for (int step = 2 * kHigh; step < 2 * kHigh + 2; ++step) {
+            octave(step, step + 1 < 2 * kHigh + 2 ? step_gen(step + 1) : kGenCount);
+            const bool n3 = step == 2 * kHigh + 1;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const double d = stash[(8 + k) * 64 + lane];
@@ -414,13 +458,10 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
                 acc[k] = 0.0;
             }
         }
+        open = open && stage1a_open(d9, b, kSplitBound);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        d9[k] = stash[k * 64 + lane];
-        d12[k] = stash[(8 + k) * 64 + lane];
-    }
-    if (step == 2 * kHigh + 2) open = open && stage1a_open(d9, b, kSplitBound);
+    for (int k = 0; k < 8; ++k) d12[k] = stash[(8 + k) * 64 + lane];
     return open;
 }
 

@@ -1931,7 +1931,9 @@ void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
 // lo[k] <= 0 < hi[k + 1].  The x axis of an octave is formed once per distinct lattice x, the y
 // axis once per octave (Stage 1a: shared by the seed's columns; Stage 1b: per column, each open
 // column being evaluated alone).  Stage 1a's noise2 / noise3 octaves only feed the bounds and take
-// one FMA per level from their Y cell's corners (octaves3dAt's bound form), as the kernel does.
+// one FMA per level from their Y cell's corners (octaves3dAt's bound form), as the kernel does.  As
+// in the kernel, noise2 / noise3 over octaves 11..15 come before noise1: when the hull of the two
+// sums (any blend factor in [0, 1]) closes every column, the seed ends without noise1.
 // ---------------------------------------------------------------------------
 constexpr int kSplitOctave1 = 10;
 
@@ -2075,11 +2077,11 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         }
     }
     const size_t nc = cols.size();
-    // Stage 1a
+    // Stage 1a: noise4 / noise5 / climate (the height biases), noise2 / noise3 over octaves 11..15,
+    // the hull check; noise1 (the blend factors) only if some column is still open
     t_tally = t1a;
     std::vector<std::array<T, kS2Levels>> a1(nc);
     std::vector<T> a4(nc), a5(nc);
-    octaves3dAt<T>(t.n1, 684.412 / 80.0, 684.412 / 160.0, 0, 8, cols, a1);
     octaves2dAt<T>(t.n4, 1.121, cols, a4);
     octaves2dAt<T>(t.n5, 200.0, cols, a5);
     const T slack = T(1e-6);
@@ -2127,14 +2129,13 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
             T d9 = (i2d<T>(y) - d7) * T(12.0) / d5;
             if (d9 < T(0.0)) d9 = d9 * T(4.0);
             d9s[c][k] = d9;
-            d12s[c][k] = (a1[c][k] / T(10.0) + T(1.0)) / T(2.0);
         }
     }
     // noise2 / noise3, blended: b = (1 - cl) n2 + cl n3 with cl = d12 clamped to [0, 1] (for d12 in
     // [0, 1] the density blend n2/512 + (n3/512 - n2/512) d12 is increasing in both sums, so its
     // bounds are (b -/+ u) / 512; outside it is one sum alone).  Each group's octaves are summed
     // first, then folded in.
-    std::vector<std::array<T, kS2Levels>> b(nc), g(nc);
+    std::vector<std::array<T, kS2Levels>> b(nc), g(nc), g2(nc);
     for (size_t c = 0; c < nc; ++c)
         for (int k = 0; k < kS2Levels; ++k) b[c][k] = T(0.0);
     auto fold = [&](bool n3) {
@@ -2161,14 +2162,52 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         }
         return op;
     };
-    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g, true);
-    fold(false);
+    // the hull of the two sums: for any blend factor in [0, 1] the blend lies between them
+    auto hull_open = [&](size_t c, T u) {
+        T lo[kS2Levels + 1], hi[kS2Levels + 1];
+        for (int k = 0; k < kS2Levels; ++k) {
+            const int y = kS2Y0 + k;
+            const T mn = g2[c][k] < g[c][k] ? g2[c][k] : g[c][k];
+            const T mx = g2[c][k] < g[c][k] ? g[c][k] : g2[c][k];
+            lo[k] = taperT((mn - u) / T(512.0) - d9s[c][k], y) - slack;
+            hi[k] = taperT((mx + u) / T(512.0) - d9s[c][k], y) + slack;
+        }
+        lo[kS2Levels] = hi[kS2Levels] = T(-10.0);
+        bool op = false;
+        for (int k = 0; k < kS2Levels; ++k) {
+            const bool a = lo[k] <= T(0.0);
+            const bool bb = hi[k + 1] > T(0.0);
+            op = op || (a && bb);
+        }
+        return op;
+    };
+    const T coarse = T(2.0 * ((1 << (kSplitOctave1 + 1)) - 1));
+    octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g2, true);
     octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g, true);
-    fold(true);
     bool any = false;
     for (size_t c = 0; c < nc; ++c) {
-        open[c] = bounds_open(c, T(2.0 * ((1 << (kSplitOctave1 + 1)) - 1))) ? 1 : 0;
+        open[c] = hull_open(c, coarse) ? 1 : 0;
         any = any || open[c];
+    }
+    auto blend_factors = [&]() {
+        octaves3dAt<T>(t.n1, 684.412 / 80.0, 684.412 / 160.0, 0, 8, cols, a1);
+        for (size_t c = 0; c < nc; ++c)
+            for (int k = 0; k < kS2Levels; ++k) d12s[c][k] = (a1[c][k] / T(10.0) + T(1.0)) / T(2.0);
+    };
+    const bool with_n1 = any;
+    if (any) {
+        // noise1, the blend factors, and the two groups folded in (noise2 with weight 1 - c, then
+        // noise3 with weight c)
+        blend_factors();
+        std::swap(g, g2);
+        fold(false);
+        std::swap(g, g2);
+        fold(true);
+        any = false;
+        for (size_t c = 0; c < nc; ++c) {
+            if (open[c]) open[c] = bounds_open(c, coarse) ? 1 : 0;
+            any = any || open[c];
+        }
     }
     if (any) {
         octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g, true);
@@ -2208,6 +2247,7 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
     if (lattice) {
         OpTally scratch;
         t_tally = &scratch;
+        if (!with_n1) blend_factors();
         std::vector<std::array<T, kS2Levels>> b2(nc), b3(nc);
         octaves3dAt<T>(t.n2, 684.412, 684.412, 0, 16, cols, b2);
         octaves3dAt<T>(t.n3, 684.412, 684.412, 0, 16, cols, b3);

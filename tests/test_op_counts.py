@@ -77,6 +77,8 @@ def test_counted_stage1_computes_the_oracle_columns_and_gate(oracle, window):
 
 
 def test_bench_stage1a_ops_match_a_fresh_count(oracle):
+    # a quarter of the seeds end Stage 1a before noise1, so a seed's count is one of two levels:
+    # 2,048 seeds give the mean to about 0.3 % (one standard error)
     start, _ = bench.island_workload()
-    a, _, _ = count_ops.mean_stage1(start, 256, threads=8)
-    assert abs(a.sum() / bench.STAGE1A_FP64_OPS - 1.0) < 0.003
+    a, _, _ = count_ops.mean_stage1(start, 2048, threads=8)
+    assert abs(a.sum() / bench.STAGE1A_FP64_OPS - 1.0) < 0.01
