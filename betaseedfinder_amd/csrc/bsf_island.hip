@@ -695,9 +695,15 @@ __device__ __forceinline__ unsigned long long run_fill(unsigned long long s, uns
     return up | __builtin_bitreverse64((((rm + rs) ^ rm) & rm) | rs);
 }
 
-// Whether some candidate start of the chunk whose density columns are in dens ([25][8] doubles,
-// levels 8..15, column i * 5 + l) is not grounded within the chunk.  Every lane of the wave calls it.
-__device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
+// Whether the chunk whose density columns are in dens ([25][8] doubles, levels 8..15, column
+// i * 5 + l) can hold a start of a matching component: some candidate start is not grounded within
+// the chunk, and the stone not grounded within the chunk either touches a face that the chunk
+// shares with another chunk of the window (faces: bit 0 x = 0, bit 1 x = 15, bit 2 z = 0, bit 3
+// z = 15) or has min_blocks blocks.  Otherwise every component of that stone lies inside the chunk
+// (a start's component holds no grounded block, so it leaves the chunk only through such stone)
+// and is smaller than min_blocks: none can match (isMatch, IslandSearchEngine.java:3972-4003).
+// Every lane of the wave calls it.
+__device__ bool chunk_may_match(const double* __restrict__ dens, uint32_t faces, int min_blocks) {
     const int lane = threadIdx.x;
     const int x = lane >> 2, j1 = lane & 3;
     const int i1 = x >> 2, i2 = x & 3;
@@ -755,7 +761,22 @@ __device__ bool chunk_has_floating_start(const double* __restrict__ dens) {
     // stone over non-stone at y >= 68
     const bool sample = (x & 7) == 4 && (j1 & 1) == 1;
     const unsigned long long starts = R[0] & ~(col[0] << 1);
-    return __any(sample && (starts & ~g[0]) != 0);
+    if (!__any(sample && (starts & ~g[0]) != 0)) return false;
+    // the stone not grounded within the chunk: its blocks, and whether it reaches a shared face
+    int count = 0;
+    bool face = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned long long f = R[k] & ~g[k];
+        count += __popcll(f);
+        const bool on_face = ((faces & 1u) && x == 0) || ((faces & 2u) && x == 15) ||
+                             ((faces & 4u) && j1 == 0 && k == 0) || ((faces & 8u) && j1 == 3 && k == 3);
+        face = face || (on_face && f != 0);
+    }
+    if (__any(face)) return true;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off);
+    return count >= min_blocks;
 }
 
 __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineArgs a) {
@@ -810,7 +831,12 @@ __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineAr
                     for (int y = 0; y < 8; ++y) s_dens[k][c * 8 + y] = lattice_density(cs, 8 + y, acc.n1[y], acc.n2[y], acc.n3[y]);
                 }
                 __syncthreads();
-                for (int q = 0; q < nch && !keep; ++q) keep = chunk_has_floating_start(s_dens[q]);
+                for (int q = 0; q < nch && !keep; ++q) {
+                    const int qx = ch[q] / cz_n, qz = ch[q] % cz_n;
+                    const uint32_t faces = (qx > 0 ? 1u : 0u) | (qx < p.chunks_x - 1 ? 2u : 0u) |
+                                           (qz > 0 ? 4u : 0u) | (qz < cz_n - 1 ? 8u : 0u);
+                    keep = chunk_may_match(s_dens[q], faces, p.min_blocks);
+                }
                 // the next pair's first stage barrier orders these reads before its writes
             }
         }

@@ -194,3 +194,20 @@ def test_ground_gate_is_sound_on_wider_windows(oracle):
     res = _gate_and_scan(oracle, range(5000, 5400), kw)
     assert all(g or not h for g, h in res)
     assert sum(h for _, h in res) >= 5 and sum(g for g, _ in res) > sum(h for _, h in res)
+
+
+def test_ground_gate_is_sound_at_the_headline_thresholds(oracle):
+    """With the headline's thresholds the gate also drops a seed whose every floating start lies in
+    chunk stone that is confined to its chunk and smaller than minBlocks; it still never drops a
+    seed with a hit, and it does drop some seeds that the loosest thresholds keep."""
+    import bench
+    start, min_blocks = bench.island_workload()
+    win = dict(chunk_x0=-2, chunk_z0=-2, chunks_x=4, chunks_z=4)
+    seeds = [s for s in range(start, start + 2400)
+             if oracle.island_has_candidate(s, oracle.island_params(**win))]
+    head = _gate_and_scan(oracle, seeds, dict(win, min_blocks=min_blocks))
+    assert all(g or not h for g, h in head)
+    assert sum(h for _, h in head) >= 1
+    po = oracle.island_params(**win, **LOOSE)
+    loose = [oracle.island_ground_gate(s, po) for s in seeds]
+    assert sum(loose) > sum(g for g, _ in head) > 0
