@@ -353,9 +353,9 @@ __device__ __forceinline__ bool stage1a_open_hull(const double* d9, const double
 // order), so they are accumulated in the order evaluated, and each octave's level values take one
 // FMA per level from its Y cell's corners (perlin3_column_bound): the rounding differs from
 // Java's sums by a few ulps of values below 2^17, far inside the bounds' slack.
-// stash: the wave's LDS area of 16 x 64 doubles (lane-minor): d9 in rows 0..7; the noise2 sums in
-// rows 8..15 while noise3 and noise1 are evaluated, then d12 there, so that they hold no
-// registers meanwhile.
+// stash: the wave's LDS area of 24 x 64 doubles (lane-minor): d9 in rows 0..7; the noise2 sums in
+// rows 8..15 while noise3 and noise1 are evaluated, then d12 there; the noise3 sums in rows
+// 16..23 while noise1 is evaluated; so that they hold no registers meanwhile.
 template <class S>
 __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i, int l, bool active, double* d9,
                                                double* d12, double* stash) {
@@ -427,6 +427,11 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
     }
     // noise1: the blend factors, and the noise2 / noise3 sums blended with them (as a fold of the
     // noise2 group with weight 1 - c, then of the noise3 group with weight c, would form it)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        stash[(16 + k) * 64 + lane] = acc[k];
+        acc[k] = 0.0;
+    }
     double b[8];
     {
         double n1[8];
@@ -435,10 +440,9 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
         for (int k = 0; k < 8; ++k) {
             const double d = n1_blend(n1[k]);
             const double c = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
-            const double s2 = stash[(8 + k) * 64 + lane];
+            const double s2 = stash[(8 + k) * 64 + lane], s3 = stash[(16 + k) * 64 + lane];
             b[k] = 0.0 + (s2 - c * s2);
-            b[k] += c * acc[k];
-            acc[k] = 0.0;
+            b[k] += c * s3;
             stash[(8 + k) * 64 + lane] = d;
         }
     }
@@ -499,7 +503,7 @@ template <int KIND>
 __global__ __launch_bounds__(64, 2) void pass_stage1_kernel(EngineArgs a) {
     __shared__ GenStage st;
     __shared__ GenStageMulti stm;
-    __shared__ double s_stash[16 * 64];
+    __shared__ double s_stash[24 * 64];
     __shared__ unsigned long long s_item;
     const int tid = threadIdx.x;
     const int nseg = a.desc->nseg[KIND];
