@@ -699,13 +699,45 @@ __device__ __forceinline__ unsigned long long run_fill(unsigned long long s, uns
     return up | __builtin_bitreverse64((((rm + rs) ^ rm) & rm) | rs);
 }
 
+// Grows g (each lane's four block columns as 64-bit y masks, as chunk_may_match lays them out) to
+// its closure inside R: 6-neighbour connectivity within the chunk, the runs of a column filled in
+// one step (run_fill), x / z neighbours exchanged by lane shuffles, until no lane changes.
+__device__ __forceinline__ void grow_in_chunk(unsigned long long g[4], const unsigned long long R[4]) {
+    const int lane = threadIdx.x;
+    const int x = lane >> 2, j1 = lane & 3;
+    for (;;) {
+        unsigned long long xm[4], xp[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xm[k] = __shfl(g[k], (lane + 60) & 63);
+            xp[k] = __shfl(g[k], (lane + 4) & 63);
+            if (x == 0) xm[k] = 0;
+            if (x == 15) xp[k] = 0;
+        }
+        unsigned long long zl = __shfl(g[3], (lane + 63) & 63), zr = __shfl(g[0], (lane + 1) & 63);
+        if (j1 == 0) zl = 0;
+        if (j1 == 3) zr = 0;
+        bool grew = false;
+        unsigned long long n[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long nb = g[k] | xm[k] | xp[k] | (k > 0 ? g[k - 1] : zl) | (k < 3 ? g[k + 1] : zr);
+            n[k] = run_fill(nb & R[k], R[k]);
+            grew |= n[k] != g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g[k] = n[k];
+        if (!__any(grew)) break;
+    }
+}
+
 // Whether the chunk whose density columns are in dens ([25][8] doubles, levels 8..15, column
 // i * 5 + l) can hold a start of a matching component: some candidate start is not grounded within
-// the chunk, and the stone not grounded within the chunk either touches a face that the chunk
-// shares with another chunk of the window (faces: bit 0 x = 0, bit 1 x = 15, bit 2 z = 0, bit 3
-// z = 15) or has min_blocks blocks.  Otherwise every component of that stone lies inside the chunk
-// (a start's component holds no grounded block, so it leaves the chunk only through such stone)
-// and is smaller than min_blocks: none can match (isMatch, IslandSearchEngine.java:3972-4003).
+// the chunk, and the stone connected to such starts inside the chunk either touches a face that the
+// chunk shares with another chunk of the window (faces: bit 0 x = 0, bit 1 x = 15, bit 2 z = 0,
+// bit 3 z = 15) or has min_blocks blocks.  Otherwise each start's component lies inside the chunk
+// (it holds no grounded block, so it leaves the chunk only through that stone) and is smaller than
+// min_blocks: none can match (isMatch, IslandSearchEngine.java:3972-4003).
 // Every lane of the wave calls it.
 __device__ bool chunk_may_match(const double* __restrict__ dens, uint32_t faces, int min_blocks) {
     const int lane = threadIdx.x;
@@ -737,41 +769,26 @@ __device__ bool chunk_may_match(const double* __restrict__ dens, uint32_t faces,
         R[k] = col[k] & ~0xFULL;
         g[k] = (col[k] & 0x18ULL) == 0x18ULL ? run_fill(0x10ULL, R[k]) : 0ULL;
     }
-    for (;;) {
-        unsigned long long xm[4], xp[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            xm[k] = __shfl(g[k], (lane + 60) & 63);
-            xp[k] = __shfl(g[k], (lane + 4) & 63);
-            if (x == 0) xm[k] = 0;
-            if (x == 15) xp[k] = 0;
-        }
-        unsigned long long zl = __shfl(g[3], (lane + 63) & 63), zr = __shfl(g[0], (lane + 1) & 63);
-        if (j1 == 0) zl = 0;
-        if (j1 == 3) zr = 0;
-        bool grew = false;
-        unsigned long long n[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned long long nb = g[k] | xm[k] | xp[k] | (k > 0 ? g[k - 1] : zl) | (k < 3 ? g[k + 1] : zr);
-            n[k] = run_fill(nb & R[k], R[k]);
-            grew |= n[k] != g[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) g[k] = n[k];
-        if (!__any(grew)) break;
-    }
+    grow_in_chunk(g, R);
     // candidate starts: sample columns x, z in {4, 12} (k2 = 0 of lanes with x % 8 == 4, j1 odd),
     // stone over non-stone at y >= 68
     const bool sample = (x & 7) == 4 && (j1 & 1) == 1;
     const unsigned long long starts = R[0] & ~(col[0] << 1);
     if (!__any(sample && (starts & ~g[0]) != 0)) return false;
-    // the stone not grounded within the chunk: its blocks, and whether it reaches a shared face
+    // the not-grounded stone connected to the floating starts: its blocks, and whether it reaches
+    // a shared face
+    unsigned long long F[4], c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        F[k] = R[k] & ~g[k];
+        c[k] = k == 0 && sample ? starts & F[0] : 0ULL;
+    }
+    grow_in_chunk(c, F);
     int count = 0;
     bool face = false;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const unsigned long long f = R[k] & ~g[k];
+        const unsigned long long f = c[k];
         count += __popcll(f);
         const bool on_face = ((faces & 1u) && x == 0) || ((faces & 2u) && x == 15) ||
                              ((faces & 4u) && j1 == 0 && k == 0) || ((faces & 8u) && j1 == 3 && k == 3);

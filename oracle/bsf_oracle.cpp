@@ -758,14 +758,15 @@ int islandHasCandidate(int64_t seed, const bsfo_island_params& p) {
 // The ground gate's predicate: 1 iff some chunk of the window has a candidate start (sample
 // column, y >= 68 over non-stone; :2909-2925, :4005-4007) in its stone that is not grounded within
 // the chunk -- not 6-connected, above y = 67 and inside the chunk, to a block at y = 68 over stone
-// at y = 67 (floodFillFlat's touchesBottom, IslandSearchEngine.java:4170-4175) -- and that stone
-// either touches a face the chunk shares with another chunk of the window or holds min_blocks
-// blocks (else each of its components lies inside the chunk and is too small for isMatch,
-// :3972-4003).  Breadth-first from the contacts, per chunk.
+// at y = 67 (floodFillFlat's touchesBottom, IslandSearchEngine.java:4170-4175) -- and the stone
+// connected to such starts inside the chunk either touches a face the chunk shares with another
+// chunk of the window or holds min_blocks blocks (else each start's component lies inside the
+// chunk and is too small for isMatch, :3972-4003).  Breadth-first from the contacts, then from the
+// floating starts, per chunk.
 int islandGroundGate(int64_t seed, const bsfo_island_params& p) {
     Terrain t(seed);
     std::vector<double> lattice(425);
-    std::vector<uint8_t> blocks(32768), grounded(32768);
+    std::vector<uint8_t> blocks(32768), grounded(32768), reached(32768);
     std::vector<int> queue;
     auto at = [](int x, int y, int z) { return (x * 16 + z) * 128 + y; };
     for (int cx = p.chunk_x0; cx < p.chunk_x0 + p.chunks_x; ++cx) {
@@ -795,24 +796,39 @@ int islandGroundGate(int64_t seed, const bsfo_island_params& p) {
                     }
                 }
             }
-            bool start = false;
+            // the floating starts, and the stone connected to them that is not grounded
+            std::fill(reached.begin(), reached.end(), 0);
+            queue.clear();
             for (int sx : {4, 12})
                 for (int sz : {4, 12})
-                    for (int sy = kIslandFloorY; sy < 128; ++sy)
-                        if (blocks[at(sx, sy, sz)] && !blocks[at(sx, sy - 1, sz)] && !grounded[at(sx, sy, sz)]) start = true;
-            if (!start) continue;
+                    for (int sy = kIslandFloorY; sy < 128; ++sy) {
+                        const int i = at(sx, sy, sz);
+                        if (blocks[i] && !blocks[at(sx, sy - 1, sz)] && !grounded[i] && !reached[i]) {
+                            reached[i] = 1;
+                            queue.push_back(i);
+                        }
+                    }
+            if (queue.empty()) continue;
             const bool lo_x = cx > p.chunk_x0, hi_x = cx < p.chunk_x0 + p.chunks_x - 1;
             const bool lo_z = cz > p.chunk_z0, hi_z = cz < p.chunk_z0 + p.chunks_z - 1;
-            int count = 0;
             bool face = false;
-            for (int x = 0; x < 16; ++x)
-                for (int z = 0; z < 16; ++z)
-                    for (int y = kIslandFloorY; y < 128; ++y)
-                        if (blocks[at(x, y, z)] && !grounded[at(x, y, z)]) {
-                            ++count;
-                            face = face || (x == 0 && lo_x) || (x == 15 && hi_x) || (z == 0 && lo_z) || (z == 15 && hi_z);
-                        }
-            if (face || count >= p.min_blocks) return 1;
+            for (size_t head = 0; head < queue.size(); ++head) {
+                const int i = queue[head];
+                const int x = i / (16 * 128), z = (i / 128) % 16, y = i % 128;
+                face = face || (x == 0 && lo_x) || (x == 15 && hi_x) || (z == 0 && lo_z) || (z == 15 && hi_z);
+                const int nb[6][3] = {{x - 1, y, z}, {x + 1, y, z}, {x, y - 1, z},
+                                      {x, y + 1, z}, {x, y, z - 1}, {x, y, z + 1}};
+                for (const auto& n : nb) {
+                    if (n[0] < 0 || n[0] > 15 || n[2] < 0 || n[2] > 15) continue;
+                    if (n[1] < kIslandFloorY || n[1] > 127) continue;
+                    const int j = at(n[0], n[1], n[2]);
+                    if (blocks[j] && !grounded[j] && !reached[j]) {
+                        reached[j] = 1;
+                        queue.push_back(j);
+                    }
+                }
+            }
+            if (face || static_cast<int>(queue.size()) >= p.min_blocks) return 1;
         }
     }
     return 0;
