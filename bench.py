@@ -67,17 +67,18 @@ def island_workload():
 
 
 # Algorithmic FP64 operations of island Stage 1a per seed, 4x4 window: counted by the
-# instrumented oracle (bsfo_stage1_op_count: the 64 sample columns' noise4/5, climate, column
-# shape and per-level bias; noise2 / noise3 octaves 11..15 (one FMA per level from each Y cell's
-# corners, as the kernel forms these bound-only values) and the hull bound; for the seeds that it
-# leaves open (76 %), noise1, the blend, the blended bound and, where a column is still open
-# (41 % of seeds), octave 10 and the bounds again; run with a counting number type; its plain run
-# reproduces the oracle's densities and Stage-1 decisions), mean over the first 2^16 seeds of the
-# range: 155,216 add/sub + 87,709 mul + 4,026 div + 12,495 compare + 19,411 int<->double
-# conversions (an FMA counts as a mul and an add).  Stage 1b adds 12,537 per open column at 0.914
-# open columns per seed (python oracle/count_ops.py --seeds 65536; tests/test_op_counts.py checks
-# the Stage-1a figure against a fresh count).
-STAGE1A_FP64_OPS = 278858
+# instrumented oracle (bsfo_stage1_op_count: the 64 sample columns' noise5, then -- unless every
+# column takes the column shape's low branch (52 % of seeds), where they cannot matter -- noise4
+# and the climate; column shape and per-level bias; noise2 / noise3 octaves 11..15 (one FMA per
+# level from each Y cell's corners, as the kernel forms these bound-only values) and the hull
+# bound; for the seeds that it leaves open (73 %), noise1, the blend, the blended bound and, where
+# a column is still open (34 % of seeds), octave 10 and the bounds again; run with a counting
+# number type; its plain run reproduces the oracle's densities and Stage-1 decisions), mean over
+# the first 2^16 seeds of the range: 134,818 add/sub + 71,022 mul + 3,476 div + 9,613 compare +
+# 14,649 int<->double conversions (an FMA counts as a mul and an add).  Stage 1b adds 12,538 per
+# open column at 0.723 open columns per seed (python oracle/count_ops.py --seeds 65536;
+# tests/test_op_counts.py checks the Stage-1a figure against a fresh count).
+STAGE1A_FP64_OPS = 233577
 
 
 # Algorithmic FP64 operations of island Stage 2 per survivor, 4x4 window: counted by the

@@ -267,14 +267,15 @@ __device__ __forceinline__ void colpar_3d_n(const S& s, int g0, int octaves, dou
 
 // One 2D octave group (noise4 / noise5) at one lattice column.  Octave o samples at scale
 // sc * 2^-o and carries weight 2^o (Java's 1 / amp), both exact power-of-two scalings.
+// next_g (optional, one-world stagers): the generator to prefetch after the group's last stage.
 template <class S>
 __device__ __forceinline__ double colpar_2d(const S& s, int g0, int octaves, double sc, double X, double Z,
-                                            bool active) {
+                                            bool active, int next_g = -1) {
     double acc = 0.0;
     if constexpr (S::kUniform) {
         for (int o0 = 0; o0 < octaves; o0 += kStageBatch) {
             const int nb = octaves - o0 < kStageBatch ? octaves - o0 : kStageBatch;
-            s.stage_batch(g0 + o0, nb);
+            s.stage_batch(g0 + o0, nb, kStageBatch, o0 + nb < octaves ? -1 : next_g);
             if (active) {
 #pragma unroll
                 for (int q = 0; q < kStageBatch; ++q) {

@@ -863,6 +863,28 @@ __device__ __forceinline__ ColumnShape column_shape(double temp, double rain, do
     s.d7 = 17.0 / 2.0 + d6 * 4.0;
     return s;
 }
+// Whether noise5 takes column_shape's low branch (d6 * 3 - 2 < 0), where d5 is set to 0 and so
+// ends at 0.5 whatever the climate and noise4 are.
+__device__ __forceinline__ bool shape_low(double n5) {
+    double d6 = n5 / 8000.0;
+    if (d6 < 0.0) d6 = -d6 * 0.3;
+    return d6 * 3.0 - 2.0 < 0.0;
+}
+// column_shape for a noise5 in the low branch, without the climate and noise4 (the same values).
+__device__ __forceinline__ ColumnShape column_shape_low(double n5) {
+    double d6 = n5 / 8000.0;
+    if (d6 < 0.0) d6 = -d6 * 0.3;
+    d6 = d6 * 3.0 - 2.0;
+    d6 /= 2.0;
+    if (d6 < -1.0) d6 = -1.0;
+    d6 /= 1.4;
+    d6 /= 2.0;
+    d6 = d6 * 17.0 / 16.0;
+    ColumnShape s;
+    s.d5 = 0.5;
+    s.d7 = 17.0 / 2.0 + d6 * 4.0;
+    return s;
+}
 
 // One lattice density (BetaTerrain173.java:792-829) at level y of a 17-level column, in
 // three parts: the height bias d9, the noise1 blend factor d12, and the density from them

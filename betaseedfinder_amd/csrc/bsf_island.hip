@@ -372,11 +372,18 @@ __device__ __forceinline__ bool stage1a_column(const S& s, int cx, int cz, int i
     auto step_oct = [](int step) { return step < 2 * kHigh ? kSplitOctave + 1 + step % kHigh : kSplitOctave; };
     const int lane = threadIdx.x & 63;
     {
-        const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active);
-        const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active);
-        double temp = 0.0, rain = 0.0;
-        colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain, step_gen(0));
-        const ColumnShape cs = column_shape(temp, rain, n4, n5);
+        // noise5 first: when every column of the wave takes the low branch of the column shape
+        // (half of the worlds), noise4 and the climate cannot change it and are not evaluated
+        const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active, kGenN4);
+        ColumnShape cs;
+        if (__all(!active || shape_low(n5))) {
+            cs = column_shape_low(n5);
+        } else {
+            const double n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active, kGenTemp);
+            double temp = 0.0, rain = 0.0;
+            colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &temp, &rain, step_gen(0));
+            cs = column_shape(temp, rain, n4, n5);
+        }
 #pragma unroll
         for (int k = 0; k < 8; ++k) stash[k * 64 + lane] = level_bias(cs, 8 + k);
     }
@@ -841,13 +848,20 @@ __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineAr
                 const int i = active ? c / 5 : 0, l = active ? c % 5 : 0;
                 const double BX = static_cast<double>(cx * 16 + 3 * i + 1);
                 const double BZ = static_cast<double>(cz * 16 + 3 * l + 1);
-                double t = 0.0, r = 0.0;
-                colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &t, &r);
+                const double X = static_cast<double>(cx * 4 + i), Z = static_cast<double>(cz * 4 + l);
+                // noise5 first; noise4 and the climate only when some column leaves the shape's
+                // low branch (which sets d5 whatever they are)
+                const double n5 = colpar_2d(s, kGenN5, 16, 200.0, X, Z, active, kGenN4);
+                const bool low = __all(!active || shape_low(n5));
+                double t = 0.0, r = 0.0, n4 = 0.0;
+                if (!low) {
+                    n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active, kGenTemp);
+                    colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &t, &r, kGenN2);
+                }
                 ColAcc<8, 16> acc;
-                colpar_noise<8, 16, true>(s, static_cast<double>(cx * 4 + i), static_cast<double>(cz * 4 + l), active,
-                                          acc);
+                colpar_noise<8, 16, false>(s, X, Z, active, acc);
                 if (active) {
-                    const ColumnShape cs = column_shape(t, r, acc.n4, acc.n5);
+                    const ColumnShape cs = low ? column_shape_low(n5) : column_shape(t, r, n4, n5);
 #pragma unroll
                     for (int y = 0; y < 8; ++y) s_dens[k][c * 8 + y] = lattice_density(cs, 8 + y, acc.n1[y], acc.n2[y], acc.n3[y]);
                 }

@@ -109,16 +109,23 @@ enum {
 // Stage 1 in two steps for one-world waves.  Stage 1a sums noise2 / noise3 over octaves
 // kSplitOctave..15 only (few Y cells each) plus noise1, noise4, noise5 and the climate in
 // full; octaves 0..kSplitOctave-1 of noise2 and noise3 are bounded.  Each octave is a 3D
-// Perlin value (|value| <= 2: a trilinear blend of gradients +/-u +/-v with |u|, |v| <= 1)
-// times its weight 2^o, so the two sums are within 2 * (2^kSplitOctave - 1) of their
-// low-frequency parts.  A column whose density bounds rule out an upward air-to-stone
+// Perlin value times its weight 2^o, and |value| <= kOctaveBound: the value is a blend of the
+// eight corner gradients with the fade weights w_c >= 0 (sum 1), and a gradient is +/-u +/-v
+// for two different corner offsets u, v, so |value| <= sum_c w_c top2(|dx|, |dy|, |dz|), the
+// two largest offsets of corner c.  The y offsets (the cell start's fraction and that minus 1,
+// Beta's Y-cell quirk) are at most 1, and top2 grows with each argument, so |value| <=
+// 1 + sum_{x,z corners} w_x w_z max(|dx|, |dz|), whose maximum over the x and z fractions is
+// 0.53528 (at fx = 0.349, fz = 0.5; over a 4001^2 grid, which with the function's slope below
+// 6 bounds it by 0.5361).  So the two sums are within kOctaveBound (2^kSplitOctave - 1) of
+// their low-frequency parts.  A column whose density bounds rule out an upward air-to-stone
 // crossing is closed; the others ("open" columns) are evaluated exactly by Stage 1b.
 // Octave kSplitOctave itself (the most Y cells of those summed) is evaluated only when some
 // column of the seed is still open with the coarser bound of octaves 0..kSplitOctave left out
-// (41 % of the seeds of the 4x4 headline window).
+// (34 % of the seeds of the 4x4 headline window).
 constexpr int kSplitOctave = 10;
-constexpr double kSplitBound = 2.0 * ((1 << kSplitOctave) - 1);
-constexpr double kSplitBoundCoarse = 2.0 * ((1 << (kSplitOctave + 1)) - 1);
+constexpr double kOctaveBound = 1.54;
+constexpr double kSplitBound = kOctaveBound * ((1 << kSplitOctave) - 1);
+constexpr double kSplitBoundCoarse = kOctaveBound * ((1 << (kSplitOctave + 1)) - 1);
 constexpr double kSplitSlack = 1e-6;   // rounding of the bounds and of the y-chain, far above either
 
 // One open column: where it is and the exact per-level parts that Stage 1a computed.

@@ -1964,6 +1964,12 @@ void stage2Window(const Terrain& t, const bsfo_island_params& p, T* out) {
 // sums (any blend factor in [0, 1]) closes every column, the seed ends without noise1.
 // ---------------------------------------------------------------------------
 constexpr int kSplitOctave1 = 10;
+// |3D Perlin octave value| <= 1.54 per unit weight: a blend of the corner gradients (+/-u +/-v,
+// two different corner offsets) with nonnegative fade weights, the y offsets at most 1 (the Y-cell
+// quirk's stale fraction included), so |value| <= 1 + max over (fx, fz) of
+// sum_{x,z corners} w_x w_z max(|dx|, |dz|) = 1 + 0.5353 (tests/test_oracle_pinning.py checks the
+// maximum and samples of the octave values).
+constexpr double kOctaveBound1 = 1.54;
 
 template <class T> T fmaT(T a, T b, T c);
 template <> inline double fmaT<double>(double a, double b, double c) { return std::fma(a, b, c); }
@@ -2110,11 +2116,34 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
     t_tally = t1a;
     std::vector<std::array<T, kS2Levels>> a1(nc);
     std::vector<T> a4(nc), a5(nc);
-    octaves2dAt<T>(t.n4, 1.121, cols, a4);
+    // noise5 first: when every column takes the low branch of the column shape (d6 * 3 - 2 < 0,
+    // where d5 is set to 0 whatever noise4 and the climate are), those are not evaluated
     octaves2dAt<T>(t.n5, 200.0, cols, a5);
+    auto shape_d6 = [](T n5) {
+        T d6 = n5 / T(8000.0);
+        if (d6 < T(0.0)) d6 = -d6 * T(0.3);
+        return d6 * T(3.0) - T(2.0);
+    };
+    bool all_low = true;
+    for (size_t c = 0; c < nc; ++c) all_low = all_low && shape_d6(a5[c]) < T(0.0);
     const T slack = T(1e-6);
     std::vector<std::array<T, kS2Levels>> d9s(nc), d12s(nc);
+    if (!all_low) octaves2dAt<T>(t.n4, 1.121, cols, a4);
     for (size_t c = 0; c < nc; ++c) {
+        if (all_low) {
+            T d6 = shape_d6(a5[c]) / T(2.0);
+            if (d6 < T(-1.0)) d6 = T(-1.0);
+            d6 = d6 / T(1.4);
+            d6 = d6 / T(2.0);
+            d6 = d6 * T(17.0) / T(16.0);
+            const T d7 = T(17.0) / T(2.0) + d6 * T(4.0);
+            for (int k = 0; k < kS2Levels; ++k) {
+                T d9 = (i2d<T>(kS2Y0 + k) - d7) * T(12.0) / T(0.5);
+                if (d9 < T(0.0)) d9 = d9 * T(4.0);
+                d9s[c][k] = d9;
+            }
+            continue;
+        }
         const T tr = stage2ClimateGroup<T>(t.climate.e, 0.02500000037252903, 0.25, blocks[c].first, blocks[c].second);
         const T rr = stage2ClimateGroup<T>(t.climate.f, 0.05000000074505806, 0.3333333333333333, blocks[c].first,
                                            blocks[c].second);
@@ -2209,7 +2238,7 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         }
         return op;
     };
-    const T coarse = T(2.0 * ((1 << (kSplitOctave1 + 1)) - 1));
+    const T coarse = T(kOctaveBound1 * ((1 << (kSplitOctave1 + 1)) - 1));
     octaves3dAt<T>(t.n2, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g2, true);
     octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1 + 1, 16, cols, g, true);
     bool any = false;
@@ -2243,7 +2272,7 @@ void stage1Seed(const Terrain& t, const bsfo_island_params& p, OpTally* t1a, OpT
         octaves3dAt<T>(t.n3, 684.412, 684.412, kSplitOctave1, kSplitOctave1 + 1, cols, g, true);
         fold(true);
         for (size_t c = 0; c < nc; ++c)
-            if (open[c]) open[c] = bounds_open(c, T(2.0 * ((1 << kSplitOctave1) - 1))) ? 1 : 0;
+            if (open[c]) open[c] = bounds_open(c, T(kOctaveBound1 * ((1 << kSplitOctave1) - 1))) ? 1 : 0;
     }
     // Stage 1b at the open columns
     t_tally = t1b;

@@ -211,3 +211,40 @@ def test_ground_gate_is_sound_at_the_headline_thresholds(oracle):
     po = oracle.island_params(**win, **LOOSE)
     loose = [oracle.island_ground_gate(s, po) for s in seeds]
     assert sum(loose) > sum(g for g, _ in head) > 0
+
+
+def test_octave_bound_of_stage1_bounds(oracle):
+    """Stage 1 closes columns with the bound |3D octave value| <= 1.54 per unit weight: every
+    corner gradient is +/-u +/-v of two different corner offsets, the fade weights are >= 0 with
+    sum 1, and the y offsets are at most 1 (the Y-cell quirk's stale fraction included), so
+    |value| <= 1 + max over (fx, fz) of sum_{x,z corners} w_x w_z max(|dx|, |dz|).  The maximum
+    is taken on a grid, widened by the function's slope (below 2.875 per axis) times the half
+    spacing; then octave values of the legacy bulk path (Y-cell quirk included) are sampled."""
+    fade = lambda f: f * f * f * (f * (f * 6 - 15) + 10)
+    n = 2001
+    g = np.linspace(0.0, 1.0, n)
+    fx, fz = g[:, None], g[None, :]
+    sx, sz = fade(fx), fade(fz)
+    m = np.zeros((n, n))
+    for ix in (0, 1):
+        for iz in (0, 1):
+            w = (sx if ix else 1 - sx) * (sz if iz else 1 - sz)
+            m += w * np.maximum(np.abs(fx - ix), np.abs(fz - iz))
+    bound = 1.0 + m.max() + (0.5 / (n - 1)) * 2.875 * 2
+    assert 1.535 < bound <= 1.54
+    # sampled octave values of the generators of a few worlds, over y steps that share cells
+    rng = np.random.default_rng(7)
+    perm = (ctypes.c_uint8 * 256)()
+    abc = (ctypes.c_double * 3)()
+    out = (ctypes.c_double * 17)()
+    top = 0.0
+    for seed in range(20):
+        for gi in (0, 5, 16, 21, 32):
+            oracle.lib().bsfo_perlin_state(ctypes.c_int64(1000003 * seed + 17), gi, perm, abc)
+            for _ in range(200):
+                x, z = rng.uniform(-300, 300, 2)
+                sy = float(rng.choice([0.05, 0.3, 0.7, 1.3, 4.3]))
+                oracle.lib().bsfo_perlin3_column(perm, abc, ctypes.c_double(x), ctypes.c_double(sy),
+                                                 ctypes.c_double(z), 17, out)
+                top = max(top, max(abs(v) for v in out))
+    assert 0.7 < top <= 1.54
