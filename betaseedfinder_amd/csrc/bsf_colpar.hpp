@@ -244,6 +244,31 @@ __device__ __forceinline__ void colpar_3d(const S& s, int g0, int octaves, doubl
     }
 }
 
+// colpar_3d's group-table form for two lanes per column: over the eight levels [Y0, Y0 + 8) of
+// the group's tables, this lane takes levels [Y0 + 4h, Y0 + 4h + 4).
+template <int Y0, class S>
+__device__ __forceinline__ void colpar_3d_half(const S& s, int g0, int octaves, double s_xz, double s_y, double X,
+                                               double Z, bool active, int h, double* acc) {
+    static_assert(S::kUniform && S::kYTabLevels == 8, "eight-level group tables");
+    __syncthreads();
+    build_ytabs<8>(s.ytabs, octaves, s_y, Y0, [&](int q) { return q; },
+                   [&](int q) { return s.offs[(g0 + q) * 3 + 1]; });
+    for (int o = 0; o < octaves; ++o) {
+        const double sx = ldexp(s_xz, -o), w = ldexp(1.0, o);
+        s.stage_batch(g0 + o, 1, o + 1 < octaves ? 1 : kStageBatch);
+        if (active) {
+            const auto& yt = s.ytabs[o];
+            const uint32_t fresh = __builtin_amdgcn_readfirstlane(yt.fresh);
+            const Axis ax = axis(X * sx + s.off(0));
+            const Axis az = axis(Z * sx + s.off(2));
+            double v[4];
+            perlin3_column_half<S::kGradForm>(s.perm(), ax, az, yt, fresh, h, w, v);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = o == 0 ? 0.0 + v[k] : acc[k] + v[k];
+        }
+    }
+}
+
 // colpar_3d for one world per workgroup with the first level y0 a run-time value (levels
 // [y0, y0 + NY)), so that a kernel can walk level blocks in a loop.
 template <int NY, class S>

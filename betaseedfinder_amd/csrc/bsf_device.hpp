@@ -742,6 +742,62 @@ __device__ __forceinline__ void perlin3_column_table(Perm p, const Axis& ax, con
     }
 }
 
+// perlin3_table_cells over four of an eight-level table's levels, [4h, 4h + 4) with h per lane
+// (two lanes share a column and split its levels).  The table holds each level's cell-start
+// values, so the first level of the upper half forms its cell's corners even when the cell
+// starts below it: the same corners, the same values.  fresh_h (this lane's four bits, bit 0
+// set) differs between the halves, so the cell branch is per lane.
+template <int FORM, class Perm, class YT>
+__device__ __forceinline__ void perlin3_half_cells(Perm p, const Axis& ax, const Axis& az, const YT& yt,
+                                                   uint32_t fresh_h, int h, double weight, double* out) {
+    const int xp0 = p.get(ax.i);
+    const int xp1 = p.get(ax.i + 1);
+    const int b = 4 * h;
+    double q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if ((fresh_h >> k) & 1u) {
+            const int yi = yt.yi[b + k];
+            const double yf = yt.cell_f[b + k];
+            const double yfm1 = yt.cell_fm1[b + k];
+            const int p0 = xp0 + yi;
+            const int p00 = p.get(p0) + az.i;
+            const int p01 = p.get(p0 + 1) + az.i;
+            const int p1 = xp1 + yi;
+            const int p10 = p.get(p1) + az.i;
+            const int p11 = p.get(p1 + 1) + az.i;
+            q00 = lerp(ax.fd, corner_grad<FORM>(p, nullptr, p00, ax.f, yf, az.f),
+                       corner_grad<FORM>(p, nullptr, p10, ax.fm1, yf, az.f));
+            q01 = lerp(ax.fd, corner_grad<FORM>(p, nullptr, p01, ax.f, yfm1, az.f),
+                       corner_grad<FORM>(p, nullptr, p11, ax.fm1, yfm1, az.f));
+            q10 = lerp(ax.fd, corner_grad<FORM>(p, nullptr, p00 + 1, ax.f, yf, az.fm1),
+                       corner_grad<FORM>(p, nullptr, p10 + 1, ax.fm1, yf, az.fm1));
+            q11 = lerp(ax.fd, corner_grad<FORM>(p, nullptr, p01 + 1, ax.f, yfm1, az.fm1),
+                       corner_grad<FORM>(p, nullptr, p11 + 1, ax.fm1, yfm1, az.fm1));
+        }
+        const double fd = yt.fd[b + k];
+        const double r0 = lerp(fd, q00, q01);
+        const double r1 = lerp(fd, q10, q11);
+        out[k] = lerp(az.fd, r0, r1) * weight;
+    }
+}
+// perlin3_column_table's dispatch for the half form (FORM kGradSelect or kGradBits); fresh is the
+// table's uniform eight-level mask.
+template <int FORM, class Perm, class YT>
+__device__ __forceinline__ void perlin3_column_half(Perm p, const Axis& ax, const Axis& az, const YT& yt,
+                                                    uint32_t fresh, int h, double weight, double* out) {
+    static_assert(FORM != kGradTable, "the half form has no coefficient table");
+    const uint32_t fresh_h = ((fresh >> (4 * h)) & 15u) | 1u;
+    if constexpr (FORM == kGradSelect) {
+        perlin3_half_cells<kGradSelect>(p, ax, az, yt, fresh_h, h, weight, out);
+    } else {
+        if ((yt.yzero & (fresh | 16u)) == 0u && __all(ax.f != 0.0 && az.f != 0.0))
+            perlin3_half_cells<FORM>(p, ax, az, yt, fresh_h, h, weight, out);
+        else
+            perlin3_half_cells<kGradSelect>(p, ax, az, yt, fresh_h, h, weight, out);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 2D simplex (NoiseGenerator2173.aStrided, :117-172) at already-offset coordinates.
 // ---------------------------------------------------------------------------

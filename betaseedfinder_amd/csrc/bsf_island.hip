@@ -839,11 +839,12 @@ __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineAr
                 ch[1] = mask != 0 ? __ffsll(static_cast<long long>(mask)) - 1 : -1;
                 if (ch[1] >= 0) mask &= mask - 1;
                 const int nch = ch[1] >= 0 ? 2 : 1;
-                // density column c of pair member k: lattice (i, l) = (c / 5, c % 5) of chunk ch[k]
+                // density column c of pair member k: lattice (i, l) = (c / 5, c % 5) of chunk ch[k];
+                // a chunk alone takes two lanes per column, k then naming the column's level half
                 const int k = tid < 25 ? 0 : 1;
                 const int c = tid - 25 * k;
-                const bool active = tid < 25 * nch;
-                const int chk = active ? ch[k] : ch[0];
+                const bool active = tid < 50;
+                const int chk = nch == 2 && active ? ch[k] : ch[0];
                 const int cx = p.chunk_x0 + chk / cz_n, cz = p.chunk_z0 + chk % cz_n;
                 const int i = active ? c / 5 : 0, l = active ? c % 5 : 0;
                 const double BX = static_cast<double>(cx * 16 + 3 * i + 1);
@@ -858,12 +859,27 @@ __global__ __launch_bounds__(kGroundThreads, 2) void pass_ground_kernel(EngineAr
                     n4 = colpar_2d(s, kGenN4, 10, 1.121, X, Z, active, kGenTemp);
                     colpar_climate_all<1>(s, &BX, &BZ, active ? 1 : 0, &t, &r, kGenN2);
                 }
-                ColAcc<8, 16> acc;
-                colpar_noise<8, 16, false>(s, X, Z, active, acc);
-                if (active) {
-                    const ColumnShape cs = low ? column_shape_low(n5) : column_shape(t, r, n4, n5);
+                if (nch == 2) {
+                    ColAcc<8, 16> acc;
+                    colpar_noise<8, 16, false>(s, X, Z, active, acc);
+                    if (active) {
+                        const ColumnShape cs = low ? column_shape_low(n5) : column_shape(t, r, n4, n5);
 #pragma unroll
-                    for (int y = 0; y < 8; ++y) s_dens[k][c * 8 + y] = lattice_density(cs, 8 + y, acc.n1[y], acc.n2[y], acc.n3[y]);
+                        for (int y = 0; y < 8; ++y)
+                            s_dens[k][c * 8 + y] = lattice_density(cs, 8 + y, acc.n1[y], acc.n2[y], acc.n3[y]);
+                    }
+                } else {
+                    // levels 8 + 4k .. 11 + 4k of column c (generateTerrainNoise's group order)
+                    double n1[4], n2[4], n3[4];
+                    colpar_3d_half<8>(s, kGenN2, 16, 684.412, 684.412, X, Z, active, k, n2);
+                    colpar_3d_half<8>(s, kGenN3, 16, 684.412, 684.412, X, Z, active, k, n3);
+                    colpar_3d_half<8>(s, kGenN1, 8, 684.412 / 80.0, 684.412 / 160.0, X, Z, active, k, n1);
+                    if (active) {
+                        const ColumnShape cs = low ? column_shape_low(n5) : column_shape(t, r, n4, n5);
+#pragma unroll
+                        for (int y = 0; y < 4; ++y)
+                            s_dens[0][c * 8 + 4 * k + y] = lattice_density(cs, 8 + 4 * k + y, n1[y], n2[y], n3[y]);
+                    }
                 }
                 __syncthreads();
                 for (int q = 0; q < nch && !keep; ++q) {
