@@ -38,7 +38,7 @@ FP64_NONFUSED_PEAK = 39.3      # 256 CU x 64 lanes x 2.4 GHz single-op issue (no
 # HBM bytes per unit of work of the FP64 kernels: FETCH_SIZE x 1.99975 (the factor by which
 # a 256 MiB device copy under-reads) + WRITE_SIZE, from the rocprofv3 --pmc runs summarised in
 # profiles/island_pipeline_4x4_v1134.md.
-HBM_BYTES_PER_UNIT = {"island_stage1": 21906, "island_stage2": 118909}
+HBM_BYTES_PER_UNIT = {"island_stage1": 22519, "island_stage2": 118962}
 
 
 T_START = time.perf_counter()
